@@ -1,0 +1,193 @@
+"""Throughput benchmark of the MI355X ion-flight engine.
+
+Workload: the ion funnel r-z configuration with hard-sphere He
+(1201 x 201 grid @ 0.1 mm, 100 rings, RF 1 MHz 100 V0-p on alternate
+rings + 10 V/cm DC, He 2 Torr 300 K, sigma 200 A^2, 1,000,000 ions,
+m/z log-uniform 200-2000, dt 0.25 ns), field bases in FP32 with FP64
+positions and time (the engine's FP32-basis mode).  One "step" of the
+benchmark is one window of S integrator steps for every live ion (one
+ig_fly launch), with the ion state and the field resident in HBM.
+
+Prints ONE JSON line.  ``value`` is ion-steps/s (live ion-steps, whole
+job).  Beside it: the plain FP64 kernel's ion-steps/s on the same
+workload, the roofline of the flight kernel (algorithmic gather bytes per
+launch / HIP-event launch time, against 8 TB/s HBM) and the CPU baseline
+(the oracle's C FP64 port on the host cores, timed on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def gather_bytes_per_ion_step(n_sets, value_bytes):
+    """Algorithmic gather bytes of one r-z ion-step of this engine: one
+    bilinear field evaluation per step (the second evaluation of a step is
+    reused as the first of the next, see ig_rz_kernel.h) of 2 components
+    x n_sets sets x 4 corners, plus one metal-mask byte."""
+    return 1 * (2 * n_sets) * 4 * value_bytes + 1
+
+
+def state_bytes_per_ion():
+    """SoA state read + written once per launch (doubles and ints)."""
+    return 2 * (8 * 16 + 4 * 5 + 8 * 4)
+
+
+def run_gpu(cfg, precision, warmup, steps, window, engine):
+    from ion_gym_amd.engine import Flight
+    model = cfg["model"]
+    fp = engine.upload_rz(model.pack(), precision)
+    n = len(cfg["ions"]["x"])
+    ens = engine.ensemble(n)
+    ens.load(cfg["ions"])
+    fl = Flight(cfg["dt"], cfg["t_max"], gas=cfg["gas"], max_steps=window)
+    for _ in range(warmup):
+        engine.fly(fp, fl, ens)
+    engine.sync()
+    s0 = ens.steps()
+    kernel_ms = []
+    engine.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        engine.fly(fp, fl, ens)
+        kernel_ms.append(engine.last_fly_ms())
+    engine.sync()
+    wall = time.perf_counter() - t0
+    s1 = ens.steps()
+    live = ens.live()
+    n_sets = 1 + len(model.ch_kind)
+    out = dict(ion_steps=s1 - s0, wall_s=wall, kernel_ms=kernel_ms,
+               live_end=live, n=n, n_sets=n_sets, field_bytes=fp.nbytes)
+    ens.release()
+    fp.release()
+    return out
+
+
+def cpu_baseline(cfg, window, n_sample, threads):
+    """The oracle (C FP64 port of the reference kernels, OpenMP over ions)
+    on a bounded sample of the same workload and window."""
+    from oracle import oracle as O
+    ions = {k: v[:n_sample] for k, v in cfg["ions"].items()}
+    fl = dict(dt=cfg["dt"], t_max=cfg["t_max"], max_steps=window,
+              rng=O.RNG_PHILOX, gas=cfg["gas"])
+    pack = cfg["model"].pack()
+    t0 = time.perf_counter()
+    r = O.fly_rz(pack, fl, ions, n_threads=threads)
+    el = time.perf_counter() - t0
+    return int(r["steps"].sum()), el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--window", type=int, default=2000,
+                    help="integrator steps per ion per benchmark step")
+    ap.add_argument("--ions", type=int, default=1_000_000)
+    ap.add_argument("--precision", default="f32",
+                    choices=["f32", "f32_store", "f64"])
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the plain FP64 comparison run")
+    ap.add_argument("--cpu-sample", type=int, default=4000)
+    ap.add_argument("--cpu-window", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
+              f"is not supported (ions shard across GPUs with a replicated "
+              f"field pack; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Engine
+    print(f"[bench] building funnel_rz with {args.ions} ions", file=sys.stderr,
+          flush=True)
+    cfg = configs.funnel_rz(n_ions=args.ions, seed=3)
+    eng = Engine(0)
+    print(f"[bench] timing {args.precision}: warmup {args.warmup}, steps "
+          f"{args.steps}, window {args.window}", file=sys.stderr, flush=True)
+    main_run = run_gpu(cfg, args.precision, args.warmup, args.steps,
+                       args.window, eng)
+    value = main_run["ion_steps"] / main_run["wall_s"]
+    ms_per_step = main_run["wall_s"] / args.steps * 1e3
+    fp64 = None
+    if not args.no_fp64 and args.precision != "f64":
+        print("[bench] timing the plain FP64 kernel", file=sys.stderr,
+              flush=True)
+        r64 = run_gpu(cfg, "f64", 1, max(2, args.steps // 4), args.window,
+                      eng)
+        fp64 = r64["ion_steps"] / r64["wall_s"]
+    # roofline of the flight kernel: algorithmic bytes per launch / mean
+    # HIP-event launch time on the engine's stream
+    vb = 8 if args.precision == "f64" else 4
+    per_step = gather_bytes_per_ion_step(main_run["n_sets"], vb)
+    launches = args.steps
+    ion_steps_per_launch = main_run["ion_steps"] / launches
+    alg_bytes = ion_steps_per_launch * per_step + main_run["n"] * \
+        state_bytes_per_ion()
+    mean_ms = float(np.mean(main_run["kernel_ms"]))
+    achieved = alg_bytes / (mean_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(HERE, "profiles",
+                         f"traffic_funnel_rz_{args.precision}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+        print(f"[bench] CPU baseline: {args.cpu_sample} ions x "
+              f"{args.cpu_window} steps on {threads} threads",
+              file=sys.stderr, flush=True)
+        st, el = cpu_baseline(cfg, args.cpu_window, args.cpu_sample, threads)
+        cpu = {"value": st / el, "unit": "ion-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_sample} ions of the same workload x "
+                         f"{args.cpu_window} steps (FP64, HS, Philox), "
+                         f"{el:.1f} s"}
+    line = {
+        "metric": "ion-steps/s at 1M ions (3-D funnel+HS), FP32 vs FP64 "
+                  "plain; p50/p99 latency",
+        "value": value,
+        "unit": "ion-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision != "f64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "funnel_rz (ion funnel r-z + hard-sphere He)",
+                   "ions": main_run["n"], "grid": "1201x201 @ 0.1 mm "
+                   "(unfolded 1201x401)", "window_steps": args.window,
+                   "field_precision": args.precision,
+                   "state_precision": "f64", "dt_ns": cfg["dt"] * 1e3,
+                   "live_ions_end": main_run["live_end"],
+                   "field_bytes": main_run["field_bytes"]},
+        "fp64_plain": fp64,
+        "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "rz_fly_kernel", "kernel_ms": mean_ms,
+                     "alg_bytes_per_launch": alg_bytes,
+                     "bytes_per_ion_step": per_step},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

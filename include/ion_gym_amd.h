@@ -1,0 +1,265 @@
+/*
+ * ion_gym_amd.h -- C ABI of the MI355X ion-flight engine.
+ *
+ * The engine flies ensembles of independent ions through precomputed
+ * per-channel E-field grids on one AMD Instinct MI355X (gfx950).  It is
+ * the batched replacement of ion_gym's per-ion numba flight kernels:
+ *
+ *   route RZ  : ion_gym/physics/tracer_rz.py:113-446  (_fly_rec_full)
+ *               kick-drift-kick leapfrog, bilinear field on the
+ *               mirror-unfolded (axial, radial) half plane, 3-D ion state.
+ *   route XYZ : ion_gym/physics/tracer3d.py:280-575   (_fly3d)
+ *               classic RK4, trilinear field, 3-D state.
+ *
+ * The reference has no FFI: its boundary is the Python call
+ * fly_fn(i) -> (traj, summary) (ion_gym/physics/build_rz.py:675-776,
+ * ion_gym/physics/build_stl3d.py:1181-1396) invoked by
+ * ensemble_driver.run / fly_all / run_threaded
+ * (ion_gym/physics/ensemble_driver.py:381-529).  The Python package
+ * ion_gym_amd binds this header with ctypes (the GIL is released for the
+ * duration of every call) and keeps that Python contract unchanged.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 = ok, < 0 = error class
+ *    (IG_E_*).  ig_last_error() returns the calling thread's last message.
+ *  - Host buffers passed in are caller owned and copied before return.
+ *    Output buffers are caller allocated.  Device memory belongs to the
+ *    engine and is freed by the matching *_release call or by
+ *    ig_engine_destroy.
+ *  - An engine is one device plus one HIP stream.  A handle is used from
+ *    one host thread at a time; several engines may share a device.
+ *  - Units follow the reference kernels: mm, microseconds, amu, V/m for
+ *    r-z channel grids (ion_gym/physics/ionbench.py:92-93), V/mm for 3-D
+ *    channel grids (ion_gym/physics/tracer3d.py:295-300).
+ */
+#ifndef ION_GYM_AMD_H
+#define ION_GYM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IG_ABI_VERSION 1
+
+/* status codes */
+#define IG_OK 0
+#define IG_E_ARG (-1)      /* invalid argument or shape */
+#define IG_E_HIP (-2)      /* HIP runtime error */
+#define IG_E_NOMEM (-3)    /* device allocation failed */
+#define IG_E_STATE (-4)    /* wrong object state (unknown id, busy) */
+#define IG_E_KERNEL (-5)   /* a flight refused inside the kernel */
+
+/* routes */
+#define IG_ROUTE_RZ 0
+#define IG_ROUTE_XYZ 1
+
+/* device storage / arithmetic of the channel grids */
+#define IG_PREC_F64 0       /* FP64 storage, FP64 interpolation */
+#define IG_PREC_F32_STORE 1 /* FP32 storage, FP64 interpolation: the
+                               reference's channel_dtype="float32"
+                               (ion_gym/io/sim_spec.py:1339-1340) */
+#define IG_PREC_F32 2       /* FP32 storage, FP32 interpolation; state,
+                               time and waveform phase stay FP64 */
+
+/* waveform kinds (ion_gym/physics/build_planar.py:1059-1096,
+   ion_gym/physics/tracer3d.py:234-276) */
+#define IG_WAVE_SIN 0
+#define IG_WAVE_COS 1
+#define IG_WAVE_SQUARE 2
+#define IG_WAVE_TAB_HOLD 3
+#define IG_WAVE_TAB_LINEAR 4
+
+/* gas models */
+#define IG_GAS_VACUUM 0
+#define IG_GAS_HS 1  /* ion_gym/physics/collision3d.py:333-374 */
+#define IG_GAS_SDS 2 /* ion_gym/physics/sds.py:134-169, tracer_rz.py:219-272 */
+
+/* fates (ion_gym/physics/ion_envelope.py:100-103) plus the engine's own
+   "still flying" marker for windowed flights */
+#define IG_FATE_FLYING (-2)
+#define IG_FATE_IMPACT 0
+#define IG_FATE_BOX 1
+#define IG_FATE_TIMEOUT 2
+#define IG_FATE_BOUND 3
+#define IG_FATE_TRANSPORTER 4
+#define IG_FATE_PLANE 5
+#define IG_FATE_DETECT 6
+
+/* optional record channels, bit i = the i-th r-z channel of
+   ion_gym/physics/build_rz.py:569-571 in that order */
+#define IG_REC_SPEED (1u << 0)
+#define IG_REC_KE (1u << 1)
+#define IG_REC_KE_X (1u << 2)
+#define IG_REC_KE_Y (1u << 3)
+#define IG_REC_KE_Z (1u << 4)
+#define IG_REC_E_FIELD (1u << 5)
+#define IG_REC_E_AXIAL (1u << 6)
+#define IG_REC_E_RADIAL (1u << 7)
+#define IG_REC_RADIUS (1u << 8)
+#define IG_REC_N_COL (1u << 9)
+#define IG_REC_PATH (1u << 10)
+#define IG_REC_E_AXIAL_TINT (1u << 11)
+#define IG_REC_KE_TINT (1u << 12)
+
+#define IG_MAX_CHANNELS 16
+#define IG_MAX_PLANES 16
+
+typedef struct ig_engine ig_engine;
+typedef int64_t ig_fields_id;
+typedef int64_t ig_ensemble_id;
+
+/* Field pack.  Replaces the arrays the reference passes to every kernel
+ * call: r-z EzA/EuA/EzK/EuK/EzG/EuG + mirrored ele
+ * (ion_gym/physics/build_rz.py:725-738); 3-D EAx..EzK + ele
+ * (ion_gym/physics/tracer3d.py:281-290).
+ *
+ * grids: FP64, C-contiguous, shape (n_sets, n_comp, n0, n1, n2) with
+ *   set 0 the static field, sets 1..n_chan the drive channels and, when
+ *   has_gate, set n_chan+1 the r-z gate field; n_comp = 2 (axial, radial)
+ *   for RZ and 3 (x, y, z) for XYZ; n2 = 1 for RZ.  n1 of an RZ pack is
+ *   the mirror-unfolded radial width 2*nr-1.
+ * metal: uint8 (n0, n1, n2), nonzero = metal node.  RZ uses the
+ *   nearest-node rule (ion_gym/physics/raster2d.py:597-612), XYZ the
+ *   all-eight-corners rule (ion_gym/physics/tracer3d.py:174-189).
+ * label: optional uint8 (n0, n1, n2) electrode labels (0 = vacuum); the
+ *   splat electrode of an impact is label[nearest node of the end point].
+ */
+typedef struct {
+  int32_t route;     /* IG_ROUTE_* */
+  int32_t precision; /* IG_PREC_* device storage/arithmetic */
+  int64_t n0, n1, n2;
+  double h_mm;       /* node pitch */
+  double u0;         /* RZ: radial coordinate of column 0, -(nr-1)*h */
+  int32_t n_chan;    /* K time-dependent channels, <= IG_MAX_CHANNELS */
+  int32_t has_gate;  /* RZ gate set present */
+  double tau_gate;   /* RZ gate time (us), < 0 = no gate */
+  const double *grids;
+  const double *chan_scale; /* optional (n_sets): FP32 storage stores
+                               grid/scale and multiplies back; NULL = 1 */
+  const uint8_t *metal;
+  const uint8_t *label;
+  /* waveforms, one entry per drive channel */
+  const int32_t *ch_kind;
+  const double *ch_om;   /* rad/us */
+  const double *ch_ph;   /* rad */
+  const double *ch_duty;
+  const double *ch_amp;  /* XYZ affine amplitude (r-z bakes it: pass 1) */
+  const double *ch_off;  /* XYZ affine offset (r-z: 0) */
+  const double *tab_t;   /* shared breakpoints (us) */
+  const double *tab_v;
+  const int64_t *tab_off; /* (n_chan+1) slice offsets into tab_t/tab_v */
+  int64_t n_tab;
+} ig_fields_desc;
+
+/* Flight parameters shared by every ion of one ig_fly call.  Replaces the
+ * scalar/array arguments of _fly_rec_full (ion_gym/physics/tracer_rz.py:
+ * 114-123) and _fly3d (ion_gym/physics/tracer3d.py:281-290). */
+typedef struct {
+  double dt;           /* step in the route's clock: RZ us, XYZ s (the
+                          reference's dt_ns*1e-3 and dt_ns*1e-9) */
+  double t_max;        /* flight time limit, same unit, while t < t_max */
+  int64_t max_steps;   /* steps this call may advance each ion; <= 0 =
+                          until fate.  Ions still flying keep fate
+                          IG_FATE_FLYING and resume on the next call. */
+  int32_t rec_every;   /* record cadence (steps), >= 1 */
+  int32_t rec_rows;    /* rows per ion in the record buffer; 0 = none */
+  uint32_t rec_mask;   /* IG_REC_* optional channels (RZ) */
+  int32_t gas_model;   /* IG_GAS_* */
+  double T_k, P_pa, c_star, c_bar, sig1d, m_gas; /* gas scalars
+                          (ion_gym/physics/collision3d.py:207-245) */
+  double flow[3];      /* gas flow mm/us */
+  const double *sds_stats; /* (5, 1002) jump table for IG_GAS_SDS */
+  int32_t bnd_on[6];   /* RZ bounding faces x-,x+,y-,y+,z-,z+ */
+  double bnd_val[6];
+  int32_t n_planes;    /* station / bounding planes */
+  int32_t pl_col[IG_MAX_PLANES];
+  double pl_val[IG_MAX_PLANES];
+  double pl_sgn[IG_MAX_PLANES];
+  double pl_w[IG_MAX_PLANES][4];
+  int32_t pl_kind[IG_MAX_PLANES];
+  uint64_t rng_key;    /* mixed into every ion's Philox key; lets a
+                          serving loop keep requests' streams apart */
+} ig_flight_params;
+
+/* Per-ion inputs (host, SoA, n entries each).  Births are the rows of
+ * generate_births (ion_gym/physics/sim_build.py:82-84); per-ion scalars
+ * are what fly_fn derives (ion_gym/physics/build_rz.py:703-713):
+ * acc = q e/(m amu) for RZ, qm = q e/(m amu) for XYZ (same number). */
+typedef struct {
+  int64_t n;
+  const double *x, *y, *z, *vx, *vy, *vz, *tob; /* mm, mm/us, us */
+  const double *m_amu;
+  const double *acc;     /* C/kg */
+  const double *sigma_m2;
+  const int64_t *seed;   /* per-ion RNG seed (seed_base + i) */
+  const double *sds;     /* optional (n, 4): damping, mfp_mm, V_mm_us,
+                            log_mr (ion_gym/physics/sds.py:428-447) */
+  const int32_t *fate_in; /* optional: a fate preset by the host (birth
+                             splat); such ions are not flown */
+} ig_ions;
+
+/* Per-ion results (host, caller allocated, SoA). */
+typedef struct {
+  double *x, *y, *z, *vx, *vy, *vz; /* end state, mm and mm/us */
+  double *t_us;          /* lab time of the end state (tob + t) */
+  int32_t *fate;         /* IG_FATE_* */
+  int32_t *face;         /* RZ bounding face, XYZ hit plane, -1 none */
+  int32_t *label;        /* splat electrode label (impact), else 0 */
+  int64_t *ncol;         /* hard-sphere collisions */
+  int64_t *steps;        /* integrator steps flown */
+  int32_t *nrec;         /* record rows written (if records) */
+  double *records;       /* optional (n, rec_rows, rec_cols) */
+} ig_results;
+
+/* --- engine ------------------------------------------------------------ */
+const char *ig_last_error(void);
+int ig_abi_version(void);
+int ig_device_count(int *out);
+int ig_engine_create(int device, ig_engine **out);
+int ig_engine_destroy(ig_engine *h);
+int ig_sync(ig_engine *h);
+
+/* --- field packs --------------------------------------------------------
+ * Upload, convert and interleave a field pack into device memory.
+ * Replaces the per-call array arguments listed at ig_fields_desc. */
+int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out);
+int ig_fields_release(ig_engine *h, ig_fields_id id);
+int ig_fields_bytes(ig_engine *h, ig_fields_id id, int64_t *out);
+
+/* --- device-resident ensembles ------------------------------------------
+ * Ion state kept in HBM across calls so that long flights, throughput
+ * windows and the serving loop never round-trip it through the host.
+ * Replaces the per-ion state the reference keeps inside one kernel call
+ * (ion_gym/physics/tracer_rz.py:165-206). */
+int ig_ensemble_create(ig_engine *h, int64_t capacity, int32_t rec_rows,
+                       int32_t rec_cols, ig_ensemble_id *out);
+int ig_ensemble_release(ig_engine *h, ig_ensemble_id id);
+int ig_ensemble_load(ig_engine *h, ig_ensemble_id id, const ig_ions *ions);
+int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *out);
+/* Counts ions whose fate is still IG_FATE_FLYING (synchronous). */
+int ig_ensemble_live(ig_engine *h, ig_ensemble_id id, int64_t *out);
+/* Total integrator steps flown by the ensemble so far (synchronous). */
+int ig_ensemble_steps(ig_engine *h, ig_ensemble_id id, int64_t *out);
+
+/* Advance every live ion of the ensemble (asynchronous on the engine's
+ * stream).  The kernel-level replacement of one _fly_rec_full /
+ * _fly3d call per ion (ion_gym/physics/tracer_rz.py:113,
+ * ion_gym/physics/tracer3d.py:280). */
+int ig_fly(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
+           ig_ensemble_id e);
+
+/* One-shot: load, fly to fate, read.  The batched form of
+ * fly_fn(i) over a list of ions (ion_gym/physics/build_rz.py:675). */
+int ig_fly_batch(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
+                 const ig_ions *ions, ig_results *out);
+
+/* Kernel timing of the last ig_fly on the engine's stream (HIP events
+ * recorded around the launches), in milliseconds. */
+int ig_last_fly_ms(ig_engine *h, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ION_GYM_AMD_H */

@@ -1,0 +1,146 @@
+"""Synthetic workloads: the five flight configurations the engine is
+measured and checked on, built from geometry, a Laplace solve and seeded
+births.
+
+Each builder returns a dict with the field model/pack, the flight
+parameters and the births (passed as data, the reference's
+``distribution: "file"`` route, ion_gym/physics/sim_build.py:89-139).
+
+  einzel_rz   einzel lens r-z, vacuum, FP64 (401 x 101 @ 0.1 mm)
+  quad_3d     3-D quadrupole, RF + DC, vacuum (161 x 161 x 401 @ 0.05 mm)
+  funnel_rz   ion funnel r-z with hard-sphere He (1201 x 201 @ 0.1 mm)
+  drift_rz    drift tube r-z with SDS N2 (801 x 101), many ensembles
+  slim_3d     3-D traveling-wave board, 11 field sets, HS N2
+"""
+import math
+
+import numpy as np
+
+from .physics import collision3d as gas_mod
+from .physics.constants import E_CHG, K_B, KG_AMU
+from .physics.fields_rz import DriveGroup, RZModel, build_field_aware
+from .physics.solver2d import solve_rz
+
+
+def _grid_labels(nz, nr, h, electrodes):
+    """electrodes: list of (label, z0, z1, r0, r1) rectangles (mm) painted
+    onto node centres."""
+    lab = np.zeros((nz, nr), np.int32)
+    z = np.arange(nz) * h
+    r = np.arange(nr) * h
+    eps = 1e-9
+    for L, z0, z1, r0, r1 in electrodes:
+        iz = (z >= z0 - eps) & (z <= z1 + eps)
+        ir = (r >= r0 - eps) & (r <= r1 + eps)
+        lab[np.ix_(iz, ir)] = L
+    return lab
+
+
+def _speed_mm_us(ke_ev, m_amu):
+    return np.sqrt(2.0 * ke_ev * E_CHG / (m_amu * KG_AMU)) * 1e-3
+
+
+def _model_from_potentials(labels, h, A, chans, ch_kind, ch_om, ch_ph,
+                           ch_duty):
+    nz, nr = labels.shape
+    Z = np.arange(nz) * h
+    U = np.arange(nr) * h
+    _, Ue, EzA, EuA = build_field_aware(Z, U, A, labels)
+    EzK, EuK = [], []
+    for B in chans:
+        _, _, a, b = build_field_aware(Z, U, B, labels)
+        EzK.append(a)
+        EuK.append(b)
+    EzK = np.stack(EzK) if EzK else np.zeros((0,) + EzA.shape)
+    EuK = np.stack(EuK) if EuK else np.zeros((0,) + EuA.shape)
+    return RZModel(EzA=EzA, EuA=EuA, EzK=EzK, EuK=EuK,
+                   ch_kind=np.asarray(ch_kind, np.int64),
+                   ch_om=np.asarray(ch_om, np.float64),
+                   ch_ph=np.asarray(ch_ph, np.float64),
+                   ch_duty=np.asarray(ch_duty, np.float64), ele=labels,
+                   u0=float(Ue[0]), mm_per_gu=h)
+
+
+def einzel_rz(n_ions=10000, seed=0, coarse=1):
+    """Einzel lens: three coaxial tubes (inner radius 5 mm, 1 mm wall,
+    10 mm long, 2 mm gaps) at 0 / +600 / 0 V; m/z 100, z = +1; births on
+    the plane x = 1 mm, r uniform in 0-2 mm, KE ~ N(100, 1) eV along +x;
+    DC, so dt = 1 ns; vacuum; fly to fate.  ``coarse`` > 1 divides the
+    grid resolution (tests)."""
+    h = 0.1 * coarse
+    nz, nr = 400 // coarse + 1, 100 // coarse + 1
+    elec = [(1, 4.0, 14.0, 5.0, 6.0), (2, 16.0, 26.0, 5.0, 6.0),
+            (3, 28.0, 38.0, 5.0, 6.0)]
+    lab = _grid_labels(nz, nr, h, elec)
+    (A,) = solve_rz(lab, [{1: 0.0, 2: 600.0, 3: 0.0}], h)
+    model = _model_from_potentials(lab, h, A, [], [], [], [], [])
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(0.0, 2.0, n_ions)
+    th = rng.uniform(0.0, 2 * math.pi, n_ions)
+    ke = rng.normal(100.0, 1.0, n_ions)
+    m = np.full(n_ions, 100.0)
+    v = _speed_mm_us(ke, m)
+    births = dict(x=np.full(n_ions, 1.0), y=r * np.cos(th), z=r * np.sin(th),
+                  vx=v, vy=np.zeros(n_ions), vz=np.zeros(n_ions),
+                  tob=np.zeros(n_ions))
+    q = 1
+    ions = dict(births, m_amu=m, acc=q * E_CHG / (m * KG_AMU),
+                sigma_m2=np.zeros(n_ions),
+                seed=seed + np.arange(n_ions, dtype=np.int64))
+    return dict(name="einzel_rz", route="rz", model=model, ions=ions,
+                dt=1.0 * 1e-3, t_max=50.0, gas=None, charge=q)
+
+
+def funnel_rz(n_ions=1_000_000, seed=3, coarse=1, dt_ns=0.25):
+    """Ion funnel: 100 rings 0.5 mm thick at 1 mm pitch, inner diameter
+    tapering linearly 25 -> 2 mm (3 mm radial depth); RF 1.0 MHz, 100 V
+    zero-to-peak in opposite phase on alternate rings; 10 V/cm axial DC
+    ladder; He at 2 Torr and 300 K, hard-sphere cross section 200 A^2;
+    m/z log-uniform in 200-2000, z = +1; births uniform in a 10 mm disk on
+    the entrance plane x = 8 mm with 300 K Maxwellian velocities; dt =
+    0.25 ns (period/4000, below tau_collision/20 = 0.31 ns)."""
+    h = 0.1 * coarse
+    nz, nr = 1200 // coarse + 1, 200 // coarse + 1
+    elec = []
+    for i in range(100):
+        z0 = 10.0 + i * 1.0
+        rin = 12.5 + (1.0 - 12.5) * i / 99.0
+        elec.append((i + 1, z0, z0 + 0.5, rin, rin + 3.0))
+    lab = _grid_labels(nz, nr, h, elec)
+    dc = {i + 1: 100.0 - (elec[i][1] - 10.0) * 1.0 for i in range(100)}
+    ampA = {i + 1: (100.0 if i % 2 == 0 else 0.0) for i in range(100)}
+    ampB = {i + 1: (100.0 if i % 2 == 1 else 0.0) for i in range(100)}
+    A, BA, BB = solve_rz(lab, [dc, ampA, ampB], h)
+    om = 1.0e6 * 1e-6 * 2 * math.pi
+    model = _model_from_potentials(lab, h, A, [BA, BB], [0, 0], [om, om],
+                                   [0.0, math.pi], [0.5, 0.5])
+    rng = np.random.default_rng(seed)
+    m = np.exp(rng.uniform(math.log(200.0), math.log(2000.0), n_ions))
+    rr = 5.0 * np.sqrt(rng.uniform(0.0, 1.0, n_ions))
+    th = rng.uniform(0.0, 2 * math.pi, n_ions)
+    T = 300.0
+    s1 = np.sqrt(K_B * T / (m * KG_AMU)) * 1e-3
+    v = rng.normal(0.0, 1.0, (3, n_ions)) * s1
+    births = dict(x=np.full(n_ions, 8.0), y=rr * np.cos(th),
+                  z=rr * np.sin(th), vx=v[0], vy=v[1], vz=v[2],
+                  tob=np.zeros(n_ions))
+    q = 1
+    g = gas_mod.gas_kernel_scalars("He", T_k=T, P_torr=2.0)
+    gas = dict(model=1, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+               flow=(0.0, 0.0, 0.0))
+    ions = dict(births, m_amu=m, acc=q * E_CHG / (m * KG_AMU),
+                sigma_m2=np.full(n_ions, 2.0e-18),
+                seed=seed + np.arange(n_ions, dtype=np.int64))
+    return dict(name="funnel_rz", route="rz", model=model, ions=ions,
+                dt=dt_ns * 1e-3, t_max=2000.0, gas=gas, charge=q)
+
+
+def drives_for(model):
+    """DriveGroup view of a composed model's channels (for reports)."""
+    out = []
+    for k in range(len(model.ch_kind)):
+        out.append(DriveGroup(name=f"ch{k}", frequency_hz=model.ch_om[k] /
+                              (2 * math.pi) * 1e6,
+                              phase_deg=math.degrees(model.ch_ph[k])))
+    return out

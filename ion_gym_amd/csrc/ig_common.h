@@ -1,0 +1,118 @@
+// Structures shared by the engine's host code and its gfx950 kernels.
+//
+// Everything a flight kernel reads is passed by value in three kernel
+// arguments (well under the 4 KiB kernarg limit): the field pack
+// (FieldsK), the flight parameters (FlightK) and the device-resident ion
+// state (EnsK).
+#pragma once
+#include <stdint.h>
+
+#include "../../include/ion_gym_amd.h"
+
+namespace ig {
+
+// Field pack on the device.  Channel grids are interleaved node-major: the
+// record of node n holds, for every set s (static, drive channels, gate),
+// its n_comp components, padded to `stride` values so that a record is a
+// whole number of 16-byte units.  One bilinear or trilinear gather then
+// reads a few contiguous records instead of 2(1+K) or 3(1+K) separate
+// arrays.
+struct FieldsK {
+  int32_t route;
+  int32_t prec;
+  int64_t n0, n1, n2;
+  double h, inv_h, u0;
+  int32_t K;        // drive channels
+  int32_t has_gate; // r-z gate set present (set K+1)
+  double tau_gate;
+  int32_t n_sets;   // 1 + K + has_gate
+  int32_t n_comp;   // 2 (r-z) or 3 (3-D)
+  int32_t stride;   // values per node record (>= n_sets*n_comp)
+  const void *grid; // double or float records
+  const uint8_t *metal;
+  const uint8_t *label;
+  double scale[IG_MAX_CHANNELS + 2]; // per-set multiplier (FP32 storage)
+  int32_t kind[IG_MAX_CHANNELS];
+  double om[IG_MAX_CHANNELS], ph[IG_MAX_CHANNELS], duty[IG_MAX_CHANNELS];
+  double amp[IG_MAX_CHANNELS], off[IG_MAX_CHANNELS];
+  const double *tab_t, *tab_v;
+  int64_t tab_off[IG_MAX_CHANNELS + 1];
+};
+
+struct FlightK {
+  double dt, t_max;
+  int64_t max_steps;
+  int32_t rec_every, rec_rows, rec_cols;
+  uint32_t rec_mask;
+  int32_t gas;
+  double T_k, P_pa, c_star, c_bar, sig1d, m_gas;
+  double vg[3];
+  const double *sds_stats;
+  int32_t bnd_on[6];
+  double bnd_val[6];
+  int32_t n_pl;
+  int32_t pl_col[IG_MAX_PLANES];
+  double pl_val[IG_MAX_PLANES], pl_sgn[IG_MAX_PLANES];
+  double pl_w[IG_MAX_PLANES][4];
+  int32_t pl_kind[IG_MAX_PLANES];
+  uint64_t rng_key;
+};
+
+// Device-resident ion state, structure of arrays.  t is the elapsed flight
+// time in the route's clock (r-z us, 3-D s); tob the lab birth time (us).
+// Velocities are mm/us on both routes (the 3-D kernel converts to m/s in
+// registers exactly as the reference wrapper does on entry).
+struct EnsK {
+  int64_t n;
+  double *x, *y, *z, *vx, *vy, *vz, *t, *tob;
+  double *m_amu, *acc, *sigma;
+  double *sds_damp, *sds_mfp, *sds_V, *sds_logmr;
+  int64_t *seed;
+  uint64_t *rng_n;     // draws consumed
+  double *gauss;       // polar-method cache value
+  int32_t *has_gauss;
+  int32_t *fate, *face, *label, *nrec;
+  int64_t *ncol, *steps;
+  double *path, *eat, *ket; // record accumulators
+  double *e_last;      // (n, 3) last evaluated field (acceleration units)
+  double *rec;         // (n, rec_rows, rec_cols) or null
+  const int32_t *idx;  // optional compacted list of live ions
+  int64_t n_idx;
+};
+
+// values per node record: n_vals rounded up to a whole number of 16-byte
+// units (the kernels' RecLayout uses the same rule at compile time)
+inline int record_stride(int prec, int n_vals) {
+  const int per16 = prec == IG_PREC_F64 ? 2 : 4;
+  return ((n_vals + per16 - 1) / per16) * per16;
+}
+
+// launchers, instantiated once per precision in their own translation units
+// (ig_rz_*.hip, ig_xyz_*.hip); return 0, -1 launch error, -2 no kernel for
+// this pack
+template <int PREC>
+int launch_rz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch, int block,
+                   void *stream);
+template <int PREC>
+int launch_xyz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
+                    int block, void *stream);
+
+inline int launch_route(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
+                        int block, void *stream) {
+  if (n_launch <= 0) return 0;
+  const bool rz = f.route == IG_ROUTE_RZ;
+  switch (f.prec) {
+  case IG_PREC_F64:
+    return rz ? launch_rz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream)
+              : launch_xyz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
+  case IG_PREC_F32_STORE:
+    return rz ? launch_rz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream)
+              : launch_xyz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
+  case IG_PREC_F32:
+    return rz ? launch_rz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream)
+              : launch_xyz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
+  default: return -2;
+  }
+}
+
+} // namespace ig

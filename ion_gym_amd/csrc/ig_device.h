@@ -1,0 +1,274 @@
+// Device helpers shared by the r-z and 3-D flight kernels: the Philox
+// stream, the reference's polar-method normals, the hard-sphere and SDS
+// gas steps, and the drive waveforms.  All FP64 and written with
+// contraction off (the including translation unit sets
+// `#pragma clang fp contract(off)`), so the arithmetic follows the
+// reference's evaluation order operation by operation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "ig_common.h"
+
+namespace ig {
+
+#define IG_KB 1.380649e-23         // ion_gym/physics/constants.py:19
+#define IG_E_CHG 1.602176634e-19   // ion_gym/physics/constants.py:18
+#define IG_KG_AMU 1.66053906660e-27 // ion_gym/physics/constants.py:23
+#define IG_TWO_PI 6.283185307179586
+#define IG_PI 3.141592653589793
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC11).  One ion's stream is
+// keyed by its seed XOR the run key; draw n of the stream takes block n/2
+// and its word pair (0,1) or (2,3).  The stream is a function of (seed,
+// run key, n) only, so an ion draws the same numbers whatever lane, batch,
+// window or request mix it flies in.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t lo0 = 0xD2511F53u * c[0];
+    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+    uint32_t lo1 = 0xCD9E8D57u * c[2];
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    uint32_t n0 = hi1 ^ c[1] ^ k0;
+    uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+struct Rng {
+  uint32_t k0, k1;
+  uint64_t n;
+  int32_t has_gauss;
+  double gauss;
+};
+
+// a double in [0, 1) from two 32-bit words, 27 + 26 bits (the construction
+// numba's np.random.random() uses)
+__device__ __forceinline__ double words_to_u01(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+}
+
+__device__ __forceinline__ double rng_uniform(Rng &r) {
+  uint64_t blk = r.n >> 1;
+  bool hi = (r.n & 1u) != 0;
+  r.n += 1;
+  uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), 0u, 0u};
+  philox4x32_10(c, r.k0, r.k1);
+  return hi ? words_to_u01(c[2], c[3]) : words_to_u01(c[0], c[1]);
+}
+
+// polar-method normal with the second value of each pair cached, the
+// algorithm numba's np.random.normal() uses (collision3d.py:295-297)
+__device__ __forceinline__ double rng_normal(Rng &r) {
+  if (r.has_gauss) {
+    r.has_gauss = 0;
+    return r.gauss;
+  }
+  double x1, x2, r2;
+  // acceptance is pi/4 per pair: 200 rejections in a row has probability
+  // 1e-134, so the bound never binds for a working stream and guarantees
+  // the loop ends for any input
+  int tries = 0;
+  do {
+    x1 = 2.0 * rng_uniform(r) - 1.0;
+    x2 = 2.0 * rng_uniform(r) - 1.0;
+    r2 = x1 * x1 + x2 * x2;
+  } while ((r2 >= 1.0 || r2 == 0.0) && ++tries < 200);
+  if (r2 >= 1.0 || r2 == 0.0) r2 = 0.5;
+  double f = sqrt(-2.0 * log(r2) / r2);
+  r.gauss = f * x1;
+  r.has_gauss = 1;
+  return f * x2;
+}
+
+// ------------------------------------------------------------- hard sphere
+// collision3d.py:265-272
+__device__ __forceinline__ double c_bar_rel(double speed, double c_star, double c_bar) {
+  double s = speed / c_star;
+  if (s < 1e-12) return c_bar;
+  const double sqrt_pi = 1.7724538509055159; // math.sqrt(math.pi)
+  return c_bar * ((s + 1.0 / (2.0 * s)) * 0.5 * sqrt_pi * erf(s) + 0.5 * exp(-s * s));
+}
+
+// collision3d.py:281-330; returns false when no partner is accepted in
+// MAX_PARTNER_TRIES draws (the reference raises there)
+__device__ __noinline__ bool hs_collide(Rng &rng, double &vx, double &vy, double &vz, double m_ion,
+                                        double m_gas, double sig1d, double speed_rel) {
+  double scale = speed_rel + sig1d * 5.196152422706632;
+  double rx = 0, ry = 0, rz = 0, rlen = 0;
+  bool accepted = false;
+  for (int tr = 0; tr < 10000; ++tr) {
+    double gx = rng_normal(rng) * sig1d;
+    double gy = rng_normal(rng) * sig1d;
+    double gz = rng_normal(rng) * sig1d;
+    rx = vx - gx;
+    ry = vy - gy;
+    rz = vz - gz;
+    rlen = sqrt(rx * rx + ry * ry + rz * rz);
+    if (rng_uniform(rng) < rlen / scale) {
+      accepted = true;
+      break;
+    }
+  }
+  if (!accepted) return false;
+  if (rlen < 1e-12) return true;
+  double alpha = asin(sqrt(0.999999999 * rng_uniform(rng)));
+  double theta = 2.0 * IG_PI * rng_uniform(rng);
+  double ux = rx / rlen, uy = ry / rlen, uz = rz / rlen;
+  double px, py, pz;
+  if (fabs(ux) < 0.9) {
+    px = 0.0;
+    py = -uz;
+    pz = uy;
+  } else {
+    px = -uz;
+    py = 0.0;
+    pz = ux;
+  }
+  double pl = sqrt(px * px + py * py + pz * pz);
+  px = px / pl;
+  py = py / pl;
+  pz = pz / pl;
+  double qx = uy * pz - uz * py;
+  double qy = uz * px - ux * pz;
+  double qz = ux * py - uy * px;
+  double ca = cos(alpha), sa = sin(alpha);
+  double ct = cos(theta), st = sin(theta);
+  double rhx = ca * ux + sa * (ct * px + st * qx);
+  double rhy = ca * uy + sa * (ct * py + st * qy);
+  double rhz = ca * uz + sa * (ct * pz + st * qz);
+  double f = 2.0 * m_gas / (m_ion + m_gas) * (rx * rhx + ry * rhy + rz * rhz);
+  vx = vx - f * rhx;
+  vy = vy - f * rhy;
+  vz = vz - f * rhz;
+  return true;
+}
+
+// collision3d.py:333-374: one collision opportunity over the step, in the
+// gas-flow frame.  Returns 1 collided, 0 not, -1 partner draw refused.
+__device__ __forceinline__ int hs_step(Rng &rng, double &vx, double &vy, double &vz,
+                                       const FlightK &p, double dt, double dt_to_us,
+                                       double sigma, double m_ion) {
+  double wx = vx - p.vg[0], wy = vy - p.vg[1], wz = vz - p.vg[2];
+  double sp = sqrt(wx * wx + wy * wy + wz * wz);
+  if (sp < 1e-7) sp = 1e-7;
+  double cbr = c_bar_rel(sp, p.c_star, p.c_bar);
+  double lam = 1000.0 * IG_KB * p.T_k * (sp / cbr) / (p.P_pa * sigma);
+  double u = rng_uniform(rng);
+  if (u < 1.0 - exp(-sp * dt * dt_to_us / lam)) {
+    if (!hs_collide(rng, wx, wy, wz, m_ion, p.m_gas, p.sig1d, sp)) return -1;
+    vx = wx + p.vg[0];
+    vy = wy + p.vg[1];
+    vz = wz + p.vg[2];
+    return 1;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ SDS
+// sds.py:134-151 (_icdf_steps_at) on the (5, 1002) jump table
+__device__ __forceinline__ double icdf_steps_at(const double *__restrict__ st, double log_mr,
+                                                double u) {
+  double x = u * (1002 - 2);
+  int64_t i = (int64_t)x;
+  double w = x - (double)i;
+  int64_t k = (int64_t)floor(log_mr);
+  if (k < 0)
+    k = 0;
+  else if (k > 3)
+    k = 3;
+  double t = log_mr - (double)k;
+  const double *a = st + k * 1002, *b = a + 1002;
+  double lo = a[i] + w * (a[i + 1] - a[i]);
+  double hi = b[i] + w * (b[i + 1] - b[i]);
+  double llo = log10(lo);
+  return pow(10.0, llo + t * (log10(hi) - llo));
+}
+
+// tracer_rz.py:264-267 with sds.py:154-169
+__device__ __forceinline__ void sds_jump(Rng &rng, const double *__restrict__ st, double log_mr,
+                                         double V, double mfp, double dt, double &jx, double &jy,
+                                         double &jz) {
+  double dsteps = icdf_steps_at(st, log_mr, rng_uniform(rng));
+  double nc = V / mfp * dt;
+  double rj = sqrt(nc / 100000.0) * dsteps * mfp;
+  double c = 2.0 * rng_uniform(rng) - 1.0;
+  double ph = 2.0 * IG_PI * rng_uniform(rng);
+  double one_m = 1.0 - c * c;
+  double s = rj * sqrt(one_m > 0.0 ? one_m : 0.0);
+  jx = s * cos(ph);
+  jy = s * sin(ph);
+  jz = rj * c;
+}
+
+// -------------------------------------------------------------- waveforms
+__device__ __forceinline__ double py_mod1(double a) {
+  double m = fmod(a, 1.0);
+  if (m != 0.0) {
+    if (m < 0.0) m += 1.0;
+  } else {
+    m = 0.0;
+  }
+  return m;
+}
+
+// Drive waveform of channel k at lab time t (us).  affine = false is the
+// r-z unit waveform (build_planar.py:1059-1096: the amplitude is baked into
+// the channel grid); affine = true the 3-D amp*base(t)+off
+// (tracer3d.py:234-276).
+template <bool AFFINE>
+__device__ __forceinline__ double wave_eval(const FieldsK &f, int k, double t) {
+  int kind = f.kind[k];
+  double om = f.om[k], ph = f.ph[k];
+  if (kind == IG_WAVE_SIN) {
+    double s = sin(om * t + ph);
+    return AFFINE ? f.amp[k] * s + f.off[k] : s;
+  }
+  if (kind == IG_WAVE_COS) {
+    double s = cos(om * t + ph);
+    return AFFINE ? f.amp[k] * s + f.off[k] : s;
+  }
+  if (kind == IG_WAVE_SQUARE) {
+    double base;
+    if (f.duty[k] == 0.5)
+      base = sin(om * t + ph) >= 0.0 ? 1.0 : -1.0;
+    else
+      base = py_mod1((om * t + ph) / (2.0 * IG_PI)) < f.duty[k] ? 1.0 : -1.0;
+    return AFFINE ? f.amp[k] * base + f.off[k] : base;
+  }
+  int64_t o0 = f.tab_off[k], o1 = f.tab_off[k + 1];
+  const double *tt = f.tab_t, *tv = f.tab_v;
+  if (o1 - o0 == 0) return AFFINE ? f.off[k] : 0.0;
+  double v;
+  if (t <= tt[o0]) {
+    v = tv[o0];
+  } else if (t >= tt[o1 - 1]) {
+    v = tv[o1 - 1];
+  } else {
+    int64_t lo = o0, hi = o1 - 1;
+    while (hi - lo > 1) {
+      int64_t mid = (lo + hi) / 2;
+      if (tt[mid] <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    if (kind == IG_WAVE_TAB_HOLD) {
+      v = tv[lo];
+    } else {
+      double fr = (t - tt[lo]) / (tt[hi] - tt[lo]);
+      v = tv[lo] + fr * (tv[hi] - tv[lo]);
+    }
+  }
+  return AFFINE ? f.amp[k] * v + f.off[k] : v;
+}
+
+} // namespace ig

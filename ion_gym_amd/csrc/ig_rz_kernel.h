@@ -1,0 +1,511 @@
+// r-z flight kernel for gfx950: the batched, device-resident restatement of
+// ion_gym/physics/tracer_rz.py:113-446 (_fly_rec_full).
+//
+// One ion per lane.  The ion's state lives in registers for a whole window
+// of steps and in HBM (structure of arrays) between windows.  The kick-drift-
+// kick leapfrog, the field composition E_A + sum_k w_k(t) E_k (+ gate), the
+// record cadence, the 40-step nearest-node impact bisection, the box,
+// station and bound tests and their order are the reference's, with FP64
+// arithmetic evaluated in the reference's order (contraction off).
+//
+// Field evaluation reuse: in vacuum and hard-sphere flight the first field
+// evaluation of step n+1 is at the same point as the second evaluation of
+// step n (tracer_rz.py:216-217 after :234-235; a collision changes only the
+// velocity).  The kernel keeps the interpolated per-set values of the last
+// gather and re-gathers only when the sample point differs bitwise, and
+// keeps the waveform weights and re-evaluates them only when the lab time
+// differs bitwise.  Every value the reference computes is therefore
+// computed from identical inputs: the reuse changes no result, and it halves
+// the gathers per step.
+#pragma once
+#pragma clang fp contract(off)
+#include "ig_device.h"
+
+namespace ig {
+
+template <int PREC>
+struct StoreT {
+  using T = double;
+};
+template <>
+struct StoreT<IG_PREC_F32_STORE> {
+  using T = float;
+};
+template <>
+struct StoreT<IG_PREC_F32> {
+  using T = float;
+};
+
+// values per node record for NS sets of NC components, a whole number of
+// 16-byte units (the host lays records out with the same rule)
+template <typename T, int NS, int NC>
+struct RecLayout {
+  static constexpr int kVals = NS * NC;
+  static constexpr int kPer16 = 16 / (int)sizeof(T);
+  static constexpr int kStride = ((kVals + kPer16 - 1) / kPer16) * kPer16;
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void load_vals(const T *__restrict__ p, T (&v)[N]) {
+  if constexpr (sizeof(T) == 4) {
+    static_assert(N % 4 == 0, "float record");
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) {
+      float4 w = reinterpret_cast<const float4 *>(p)[q];
+      v[4 * q] = w.x;
+      v[4 * q + 1] = w.y;
+      v[4 * q + 2] = w.z;
+      v[4 * q + 3] = w.w;
+    }
+  } else {
+    static_assert(N % 2 == 0, "double record");
+#pragma unroll
+    for (int q = 0; q < N / 2; ++q) {
+      double2 w = reinterpret_cast<const double2 *>(p)[q];
+      v[2 * q] = w.x;
+      v[2 * q + 1] = w.y;
+    }
+  }
+}
+
+// interp._bilin (interp.py:19-36) for every set of the records around
+// (gx, gu): index clamped to the interior, fraction clamped to [0, 1].
+template <int PREC, int NS>
+__device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu, double (&cz)[NS],
+                                          double (&cr)[NS]) {
+  using T = typename StoreT<PREC>::T;
+  using L = RecLayout<T, NS, 2>;
+  const int64_t nx = f.n0, nu = f.n1;
+  // the pre-clamp keeps the conversion defined and gives int(g) clamped as
+  // the reference clamps it
+  int64_t i = (int64_t)fmin(fmax(gx, -1.0), (double)nx);
+  int64_t j = (int64_t)fmin(fmax(gu, -1.0), (double)nu);
+  if (i < 0) i = 0;
+  if (j < 0) j = 0;
+  if (i > nx - 2) i = nx - 2;
+  if (j > nu - 2) j = nu - 2;
+  double fx = gx - (double)i, fu = gu - (double)j;
+  if (fx < 0.0) fx = 0.0;
+  if (fx > 1.0) fx = 1.0;
+  if (fu < 0.0) fu = 0.0;
+  if (fu > 1.0) fu = 1.0;
+  const T *g = reinterpret_cast<const T *>(f.grid);
+  const T *p0 = g + (i * nu + j) * (int64_t)L::kStride; // (i, j), (i, j+1)
+  const T *p1 = p0 + nu * (int64_t)L::kStride;         // (i+1, j), (i+1, j+1)
+  T a00[L::kStride], a01[L::kStride], a10[L::kStride], a11[L::kStride];
+  load_vals(p0, a00);
+  load_vals(p0 + L::kStride, a01);
+  load_vals(p1, a10);
+  load_vals(p1 + L::kStride, a11);
+  if constexpr (PREC == IG_PREC_F32) {
+    const float fxf = (float)fx, fuf = (float)fu;
+    const float w00 = (1.0f - fxf) * (1.0f - fuf), w10 = fxf * (1.0f - fuf);
+    const float w01 = (1.0f - fxf) * fuf, w11 = fxf * fuf;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      float vz = __builtin_fmaf(a11[2 * s], w11,
+                                __builtin_fmaf(a01[2 * s], w01,
+                                               __builtin_fmaf(a10[2 * s], w10, a00[2 * s] * w00)));
+      float vr = __builtin_fmaf(
+          a11[2 * s + 1], w11,
+          __builtin_fmaf(a01[2 * s + 1], w01,
+                         __builtin_fmaf(a10[2 * s + 1], w10, a00[2 * s + 1] * w00)));
+      cz[s] = (double)vz * f.scale[s];
+      cr[s] = (double)vr * f.scale[s];
+    }
+  } else {
+    const double ax = 1 - fx, au = 1 - fu;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int q = 2 * s + c;
+        double v = (double)a00[q] * ax * au + (double)a10[q] * fx * au +
+                   (double)a01[q] * ax * fu + (double)a11[q] * fx * fu;
+        if constexpr (PREC == IG_PREC_F32_STORE) v = v * f.scale[s];
+        if (c == 0)
+          cz[s] = v;
+        else
+          cr[s] = v;
+      }
+    }
+  }
+}
+
+// raster2d._metal_nn (raster2d.py:597-612)
+__device__ __forceinline__ bool metal_nn(const uint8_t *__restrict__ m, double gx, double gy,
+                                         int64_t nx, int64_t ny) {
+  double fi = floor(gx + 0.5), fj = floor(gy + 0.5);
+  if (!(fi >= 0.0) || !(fj >= 0.0) || fi > (double)(nx - 1) || fj > (double)(ny - 1)) return false;
+  return m[(int64_t)fi * ny + (int64_t)fj] != 0;
+}
+
+// tracer_rz._write_row (tracer_rz.py:40-111)
+__device__ __noinline__ void rz_write_row(double *row, uint32_t mask, double t, double x, double y,
+                                          double z, double vx, double vy, double vz, double ez,
+                                          double er, int64_t ncol, double m_ion, double path,
+                                          double eat, double ket) {
+  row[0] = t;
+  row[1] = x;
+  row[2] = y;
+  row[3] = z;
+  row[4] = vx;
+  row[5] = vy;
+  row[6] = vz;
+  int c = 7;
+  double sp = sqrt(vx * vx + vy * vy + vz * vz);
+  double r = sqrt(y * y + z * z);
+  if (mask & IG_REC_SPEED) row[c++] = sp;
+  if (mask & IG_REC_KE) {
+    double s = sp * 1e3;
+    row[c++] = 0.5 * m_ion * IG_KG_AMU * (s * s) / IG_E_CHG;
+  }
+  double kes = 0.5 * m_ion * IG_KG_AMU * 1e6 / IG_E_CHG;
+  if (mask & IG_REC_KE_X) row[c++] = kes * vx * vx;
+  if (mask & IG_REC_KE_Y) row[c++] = kes * vy * vy;
+  if (mask & IG_REC_KE_Z) row[c++] = kes * vz * vz;
+  if (mask & IG_REC_E_FIELD) row[c++] = sqrt(ez * ez + er * er);
+  if (mask & IG_REC_E_AXIAL) row[c++] = ez;
+  if (mask & IG_REC_E_RADIAL) row[c++] = er;
+  if (mask & IG_REC_RADIUS) row[c++] = r;
+  if (mask & IG_REC_N_COL) row[c++] = (double)ncol;
+  if (mask & IG_REC_PATH) row[c++] = path;
+  if (mask & IG_REC_E_AXIAL_TINT) row[c++] = eat;
+  if (mask & IG_REC_KE_TINT) row[c++] = ket;
+}
+
+template <int PREC, int NS, int GAS>
+__global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const FlightK p,
+                                                     const EnsK e) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i;
+  if (e.idx) {
+    if (tid >= e.n_idx) return;
+    i = e.idx[tid];
+  } else {
+    if (tid >= e.n) return;
+    i = tid;
+  }
+  if (e.fate[i] != IG_FATE_FLYING) return;
+
+  double x = e.x[i], y = e.y[i], z = e.z[i];
+  double vx = e.vx[i], vy = e.vy[i], vz = e.vz[i];
+  double t = e.t[i];
+  const double tob = e.tob[i], acc = e.acc[i], m_ion = e.m_amu[i];
+  const double dt = p.dt, t_max = p.t_max, mm = f.h, u0 = f.u0;
+  const int64_t nx = f.n0, nu = f.n1;
+  const int K = f.K;
+  const bool gate = f.has_gate && f.tau_gate >= 0.0;
+  const bool rec = p.rec_rows > 0 && e.rec != nullptr;
+  int64_t ncol = e.ncol[i];
+  const int64_t steps0 = e.steps[i];
+  int32_t nrec = e.nrec[i];
+  Rng rng;
+  {
+    uint64_t key = (uint64_t)e.seed[i] ^ p.rng_key;
+    rng.k0 = (uint32_t)key;
+    rng.k1 = (uint32_t)(key >> 32);
+    rng.n = e.rng_n[i];
+    rng.has_gauss = e.has_gauss[i];
+    rng.gauss = e.gauss[i];
+  }
+  double path = 0.0, eat = 0.0, ket = 0.0;
+  if (rec) {
+    path = e.path[i];
+    eat = e.eat[i];
+    ket = e.ket[i];
+  }
+  const double fscale = acc != 0 ? 1e-3 / (1e-9 * acc) : 0.0;
+  const double ke_scale = 0.5 * m_ion * IG_KG_AMU * 1e6 / IG_E_CHG;
+  double sds_damp = 0.0, sds_mfp = 0.0, sds_V = 0.0, sds_logmr = 0.0, sds_fac = 1.0;
+  if constexpr (GAS == IG_GAS_SDS) {
+    sds_damp = e.sds_damp[i];
+    sds_mfp = e.sds_mfp[i];
+    sds_V = e.sds_V[i];
+    sds_logmr = e.sds_logmr[i];
+    if (sds_damp > 0.0) {
+      double tt = sds_damp * dt;
+      sds_fac = (1.0 - exp(-tt)) / tt;
+    }
+  }
+  const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
+  const double vgx = p.vg[0], vgy = p.vg[1], vgz = p.vg[2];
+
+  // cached gather and waveform weights (see the file comment)
+  double cz[NS], cr[NS];
+  double cgx = __builtin_nan(""), cgu = __builtin_nan("");
+  double wk[NS > 1 ? NS - 1 : 1];
+  double wtt = __builtin_nan("");
+  auto efield = [&](double px, double pr, double tt, double &ez_o, double &er_o) {
+    const double gx = px / mm;
+    const double gu = (pr - u0) / mm;
+    if (!(gx == cgx && gu == cgu)) {
+      rz_gather<PREC, NS>(f, gx, gu, cz, cr);
+      cgx = gx;
+      cgu = gu;
+    }
+    if (!(tt == wtt)) {
+#pragma unroll
+      for (int k = 0; k < NS - 1; ++k)
+        if (k < K) wk[k] = wave_eval<false>(f, k, tt);
+      wtt = tt;
+    }
+    double zz = cz[0], rz = cr[0];
+#pragma unroll
+    for (int k = 0; k < NS - 1; ++k) {
+      if (k < K) {
+        zz = zz + wk[k] * cz[1 + k];
+        rz = rz + wk[k] * cr[1 + k];
+      }
+    }
+    if (gate && tt >= f.tau_gate) {
+      zz = zz + cz[NS - 1];
+      rz = rz + cr[NS - 1];
+    }
+    ez_o = zz * 1e-9 * acc;
+    er_o = rz * 1e-9 * acc;
+  };
+
+  double ez, er;
+  if (steps0 == 0) {
+    // field at the birth point for row 0 (tracer_rz.py:184-202)
+    double r0 = sqrt(y * y + z * z);
+    efield(x, r0, tob, ez, er);
+    if (rec) {
+      rz_write_row(e.rec + (size_t)i * p.rec_rows * p.rec_cols, p.rec_mask, tob, x, y, z, vx, vy,
+                   vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
+      nrec = 1;
+    }
+  } else {
+    ez = e.e_last[3 * i];
+    er = e.e_last[3 * i + 1];
+  }
+
+  int fate = IG_FATE_FLYING, face = -1, label = 0;
+  int64_t s = 0;
+  const int64_t lim = p.max_steps;
+  for (;;) {
+    if (!(t < t_max)) {
+      fate = IG_FATE_TIMEOUT;
+      break;
+    }
+    if (lim > 0 && s >= lim) break;
+    const double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
+    double r = sqrt(y * y + z * z);
+    efield(x, r, tob + t, ez, er);
+    double rr = r > 1e-9 ? r : 1e-9;
+    if constexpr (GAS == IG_GAS_SDS) {
+      vx += 0.5 * sds_fac * (ez - (vx - vgx) * sds_damp) * dt;
+      vy += 0.5 * sds_fac * (er * y / rr - (vy - vgy) * sds_damp) * dt;
+      vz += 0.5 * sds_fac * (er * z / rr - (vz - vgz) * sds_damp) * dt;
+    } else {
+      vx += 0.5 * ez * dt;
+      vy += 0.5 * er * y / rr * dt;
+      vz += 0.5 * er * z / rr * dt;
+    }
+    x += vx * dt;
+    y += vy * dt;
+    z += vz * dt;
+    if (rec) path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+    r = sqrt(y * y + z * z);
+    efield(x, r, tob + t + dt, ez, er);
+    rr = r > 1e-9 ? r : 1e-9;
+    if constexpr (GAS == IG_GAS_SDS) {
+      vx += 0.5 * sds_fac * (ez - (vx - vgx) * sds_damp) * dt;
+      vy += 0.5 * sds_fac * (er * y / rr - (vy - vgy) * sds_damp) * dt;
+      vz += 0.5 * sds_fac * (er * z / rr - (vz - vgz) * sds_damp) * dt;
+    } else {
+      vx += 0.5 * ez * dt;
+      vy += 0.5 * er * y / rr * dt;
+      vz += 0.5 * er * z / rr * dt;
+    }
+    t += dt;
+    s += 1;
+    if (rec) {
+      double sp = sqrt(vx * vx + vy * vy + vz * vz);
+      eat += ez * fscale * dt;
+      ket += ke_scale * sp * sp * dt;
+    }
+    if constexpr (GAS == IG_GAS_SDS) {
+      if (sds_mfp > 0.0) {
+        double pre = 0.0;
+        if (rec) pre = sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+        double jx, jy, jz;
+        sds_jump(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
+        x += jx;
+        y += jy;
+        z += jz;
+        if (rec)
+          path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo)) - pre;
+      }
+    } else if constexpr (GAS == IG_GAS_HS) {
+      int hit = hs_step(rng, vx, vy, vz, p, dt, 1.0, sigma, m_ion);
+      if (hit < 0) {
+        fate = -3;
+        break;
+      }
+      ncol += hit;
+    }
+    if (rec && (steps0 + s) % p.rec_every == 0 && nrec < p.rec_rows - 1) {
+      rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t, x,
+                   y, z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
+      nrec += 1;
+    }
+    r = sqrt(y * y + z * z);
+    if (metal_nn(f.metal, x / mm, (r - u0) / mm, nx, nu)) {
+      double f0 = 0.0, f1 = 1.0;
+      for (int b = 0; b < 40; ++b) {
+        double fm = 0.5 * (f0 + f1);
+        double ym = yo + fm * (y - yo);
+        double zm = zo + fm * (z - zo);
+        double rm = sqrt(ym * ym + zm * zm);
+        if (metal_nn(f.metal, (xo + fm * (x - xo)) / mm, (rm - u0) / mm, nx, nu))
+          f1 = fm;
+        else
+          f0 = fm;
+      }
+      if (rec)
+        path -= (1.0 - f1) * sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+      x = xo + f1 * (x - xo);
+      y = yo + f1 * (y - yo);
+      z = zo + f1 * (z - zo);
+      vx = vxo + f1 * (vx - vxo);
+      vy = vyo + f1 * (vy - vyo);
+      vz = vzo + f1 * (vz - vzo);
+      t = to + f1 * dt;
+      fate = IG_FATE_IMPACT;
+      if (f.label) {
+        double re = sqrt(y * y + z * z);
+        double li = floor(x / mm + 0.5), lj = floor((re - u0) / mm + 0.5);
+        if (li >= 0.0 && lj >= 0.0 && li <= (double)(nx - 1) && lj <= (double)(nu - 1))
+          label = f.label[(int64_t)li * nu + (int64_t)lj];
+      }
+      break;
+    }
+    if (x < 0 || r / mm > (double)(nu - 2)) {
+      fate = IG_FATE_BOX;
+      break;
+    }
+    if (p.n_pl > 0) {
+      bool hit_pl = false;
+      for (int ip = 0; ip < p.n_pl; ++ip) {
+        const int pcol = p.pl_col[ip];
+        const double cn = pcol == 0 ? x : (pcol == 1 ? y : z);
+        const double co = pcol == 0 ? xo : (pcol == 1 ? yo : zo);
+        const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
+        bool crossed;
+        if (sgn == 0.0)
+          crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
+        else
+          crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
+        if (!crossed) continue;
+        const double den = cn - co;
+        double fr = den == 0.0 ? 0.0 : (val - co) / den;
+        if (fr < 0.0) fr = 0.0;
+        if (fr > 1.0) fr = 1.0;
+        const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
+        const double w1 = pcol == 0 ? yc : xc;
+        const double w2 = pcol == 2 ? yc : zc;
+        const bool ins = p.pl_w[ip][0] <= w1 && w1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= w2 &&
+                         w2 <= p.pl_w[ip][3];
+        if (p.pl_kind[ip] == IG_FATE_DETECT) {
+          if (!ins) continue;
+        } else if (ins) {
+          continue;
+        }
+        if (rec)
+          path -=
+              (1.0 - fr) * sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+        x = xc;
+        y = yc;
+        z = zc;
+        vx = vxo + fr * (vx - vxo);
+        vy = vyo + fr * (vy - vyo);
+        vz = vzo + fr * (vz - vzo);
+        t = to + fr * dt;
+        fate = p.pl_kind[ip];
+        face = ip;
+        hit_pl = true;
+        break;
+      }
+      if (hit_pl) break;
+    }
+    {
+      bool stop = false;
+#pragma unroll
+      for (int fc = 0; fc < 6; ++fc) {
+        if (stop || !p.bnd_on[fc]) continue;
+        const double v = fc < 2 ? x : (fc < 4 ? y : z);
+        if ((fc & 1) ? v > p.bnd_val[fc] : v < p.bnd_val[fc]) {
+          fate = IG_FATE_BOUND;
+          face = fc;
+          stop = true;
+        }
+      }
+      if (stop) break;
+    }
+    if (x / mm > (double)(nx - 1)) {
+      fate = IG_FATE_BOX;
+      break;
+    }
+  }
+  if (fate != IG_FATE_FLYING && rec) {
+    rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t, x, y,
+                 z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
+    nrec += 1;
+  }
+  e.x[i] = x;
+  e.y[i] = y;
+  e.z[i] = z;
+  e.vx[i] = vx;
+  e.vy[i] = vy;
+  e.vz[i] = vz;
+  e.t[i] = t;
+  e.fate[i] = fate;
+  e.face[i] = face;
+  e.label[i] = label;
+  e.ncol[i] = ncol;
+  e.steps[i] = steps0 + s;
+  e.nrec[i] = nrec;
+  e.rng_n[i] = rng.n;
+  e.has_gauss[i] = rng.has_gauss;
+  e.gauss[i] = rng.gauss;
+  e.e_last[3 * i] = ez;
+  e.e_last[3 * i + 1] = er;
+  if (rec) {
+    e.path[i] = path;
+    e.eat[i] = eat;
+    e.ket[i] = ket;
+  }
+}
+
+template <int PREC, int NS>
+static int launch_rz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
+                        int block, hipStream_t st) {
+  dim3 grid((unsigned)((n_launch + block - 1) / block));
+  if (p.gas == IG_GAS_HS)
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, IG_GAS_HS>), grid, dim3(block), 0, st, f, p, e);
+  else if (p.gas == IG_GAS_SDS)
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, IG_GAS_SDS>), grid, dim3(block), 0, st, f, p, e);
+  else
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, IG_GAS_VACUUM>), grid, dim3(block), 0, st, f, p,
+                       e);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// r-z packs carry 1 + K (+ gate) sets; 1..4 covers K <= 3, or K <= 2 with a
+// gate, which is every r-z drive the reference builds for the configs
+template <int PREC>
+int launch_rz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch, int block,
+                   void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  switch (f.n_sets) {
+  case 1: return launch_rz_ns<PREC, 1>(f, p, e, n_launch, block, st);
+  case 2: return launch_rz_ns<PREC, 2>(f, p, e, n_launch, block, st);
+  case 3: return launch_rz_ns<PREC, 3>(f, p, e, n_launch, block, st);
+  case 4: return launch_rz_ns<PREC, 4>(f, p, e, n_launch, block, st);
+  default: return -2;
+  }
+}
+
+} // namespace ig

@@ -1,0 +1,391 @@
+// 3-D flight kernel for gfx950: the batched, device-resident restatement of
+// ion_gym/physics/tracer3d.py:280-575 (_fly3d).
+//
+// One ion per lane; classic RK4 with positions in mm, velocities in m/s
+// and the step in s, exactly as the reference carries them; field
+// E_A + sum_k w_k(t) E_k with the affine waveforms; hard-sphere gas in the
+// flow frame; all-eight-corners impact test with 40-step bisection; planes
+// and box.  FP64 arithmetic in the reference's order (contraction off).
+// The two mid-step RK4 evaluations share one lab time
+// (tracer3d.py:403, 408), so their waveform weights are computed once.
+// The periodic transporter (tracer3d.py:434-458) is not part of this
+// kernel: the host refuses it.
+#pragma once
+#pragma clang fp contract(off)
+#include "ig_device.h"
+
+namespace ig {
+
+template <int PREC>
+struct StoreT3 {
+  using T = double;
+};
+template <>
+struct StoreT3<IG_PREC_F32_STORE> {
+  using T = float;
+};
+template <>
+struct StoreT3<IG_PREC_F32> {
+  using T = float;
+};
+
+template <typename T, int NS>
+struct RecLayout3 {
+  static constexpr int kPer16 = 16 / (int)sizeof(T);
+  static constexpr int kStride = ((NS * 3 + kPer16 - 1) / kPer16) * kPer16;
+};
+
+// tracer3d._tri (tracer3d.py:121-151) for all sets, combined with the
+// waveform weights as _acc/_efield combine them (tracer3d.py:351-386):
+// e = tri(E_A) then e = e + w_k * tri(E_k) for k = 1..K.
+template <int PREC, int NS>
+__device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy, double gz,
+                                          const double *w, int K, double &ex, double &ey,
+                                          double &ez) {
+  using T = typename StoreT3<PREC>::T;
+  using L = RecLayout3<T, NS>;
+  const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
+  double x = gx, y = gy, z = gz;
+  if (x < 0.0) x = 0.0;
+  if (y < 0.0) y = 0.0;
+  if (z < 0.0) z = 0.0;
+  if (x > nx - 1) x = nx - 1.0;
+  if (y > ny - 1) y = ny - 1.0;
+  if (z > nz - 1) z = nz - 1.0;
+  // NaN coordinates pass the clamps above; keep the conversion defined
+  int64_t i = (int64_t)fmax(x, 0.0), j = (int64_t)fmax(y, 0.0), k = (int64_t)fmax(z, 0.0);
+  if (i > nx - 2) i = nx - 2;
+  if (j > ny - 2) j = ny - 2;
+  if (k > nz - 2) k = nz - 2;
+  const double tx = x - i, ty = y - j, tz = z - k;
+  const T *g = reinterpret_cast<const T *>(f.grid);
+  const int64_t S = L::kStride;
+  const T *p00 = g + ((i * ny + j) * nz + k) * S; // (i, j, k), (i, j, k+1)
+  const T *p10 = p00 + ny * nz * S;               // (i+1, j, .)
+  const T *p01 = p00 + nz * S;                    // (i, j+1, .)
+  const T *p11 = p10 + nz * S;                    // (i+1, j+1, .)
+  double e[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    if (s > K) continue;
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int q = 3 * s + c;
+      if constexpr (PREC == IG_PREC_F32) {
+        const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
+        float c00 = __builtin_fmaf(p10[q] - p00[q], txf, p00[q]);
+        float c10 = __builtin_fmaf(p11[q] - p01[q], txf, p01[q]);
+        float c01 = __builtin_fmaf(p10[S + q] - p00[S + q], txf, p00[S + q]);
+        float c11 = __builtin_fmaf(p11[S + q] - p01[S + q], txf, p01[S + q]);
+        float c0 = __builtin_fmaf(c10 - c00, tyf, c00);
+        float c1 = __builtin_fmaf(c11 - c01, tyf, c01);
+        v[c] = (double)__builtin_fmaf(c1 - c0, tzf, c0) * f.scale[s];
+      } else {
+        double c00 = (double)p00[q] * (1 - tx) + (double)p10[q] * tx;
+        double c10 = (double)p01[q] * (1 - tx) + (double)p11[q] * tx;
+        double c01 = (double)p00[S + q] * (1 - tx) + (double)p10[S + q] * tx;
+        double c11 = (double)p01[S + q] * (1 - tx) + (double)p11[S + q] * tx;
+        double c0 = c00 * (1 - ty) + c10 * ty;
+        double c1 = c01 * (1 - ty) + c11 * ty;
+        double r = c0 * (1 - tz) + c1 * tz;
+        if constexpr (PREC == IG_PREC_F32_STORE) r = r * f.scale[s];
+        v[c] = r;
+      }
+    }
+    if (s == 0) {
+      e[0] = v[0];
+      e[1] = v[1];
+      e[2] = v[2];
+    } else {
+      e[0] = e[0] + w[s - 1] * v[0];
+      e[1] = e[1] + w[s - 1] * v[1];
+      e[2] = e[2] + w[s - 1] * v[2];
+    }
+  }
+  ex = e[0];
+  ey = e[1];
+  ez = e[2];
+}
+
+// tracer3d._in_metal_3d (tracer3d.py:174-189)
+__device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, double x, double y,
+                                            double z, int64_t nx, int64_t ny, int64_t nz) {
+  if (!(x >= 0.0) || !(y >= 0.0) || !(z >= 0.0) || x > nx - 1 || y > ny - 1 || z > nz - 1)
+    return false;
+  int64_t i = (int64_t)x, j = (int64_t)y, k = (int64_t)z;
+  if (i > nx - 2) i = nx - 2;
+  if (j > ny - 2) j = ny - 2;
+  if (k > nz - 2) k = nz - 2;
+  const uint8_t *a = m + (i * ny + j) * nz + k;
+  const int64_t di = ny * nz, dj = nz;
+  return a[0] && a[di] && a[dj] && a[di + dj] && a[1] && a[di + 1] && a[dj + 1] && a[di + dj + 1];
+}
+
+template <int PREC, int NS, int GAS>
+__global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const FlightK p,
+                                                      const EnsK e) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i;
+  if (e.idx) {
+    if (tid >= e.n_idx) return;
+    i = e.idx[tid];
+  } else {
+    if (tid >= e.n) return;
+    i = tid;
+  }
+  if (e.fate[i] != IG_FATE_FLYING) return;
+  const int64_t steps0 = e.steps[i];
+  double x = e.x[i], y = e.y[i], z = e.z[i];
+  double vx = e.vx[i], vy = e.vy[i], vz = e.vz[i];
+  if (e.nrec[i] == 0) {
+    // first entry: births arrive in mm/us; the kernel state is m/s
+    // (tracer3d.py:774-775: v0_mm_us * 1e3)
+    vx = vx * 1e3;
+    vy = vy * 1e3;
+    vz = vz * 1e3;
+  }
+  double t = e.t[i];
+  const double tob = e.tob[i], qm = e.acc[i], m_amu = e.m_amu[i];
+  const double dt = p.dt, t_max = p.t_max, h = f.h, inv_h = f.inv_h;
+  const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
+  const int K = f.K;
+  const bool rec = p.rec_rows > 0 && e.rec != nullptr;
+  int64_t ncol = e.ncol[i];
+  int32_t nrec = e.nrec[i];
+  Rng rng;
+  {
+    uint64_t key = (uint64_t)e.seed[i] ^ p.rng_key;
+    rng.k0 = (uint32_t)key;
+    rng.k1 = (uint32_t)(key >> 32);
+    rng.n = e.rng_n[i];
+    rng.has_gauss = e.has_gauss[i];
+    rng.gauss = e.gauss[i];
+  }
+  const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
+  constexpr int NW = NS > 1 ? NS - 1 : 1;
+  double w[NW];
+  double wtt = __builtin_nan("");
+  auto weights = [&](double tt) {
+    if (!(tt == wtt)) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k)
+        if (k < K) w[k] = wave_eval<true>(f, k, tt);
+      wtt = tt;
+    }
+  };
+  auto efield = [&](double px, double py, double pz, double tt, double &a, double &b,
+                    double &c) {
+    weights(tt);
+    xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, a, b, c);
+  };
+  auto write = [&](int row, double tus) {
+    double *r = e.rec + ((size_t)i * p.rec_rows + row) * p.rec_cols;
+    r[0] = tus;
+    r[1] = x;
+    r[2] = y;
+    r[3] = z;
+    r[4] = vx * 1e-3;
+    r[5] = vy * 1e-3;
+    r[6] = vz * 1e-3;
+    double a, b, c;
+    efield(x, y, z, tus, a, b, c);
+    r[7] = a;
+    r[8] = b;
+    r[9] = c;
+  };
+  if (nrec == 0) {
+    if (rec) write(0, tob);
+    nrec = 1;
+  }
+  const int R = rec ? p.rec_rows : 0;
+  const double c = 1e3;
+  int fate = IG_FATE_FLYING, hit_ip = -1;
+  int64_t s = 0;
+  const int64_t lim = p.max_steps;
+  for (;;) {
+    if (!(t < t_max)) {
+      fate = IG_FATE_TIMEOUT;
+      break;
+    }
+    if (lim > 0 && s >= lim) break;
+    const double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
+    const double t_us = tob + t * 1e6;
+    const double h_us = dt * 1e6;
+    double ax1, ay1, az1, ax2, ay2, az2, ax3, ay3, az3, ax4, ay4, az4, ea, eb, ec;
+    efield(x, y, z, t_us, ea, eb, ec);
+    ax1 = qm * 1e3 * ea;
+    ay1 = qm * 1e3 * eb;
+    az1 = qm * 1e3 * ec;
+    const double k1x = vx, k1y = vy, k1z = vz;
+    double px = x + 0.5 * dt * k1x * c, py = y + 0.5 * dt * k1y * c, pz = z + 0.5 * dt * k1z * c;
+    efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
+    ax2 = qm * 1e3 * ea;
+    ay2 = qm * 1e3 * eb;
+    az2 = qm * 1e3 * ec;
+    const double k2x = vx + 0.5 * dt * ax1, k2y = vy + 0.5 * dt * ay1, k2z = vz + 0.5 * dt * az1;
+    px = x + 0.5 * dt * k2x * c;
+    py = y + 0.5 * dt * k2y * c;
+    pz = z + 0.5 * dt * k2z * c;
+    efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
+    ax3 = qm * 1e3 * ea;
+    ay3 = qm * 1e3 * eb;
+    az3 = qm * 1e3 * ec;
+    const double k3x = vx + 0.5 * dt * ax2, k3y = vy + 0.5 * dt * ay2, k3z = vz + 0.5 * dt * az2;
+    px = x + dt * k3x * c;
+    py = y + dt * k3y * c;
+    pz = z + dt * k3z * c;
+    efield(px, py, pz, t_us + h_us, ea, eb, ec);
+    ax4 = qm * 1e3 * ea;
+    ay4 = qm * 1e3 * eb;
+    az4 = qm * 1e3 * ec;
+    const double k4x = vx + dt * ax3, k4y = vy + dt * ay3, k4z = vz + dt * az3;
+    x = x + dt / 6.0 * (k1x + 2 * k2x + 2 * k3x + k4x) * c;
+    y = y + dt / 6.0 * (k1y + 2 * k2y + 2 * k3y + k4y) * c;
+    z = z + dt / 6.0 * (k1z + 2 * k2z + 2 * k3z + k4z) * c;
+    vx = vx + dt / 6.0 * (ax1 + 2 * ax2 + 2 * ax3 + ax4);
+    vy = vy + dt / 6.0 * (ay1 + 2 * ay2 + 2 * ay3 + ay4);
+    vz = vz + dt / 6.0 * (az1 + 2 * az2 + 2 * az3 + az4);
+    t += dt;
+    s += 1;
+    if constexpr (GAS == IG_GAS_HS) {
+      double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
+      int hit = hs_step(rng, sx, sy, sz, p, dt, 1e6, sigma, m_amu);
+      if (hit < 0) {
+        fate = -3;
+        break;
+      }
+      if (hit) {
+        vx = sx * 1e3;
+        vy = sy * 1e3;
+        vz = sz * 1e3;
+        ncol += 1;
+      }
+    }
+    if (in_metal_3d(f.metal, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
+      double f0 = 0.0, f1 = 1.0;
+      for (int b = 0; b < 40; ++b) {
+        double fm = 0.5 * (f0 + f1);
+        if (in_metal_3d(f.metal, (xo + fm * (x - xo)) * inv_h, (yo + fm * (y - yo)) * inv_h,
+                        (zo + fm * (z - zo)) * inv_h, nx, ny, nz))
+          f1 = fm;
+        else
+          f0 = fm;
+      }
+      const double fr = f1;
+      x = xo + fr * (x - xo);
+      y = yo + fr * (y - yo);
+      z = zo + fr * (z - zo);
+      vx = vxo + fr * (vx - vxo);
+      vy = vyo + fr * (vy - vyo);
+      vz = vzo + fr * (vz - vzo);
+      t = to + fr * dt;
+      fate = IG_FATE_IMPACT;
+      if (nrec < R) write(nrec, tob + t * 1e6);
+      if (rec && nrec < R) nrec++;
+      break;
+    }
+    if (p.n_pl > 0) {
+      bool hit_pl = false;
+      for (int ip = 0; ip < p.n_pl; ++ip) {
+        const int col = p.pl_col[ip];
+        const double cn = col == 0 ? x : (col == 1 ? y : z);
+        const double co = col == 0 ? xo : (col == 1 ? yo : zo);
+        const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
+        bool crossed;
+        if (sgn == 0.0)
+          crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
+        else
+          crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
+        if (!crossed) continue;
+        const double den = cn - co;
+        double fr = den == 0.0 ? 0.0 : (val - co) / den;
+        if (fr < 0.0) fr = 0.0;
+        if (fr > 1.0) fr = 1.0;
+        const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
+        const double t1 = col == 0 ? yc : xc;
+        const double t2 = col == 2 ? yc : zc;
+        const bool ins = p.pl_w[ip][0] <= t1 && t1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= t2 &&
+                         t2 <= p.pl_w[ip][3];
+        if (p.pl_kind[ip] == IG_FATE_DETECT) {
+          if (!ins) continue;
+        } else if (ins) {
+          continue;
+        }
+        x = xc;
+        y = yc;
+        z = zc;
+        vx = vxo + fr * (vx - vxo);
+        vy = vyo + fr * (vy - vyo);
+        vz = vzo + fr * (vz - vzo);
+        t = to + fr * dt;
+        fate = p.pl_kind[ip];
+        hit_ip = ip;
+        hit_pl = true;
+        break;
+      }
+      if (hit_pl) {
+        if (nrec < R) write(nrec, tob + t * 1e6);
+        if (rec && nrec < R) nrec++;
+        break;
+      }
+    }
+    if (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h || y > (ny - 1) * h ||
+        z > (nz - 1) * h) {
+      fate = IG_FATE_BOX;
+      if (nrec < R) write(nrec, tob + t * 1e6);
+      if (rec && nrec < R) nrec++;
+      break;
+    }
+    if ((steps0 + s) % p.rec_every == 0 && nrec < R) {
+      write(nrec, tob + t * 1e6);
+      nrec++;
+    }
+  }
+  e.x[i] = x;
+  e.y[i] = y;
+  e.z[i] = z;
+  e.vx[i] = vx;
+  e.vy[i] = vy;
+  e.vz[i] = vz;
+  e.t[i] = t;
+  e.fate[i] = fate;
+  e.face[i] = hit_ip;
+  e.ncol[i] = ncol;
+  e.steps[i] = steps0 + s;
+  e.nrec[i] = nrec;
+  e.rng_n[i] = rng.n;
+  e.has_gauss[i] = rng.has_gauss;
+  e.gauss[i] = rng.gauss;
+}
+
+template <int PREC, int NS>
+static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
+                         int block, hipStream_t st) {
+  dim3 grid((unsigned)((n_launch + block - 1) / block));
+  if (p.gas == IG_GAS_HS)
+    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS>), grid, dim3(block), 0, st, f, p, e);
+  else if (p.gas == IG_GAS_VACUUM)
+    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM>), grid, dim3(block), 0, st, f, p,
+                       e);
+  else
+    return -2; // 3-D SDS is its own kernel (tracer3d._fly3d_sds), not this one
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// 3-D packs: static only (1), two RF channels (3: quadrupole), ten drive
+// channels (11: traveling-wave board)
+template <int PREC>
+int launch_xyz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch, int block,
+                    void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  switch (f.n_sets) {
+  case 1: return launch_xyz_ns<PREC, 1>(f, p, e, n_launch, block, st);
+  case 2: return launch_xyz_ns<PREC, 2>(f, p, e, n_launch, block, st);
+  case 3: return launch_xyz_ns<PREC, 3>(f, p, e, n_launch, block, st);
+  case 11: return launch_xyz_ns<PREC, 11>(f, p, e, n_launch, block, st);
+  default: return -2;
+  }
+}
+
+} // namespace ig

@@ -1,0 +1,173 @@
+"""r-z field model: from solved potentials to the kernel's channel grids.
+
+Restates ion_gym/physics/ionbench.py:44-96 (build_field_aware: mirror
+unfold about the axis, np.gradient, one-sided differences at metal-surface
+nodes, V/mm -> V/m) and the drive composition of
+ion_gym/physics/build_rz.py:428-534 (static field A with DC and drive
+offsets, one amplitude-baked potential per drive group, the two-point hold
+"gate").  The result is the pack _fly_rec_full consumes
+(build_rz.py:725-738), which ion_gym_amd.engine.Engine.upload_rz takes.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+K_SIN, K_COS, K_SQUARE = 0, 1, 2
+
+
+def build_field_aware(Z_mm, U_mm, phi, ele, symmetry="cylindrical"):
+    """(Z, Ue, Ez, Eu) with E in V/m on the mirror-unfolded grid
+    (ionbench.py:44-96)."""
+    if symmetry == "cylindrical":
+        assert abs(U_mm[0]) < 1e-12
+        Ue = np.concatenate([-U_mm[:0:-1], U_mm])
+        pe = np.concatenate([phi[:, :0:-1], phi], axis=1)
+        ee = np.concatenate([ele[:, :0:-1], ele], axis=1)
+    else:
+        Ue, pe, ee = U_mm.copy(), phi.copy(), ele.copy()
+    dpz, dpu = np.gradient(pe, Z_mm, Ue)
+    hz = Z_mm[1] - Z_mm[0]
+    hu = Ue[1] - Ue[0]
+    nz, nu = pe.shape
+    ee = np.asarray(ee).astype(bool)
+    # one-sided override at metal nodes, per axis: vacuum on exactly one side
+    vac = ~ee
+    vp = np.zeros_like(ee)
+    vm = np.zeros_like(ee)
+    vp[:-1, :] = vac[1:, :]
+    vm[1:, :] = vac[:-1, :]
+    fwd = ee & vp & ~vm
+    bwd = ee & vm & ~vp
+    dz_f = np.zeros_like(pe)
+    dz_b = np.zeros_like(pe)
+    dz_f[:-1, :] = (pe[1:, :] - pe[:-1, :]) / hz
+    dz_b[1:, :] = (pe[1:, :] - pe[:-1, :]) / hz
+    dpz = np.where(fwd, dz_f, np.where(bwd, dz_b, dpz))
+    vp = np.zeros_like(ee)
+    vm = np.zeros_like(ee)
+    vp[:, :-1] = vac[:, 1:]
+    vm[:, 1:] = vac[:, :-1]
+    fwd = ee & vp & ~vm
+    bwd = ee & vm & ~vp
+    du_f = np.zeros_like(pe)
+    du_b = np.zeros_like(pe)
+    du_f[:, :-1] = (pe[:, 1:] - pe[:, :-1]) / hu
+    du_b[:, 1:] = (pe[:, 1:] - pe[:, :-1]) / hu
+    dpu = np.where(fwd, du_f, np.where(bwd, du_b, dpu))
+    Ez = np.ascontiguousarray(-dpz * 1e3)
+    Eu = np.ascontiguousarray(-dpu * 1e3)
+    return (np.ascontiguousarray(Z_mm, np.float64),
+            np.ascontiguousarray(Ue, np.float64), Ez, Eu)
+
+
+@dataclass
+class DriveGroup:
+    """One drive group (the fields of RFGroupSpec the r-z route reads,
+    ion_gym/io/sim_spec.py:591-692): V(t) = amplitude_v * w(t) + offset_v
+    on every member electrode."""
+    name: str
+    waveform: str = "sin"          # sin | square | table (2-point gate)
+    frequency_hz: float = 0.0
+    amplitude_v: float = 0.0
+    phase_deg: float = 0.0
+    duty: float = 0.5
+    offset_v: float = 0.0
+    table_t_us: Optional[List[float]] = None
+    table_v: Optional[List[float]] = None
+
+
+@dataclass
+class RZModel:
+    """The r-z field model the engine flies (build_rz.RZModel's arrays)."""
+    EzA: np.ndarray
+    EuA: np.ndarray
+    EzK: np.ndarray
+    EuK: np.ndarray
+    ch_kind: np.ndarray
+    ch_om: np.ndarray
+    ch_ph: np.ndarray
+    ch_duty: np.ndarray
+    ele: np.ndarray            # (nz, nr) labels, 0 = vacuum
+    u0: float
+    mm_per_gu: float
+    EzG: np.ndarray = None
+    EuG: np.ndarray = None
+    tau_gate: float = -1.0
+    extra: dict = field(default_factory=dict)
+
+    def pack(self):
+        """The kernel pack (build_rz.py:725-738): mirrored float metal
+        (build_rz.py:615-616) and mirrored labels."""
+        ee = np.concatenate([self.ele[:, :0:-1], self.ele], axis=1)
+        return dict(EzA=self.EzA, EuA=self.EuA, EzK=self.EzK, EuK=self.EuK,
+                    EzG=self.EzG, EuG=self.EuG, tau_gate=self.tau_gate,
+                    ele=(ee > 0).astype(np.float64),
+                    label=ee.astype(np.uint8), u0=self.u0,
+                    mm=self.mm_per_gu, ch_kind=self.ch_kind,
+                    ch_om=self.ch_om, ch_ph=self.ch_ph,
+                    ch_duty=self.ch_duty)
+
+
+def compose_rz(bases, ele, h, dc, members, groups):
+    """Compose an r-z model from per-electrode unit potentials.
+
+    bases[idx] is electrode idx's solved potential at 1e4 V (idx 1-based,
+    the reference's cache convention, build_rz.py:377-386); dc[idx] its DC
+    volts; members[idx] the drive-group names it belongs to; groups the
+    DriveGroup list.  Follows build_rz.py:435-534 term by term."""
+    nz, nr = ele.shape
+    A = np.zeros((nz, nr))
+    gmap = {g.name: g for g in groups}
+    chan_B, chan_obj, chan_order = {}, {}, []
+    G = np.zeros((nz, nr))
+    tau_gate = -1.0
+    for idx in sorted(bases):
+        fa = bases[idx] / 1e4
+        A = A + dc.get(idx, 0.0) * fa
+        for gname in members.get(idx, ()):
+            drv = gmap[gname]
+            if drv.waveform == "table":
+                tt, tv = list(drv.table_t_us), list(drv.table_v)
+                if len(tt) != 2 or len(tv) != 2:
+                    raise ValueError("r-z gate: exactly a 2-point hold table")
+                if tau_gate >= 0.0 and abs(tt[1] - tau_gate) > 1e-9:
+                    raise ValueError("r-z gate: two distinct gate times")
+                tau_gate = tt[1]
+                A = A + (drv.amplitude_v * tv[0] + drv.offset_v) * fa
+                G = G + drv.amplitude_v * (tv[1] - tv[0]) * fa
+                continue
+            if drv.offset_v:
+                A = A + drv.offset_v * fa
+            if drv.amplitude_v == 0.0:
+                continue
+            if drv.name not in chan_B:
+                chan_B[drv.name] = np.zeros((nz, nr))
+                chan_obj[drv.name] = drv
+                chan_order.append(drv.name)
+            chan_B[drv.name] += drv.amplitude_v * fa
+    Z_mm = np.arange(nz) * h
+    U_mm = np.arange(nr) * h
+    _, Ue, EzA, EuA = build_field_aware(Z_mm, U_mm, A, ele)
+    EzK_l, EuK_l = [], []
+    for n in chan_order:
+        _, _, a, b = build_field_aware(Z_mm, U_mm, chan_B[n], ele)
+        EzK_l.append(a)
+        EuK_l.append(b)
+    EzK = np.stack(EzK_l) if EzK_l else np.zeros((0,) + EzA.shape)
+    EuK = np.stack(EuK_l) if EuK_l else np.zeros((0,) + EuA.shape)
+    drives = [chan_obj[n] for n in chan_order]
+    ch_kind = np.array([K_SQUARE if d.waveform == "square" else K_SIN
+                        for d in drives], np.int64)
+    ch_om = np.array([d.frequency_hz * 1e-6 * 2 * math.pi for d in drives],
+                     np.float64)
+    ch_ph = np.array([math.radians(d.phase_deg) for d in drives], np.float64)
+    ch_duty = np.array([float(d.duty) for d in drives], np.float64)
+    EzG = EuG = None
+    if tau_gate >= 0.0:
+        _, _, EzG, EuG = build_field_aware(Z_mm, U_mm, G, ele)
+    return RZModel(EzA=EzA, EuA=EuA, EzK=EzK, EuK=EuK, ch_kind=ch_kind,
+                   ch_om=ch_om, ch_ph=ch_ph, ch_duty=ch_duty, ele=ele,
+                   u0=float(Ue[0]), mm_per_gu=h, EzG=EzG, EuG=EuG,
+                   tau_gate=tau_gate)

@@ -1,0 +1,256 @@
+"""Pins of the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign or index, a transposed operand, a wrong draw order)
+fails it:
+
+* RNG: the reference's own pinned hard-sphere draw
+  (tests/test_collide_bound.py:21-35 of ion_gym), numpy's legacy MT19937
+  stream, and the published Philox4x32-10 known answers.
+* interpolation: scipy's RegularGridInterpolator on random grids and the
+  out-of-grid clamp rules of interp.py:26-34 and tracer3d.py:121-151.
+* waveforms: closed forms and the duty-0.5 sign(sin) rule.
+* mean relative speed: numerical quadrature of E|v - g| over the Maxwell
+  gas distribution.
+* flights: exact leapfrog/RK4 solutions in a uniform field, the
+  two-frequency closed form of ion_gym's test_multidrive (Gate 2), energy
+  conservation in a DC lens, hard-sphere thermalisation and the SDS
+  mean-square jump 6 D dt.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.interpolate import RegularGridInterpolator
+
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------- RNG
+def test_collide_matches_the_references_pinned_draws():
+    # ion_gym tests/test_collide_bound.py:21-35: _collide driven by
+    # np.random.seed(7), 2000 chained collisions, m 100 / 28, sig1d 0.17
+    a = O.collide_run(7, 2000, (0.3, -0.1, 0.05), 100.0, 28.0, 0.17)
+    assert float(a.sum()) == -21.505474884942984
+    assert float(np.abs(a).sum()) == 453.56068870741814
+    assert tuple(float(v) for v in a[-1]) == (
+        -0.04958323625398979, 0.026905754191870357, 0.007792414475627368)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 5, 2**32 - 1])
+def test_mt_stream_is_numpys_legacy_stream(seed):
+    rs = np.random.RandomState(seed)
+    assert np.array_equal(O.mt_draws(seed, 1000), rs.random_sample(1000))
+    rs = np.random.RandomState(seed)
+    assert np.array_equal(O.mt_draws(seed, 1001, normal=True),
+                          rs.standard_normal(1001))
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10
+    assert O.philox((0, 0, 0, 0), (0, 0)) == (
+        0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)
+    assert O.philox((0xffffffff,) * 4, (0xffffffff,) * 2) == (
+        0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)
+    assert O.philox((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+                    (0xa4093822, 0x299f31d0)) == (
+        0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)
+
+
+# ------------------------------------------------------ interpolation
+def test_bilinear_matches_regular_grid_interpolator():
+    rng = np.random.default_rng(1)
+    F = rng.normal(size=(7, 9))
+    rgi = RegularGridInterpolator((np.arange(7.0), np.arange(9.0)), F)
+    for gx, gu in rng.uniform([0, 0], [6, 8], (200, 2)):
+        assert O.bilin(F, gx, gu) == pytest.approx(rgi((gx, gu)), rel=1e-13,
+                                                   abs=1e-13)
+
+
+def test_bilinear_holds_the_edge_value_outside_the_grid():
+    F = np.arange(12.0).reshape(3, 4) ** 2
+    assert O.bilin(F, -3.0, 1.0) == F[0, 1]
+    assert O.bilin(F, 10.0, 1.0) == F[2, 1]
+    assert O.bilin(F, 1.0, 99.0) == F[1, 3]
+    assert O.bilin(F, 2.0, 3.0) == F[2, 3]
+    # index 0 with a negative fraction: int(-0.5) == 0, fraction clamped
+    assert O.bilin(F, -0.5, -0.5) == F[0, 0]
+
+
+def test_trilinear_matches_rgi_and_clamps():
+    rng = np.random.default_rng(2)
+    F = rng.normal(size=(5, 6, 7))
+    ax = [np.arange(float(n)) for n in F.shape]
+    rgi = RegularGridInterpolator(ax, F)
+    for p in rng.uniform([0, 0, 0], [4, 5, 6], (200, 3)):
+        assert O.tri(F, *p) == pytest.approx(rgi(p), rel=1e-13, abs=1e-13)
+    assert O.tri(F, -5.0, 2.0, 3.0) == pytest.approx(rgi((0.0, 2.0, 3.0)))
+    assert O.tri(F, 4.0, 5.0, 6.0) == F[4, 5, 6]
+    assert O.tri(F, 40.0, 50.0, 60.0) == F[4, 5, 6]
+
+
+# ------------------------------------------------------------ waveforms
+def test_unit_waveforms():
+    om, ph = 2 * math.pi * 1.3, 0.4
+    for t in (0.0, 0.17, 1.9, 33.3):
+        assert O.wave_rz(0, om, ph, 0.5, t) == math.sin(om * t + ph)
+        assert O.wave_rz(1, om, ph, 0.5, t) == math.cos(om * t + ph)
+        s = math.sin(om * t + ph)
+        assert O.wave_rz(2, om, ph, 0.5, t) == (1.0 if s >= 0 else -1.0)
+        fr = ((om * t + ph) / (2 * math.pi)) % 1.0
+        assert O.wave_rz(2, om, ph, 0.25, t) == (1.0 if fr < 0.25 else -1.0)
+    tt, tv = np.array([0.0, 1.0, 3.0]), np.array([2.0, 4.0, -1.0])
+    assert O.wave_rz(3, 0, 0, 0, -1.0, tt, tv, 0, 3) == 2.0
+    assert O.wave_rz(3, 0, 0, 0, 2.0, tt, tv, 0, 3) == 4.0
+    assert O.wave_rz(4, 0, 0, 0, 2.0, tt, tv, 0, 3) == 1.5
+    assert O.wave_rz(4, 0, 0, 0, 9.0, tt, tv, 0, 3) == -1.0
+    assert O.wave_rz(4, 0, 0, 0, 0.5, tt, tv, 0, 0) == 0.0
+
+
+# -------------------------------------------------- mean relative speed
+@pytest.mark.parametrize("speed", [1e-13, 0.05, 0.3, 1.0, 3.0])
+def test_c_bar_rel_is_the_maxwell_mean_relative_speed(speed):
+    # He at 300 K in mm/us
+    kT_m = 1.380649e-23 * 300.0 / (4.002602 * 1.66053906660e-27)
+    c_star = math.sqrt(2 * kT_m) / 1000.0
+    c_bar = math.sqrt(8 * kT_m / math.pi) / 1000.0
+    sig = math.sqrt(kT_m) / 1000.0
+    # E|v - g| for g ~ N(0, sig^2 I): integrate over |g| and the angle
+    from scipy import integrate
+
+    def f(gm, mu):
+        rel = math.sqrt(max(speed * speed + gm * gm - 2 * speed * gm * mu,
+                            0.0))
+        pdf = (gm * gm / sig ** 3) * math.sqrt(2 / math.pi) * math.exp(
+            -gm * gm / (2 * sig * sig))
+        return rel * pdf * 0.5
+    val, _ = integrate.dblquad(f, -1, 1, 0, 12 * sig)
+    assert O.c_bar_rel(speed, c_star, c_bar) == pytest.approx(val, rel=2e-7)
+
+
+# ---------------------------------------------------------------- flights
+def _uniform_rz_pack(nx=41, nr=11, h=0.5, Ez=2000.0, K_chans=()):
+    nu = 2 * nr - 1
+    pack = dict(EzA=np.full((nx, nu), Ez), EuA=np.zeros((nx, nu)),
+                EzK=np.zeros((len(K_chans), nx, nu)),
+                EuK=np.zeros((len(K_chans), nx, nu)),
+                ele=np.zeros((nx, nu)), label=np.zeros((nx, nu), np.uint8),
+                u0=-(nr - 1) * h, mm=h,
+                ch_kind=np.array([k[0] for k in K_chans], np.int64),
+                ch_om=np.array([k[1] for k in K_chans]),
+                ch_ph=np.array([k[2] for k in K_chans]),
+                ch_duty=np.full(len(K_chans), 0.5))
+    for i, (_k, _om, _ph, ampl) in enumerate(K_chans):
+        pack["EzK"][i] = ampl
+    return pack
+
+
+def _ions(n, x=1.0, vx=0.5, m=100.0, seed=0):
+    acc = 1.602176634e-19 / (m * 1.66053906660e-27)
+    return dict(x=np.full(n, x), y=np.linspace(0.0, 0.5, n), z=np.zeros(n),
+                vx=np.full(n, vx), vy=np.zeros(n), vz=np.zeros(n),
+                tob=np.zeros(n), m_amu=np.full(n, m), acc=np.full(n, acc),
+                sigma_m2=np.zeros(n),
+                seed=seed + np.arange(n, dtype=np.int64))
+
+
+def test_leapfrog_is_exact_in_a_uniform_field():
+    pack = _uniform_rz_pack(Ez=2000.0)
+    ions = _ions(4)
+    dt, nstep = 1e-3, 500
+    r = O.fly_rz(pack, dict(dt=dt, t_max=nstep * dt * (1 - 1e-12)), ions)
+    a = 2000.0 * 1e-9 * ions["acc"][0]
+    t = r["t"][0]
+    assert np.all(r["kind"] == 2)
+    assert r["steps"][0] == nstep
+    assert r["x"][0] == pytest.approx(1.0 + 0.5 * t + 0.5 * a * t * t,
+                                      rel=1e-12)
+    assert r["vx"][0] == pytest.approx(0.5 + a * t, rel=1e-12)
+
+
+def test_two_frequency_drive_closed_form():
+    # ion_gym tests/test_multidrive.py Gate 2 analogue: E(t) = E0 [cos(W t)
+    # - 1]... here w(t) = sin on two channels of different frequency; with
+    # v0 = 0 the exact solution is x(t) = x0 - sum_k (a_k/W_k) [t -
+    # sin(W_k t)/W_k] for a_k the channel acceleration amplitudes
+    W1, W2 = 2 * math.pi * 0.5, 2 * math.pi * 1.7
+    pack = _uniform_rz_pack(Ez=0.0, K_chans=[(0, W1, 0.0, 300.0),
+                                             (0, W2, 0.0, 500.0)])
+    ions = _ions(1, x=5.0, vx=0.0)
+    dt = 1e-4
+    T = 2.0
+    r = O.fly_rz(pack, dict(dt=dt, t_max=T), ions)
+    acc = ions["acc"][0] * 1e-9
+    t = r["t"][0]
+    x = 5.0
+    for W, E in ((W1, 300.0), (W2, 500.0)):
+        a = E * acc
+        x += (a / W) * (t - math.sin(W * t) / W)
+    assert r["x"][0] == pytest.approx(x, rel=1e-6)
+
+
+def test_dc_lens_conserves_energy():
+    from ion_gym_amd import configs
+    c = configs.einzel_rz(n_ions=40, coarse=2)
+    pack = c["model"].pack()
+    r = O.fly_rz(pack, dict(dt=c["dt"] / 4, t_max=1.0), c["ions"])
+    # potential from the axial field integral is awkward; use the energy
+    # change against the field work along the path via records instead:
+    # KE + q*phi with phi from the solved static potential
+    from ion_gym_amd.configs import _grid_labels  # noqa: F401
+    assert np.all(r["kind"] == 2)
+    m = c["ions"]["m_amu"][0] * 1.66053906660e-27
+    v0 = np.sqrt(c["ions"]["vx"] ** 2)
+    v1 = np.sqrt(r["vx"] ** 2 + r["vy"] ** 2 + r["vz"] ** 2)
+    ke0 = 0.5 * m * (v0 * 1e3) ** 2 / 1.602176634e-19
+    ke1 = 0.5 * m * (v1 * 1e3) ** 2 / 1.602176634e-19
+    # bilinear potential from the model's A: rebuild phi on the grid
+    from ion_gym_amd.physics.solver2d import solve_rz
+    from ion_gym_amd.configs import _grid_labels as gl
+    h = 0.2
+    lab = gl(201, 51, h, [(1, 4.0, 14.0, 5.0, 6.0), (2, 16.0, 26.0, 5.0, 6.0),
+                          (3, 28.0, 38.0, 5.0, 6.0)])
+    (A,) = solve_rz(lab, [{1: 0.0, 2: 600.0, 3: 0.0}], h)
+    rgi = RegularGridInterpolator((np.arange(201) * h, np.arange(51) * h), A)
+    phi0 = rgi(np.c_[c["ions"]["x"], np.hypot(c["ions"]["y"],
+                                               c["ions"]["z"])])
+    phi1 = rgi(np.c_[r["x"], np.hypot(r["y"], r["z"])])
+    drift = (ke1 + phi1) - (ke0 + phi0)
+    # grid-resolution energy error (field from a differenced potential)
+    assert np.max(np.abs(drift)) < 0.5, drift
+
+
+def test_hard_sphere_thermalises_to_three_halves_kT():
+    pack = _uniform_rz_pack(Ez=0.0, nx=201, h=1.0)
+    n = 400
+    ions = _ions(n, x=100.0, vx=0.0, m=100.0)
+    ions["vx"] = np.full(n, 3.0)     # far from thermal
+    ions["sigma_m2"] = np.full(n, 1e-18)
+    from ion_gym_amd.physics.collision3d import gas_kernel_scalars
+    g = gas_kernel_scalars("He", T_k=300.0, P_torr=10.0)
+    gas = dict(model=1, **{k: g[k] for k in ("T_k", "P_pa", "c_star",
+                                             "c_bar", "sig1d", "m_gas")})
+    r = O.fly_rz(pack, dict(dt=1e-4, t_max=5.0, gas=gas, rng=O.RNG_MT),
+                 ions)
+    alive = r["kind"] == 2
+    assert alive.mean() > 0.95
+    m = 100.0 * 1.66053906660e-27
+    ke = 0.5 * m * ((r["vx"] ** 2 + r["vy"] ** 2 + r["vz"] ** 2)[alive]
+                    * 1e6)
+    kT = 1.380649e-23 * 300.0
+    assert np.mean(ke) / (1.5 * kT) == pytest.approx(1.0, abs=0.1)
+
+
+def test_sds_mean_square_jump_is_6Ddt():
+    from ion_gym_amd.physics import sds
+    st = sds.default_statistics()
+    p = sds.transport_params(500.0, 1, 28.0134, 300.0, 4.0, 1.5, stats=st)
+    dt = 1e-3
+    # draw jumps with the oracle's kernel formula: E[r^2] = (V/mfp dt / N)
+    # mfp^2 <d^2> = 6 D dt by the choice of mfp (sds.py:401)
+    u = (np.arange(20000) + 0.5) / 20000
+    d = np.array([O.icdf_steps_at(st, p["log_mr_ratio"], x) for x in u])
+    rj2 = (p["V_mm_us"] / p["mfp_mm"] * dt / 100000.0) * d ** 2 * \
+        p["mfp_mm"] ** 2
+    assert np.mean(rj2) == pytest.approx(6 * p["D_mm2_per_us"] * dt,
+                                         rel=1e-3)
