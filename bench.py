@@ -97,8 +97,8 @@ def main():
                     choices=["f32", "f32_store", "f64"])
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the plain FP64 comparison run")
-    ap.add_argument("--cpu-sample", type=int, default=4000)
-    ap.add_argument("--cpu-window", type=int, default=2000)
+    ap.add_argument("--cpu-sample", type=int, default=16000)
+    ap.add_argument("--cpu-window", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

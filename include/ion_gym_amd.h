@@ -78,6 +78,7 @@ extern "C" {
 
 /* fates (ion_gym/physics/ion_envelope.py:100-103) plus the engine's own
    "still flying" marker for windowed flights */
+#define IG_FATE_EMPTY (-4)  /* ensemble slot holding no ion */
 #define IG_FATE_FLYING (-2)
 #define IG_FATE_IMPACT 0
 #define IG_FATE_BOX 1
@@ -181,7 +182,35 @@ typedef struct {
   int32_t pl_kind[IG_MAX_PLANES];
   uint64_t rng_key;    /* mixed into every ion's Philox key; lets a
                           serving loop keep requests' streams apart */
+  int32_t coll_batch;  /* hard-sphere collisions a 64-lane wave gathers
+                          before scattering them together (1 = at once;
+                          0 = engine default); results do not depend on
+                          it */
+  int32_t occupancy;   /* register-budget variant of the hard-sphere
+                          kernel: waves per SIMD 1, 2 or 3 (0 = engine
+                          default); results do not depend on it */
 } ig_flight_params;
+
+/* One tune: the parameters one request or ensemble of a batched launch
+ * flies with.  An ensemble with a tune table flies each ion with its own
+ * tune's waveforms, field-set scales, step, time limit and RNG key
+ * instead of the field pack's waveforms and the flight's dt/t_max/rng_key
+ * (all else comes from the flight).  This is how a single launch carries
+ * many ensembles of different drive tune -- the per-deck re-weighting the
+ * reference does by rebuilding its fly function per tune
+ * (ion_gym/physics/flight_conditions.py:55-243). */
+typedef struct {
+  double dt, t_max;   /* route clock, as in ig_flight_params */
+  uint64_t rng_key;
+  double set_scale[IG_MAX_CHANNELS + 2]; /* multiplies the interpolated
+                         field of set s (static, drive channels, gate) */
+  int32_t ch_kind[IG_MAX_CHANNELS];      /* sin, cos or square */
+  double ch_om[IG_MAX_CHANNELS];
+  double ch_ph[IG_MAX_CHANNELS];
+  double ch_duty[IG_MAX_CHANNELS];
+  double ch_amp[IG_MAX_CHANNELS];
+  double ch_off[IG_MAX_CHANNELS];
+} ig_tune;
 
 /* Per-ion inputs (host, SoA, n entries each).  Births are the rows of
  * generate_births (ion_gym/physics/sim_build.py:82-84); per-ion scalars
@@ -238,6 +267,21 @@ int ig_ensemble_create(ig_engine *h, int64_t capacity, int32_t rec_rows,
 int ig_ensemble_release(ig_engine *h, ig_ensemble_id id);
 int ig_ensemble_load(ig_engine *h, ig_ensemble_id id, const ig_ions *ions);
 int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *out);
+/* Load ions into slots [offset, offset + n) with tune index `tune`,
+ * leaving the other slots as they are (admission of a request into a
+ * running batch).  The kernel-level form of flying ions whose per-ion
+ * envelope (ion_gym/physics/build_rz.py:675-738) differs per request. */
+int ig_ensemble_load_at(ig_engine *h, ig_ensemble_id id, int64_t offset,
+                        const ig_ions *ions, int32_t tune);
+/* Read slots [offset, offset + n). */
+int ig_ensemble_read_at(ig_engine *h, ig_ensemble_id id, int64_t offset,
+                        int64_t n, ig_results *out);
+/* Install the ensemble's tune table (n = 0 removes it). */
+int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n,
+                          const ig_tune *tunes);
+/* Ions still flying per tune index, out[0..n) (synchronous). */
+int ig_ensemble_live_by_tune(ig_engine *h, ig_ensemble_id id, int32_t n,
+                             int64_t *out);
 /* Counts ions whose fate is still IG_FATE_FLYING (synchronous). */
 int ig_ensemble_live(ig_engine *h, ig_ensemble_id id, int64_t *out);
 /* Total integrator steps flown by the ensemble so far (synchronous). */

@@ -18,6 +18,7 @@ PRECISIONS = {"f64": PREC_F64, "f32_store": PREC_F32_STORE, "f32": PREC_F32}
 WAVE_SIN, WAVE_COS, WAVE_SQUARE, WAVE_TAB_HOLD, WAVE_TAB_LINEAR = range(5)
 GAS_VACUUM, GAS_HS, GAS_SDS = 0, 1, 2
 FATE_FLYING = -2
+FATE_EMPTY = -4
 MAX_CHANNELS = 16
 MAX_PLANES = 16
 
@@ -55,7 +56,20 @@ class FlightParams(C.Structure):
                 ("pl_sgn", C.c_double * MAX_PLANES),
                 ("pl_w", (C.c_double * 4) * MAX_PLANES),
                 ("pl_kind", C.c_int32 * MAX_PLANES),
-                ("rng_key", C.c_uint64)]
+                ("rng_key", C.c_uint64), ("coll_batch", C.c_int32),
+                ("occupancy", C.c_int32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("dt", C.c_double), ("t_max", C.c_double),
+                ("rng_key", C.c_uint64),
+                ("set_scale", C.c_double * (MAX_CHANNELS + 2)),
+                ("ch_kind", C.c_int32 * MAX_CHANNELS),
+                ("ch_om", C.c_double * MAX_CHANNELS),
+                ("ch_ph", C.c_double * MAX_CHANNELS),
+                ("ch_duty", C.c_double * MAX_CHANNELS),
+                ("ch_amp", C.c_double * MAX_CHANNELS),
+                ("ch_off", C.c_double * MAX_CHANNELS)]
 
 
 class Ions(C.Structure):
@@ -91,6 +105,14 @@ SIGNATURES = {
     "ig_ensemble_release": (C.c_int, [_VP, C.c_int64]),
     "ig_ensemble_load": (C.c_int, [_VP, C.c_int64, C.POINTER(Ions)]),
     "ig_ensemble_read": (C.c_int, [_VP, C.c_int64, C.POINTER(Results)]),
+    "ig_ensemble_load_at": (C.c_int, [_VP, C.c_int64, C.c_int64,
+                                      C.POINTER(Ions), C.c_int32]),
+    "ig_ensemble_read_at": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int64,
+                                      C.POINTER(Results)]),
+    "ig_ensemble_set_tunes": (C.c_int, [_VP, C.c_int64, C.c_int32,
+                                        C.POINTER(Tune)]),
+    "ig_ensemble_live_by_tune": (C.c_int, [_VP, C.c_int64, C.c_int32,
+                                           C.POINTER(C.c_int64)]),
     "ig_ensemble_live": (C.c_int, [_VP, C.c_int64, C.POINTER(C.c_int64)]),
     "ig_ensemble_steps": (C.c_int, [_VP, C.c_int64, C.POINTER(C.c_int64)]),
     "ig_fly": (C.c_int, [_VP, C.c_int64, C.POINTER(FlightParams),

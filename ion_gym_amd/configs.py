@@ -144,3 +144,81 @@ def drives_for(model):
                               (2 * math.pi) * 1e6,
                               phase_deg=math.degrees(model.ch_ph[k])))
     return out
+
+
+def _grad3(phi, h):
+    """E = -grad(phi) in V/mm (np.gradient: central inside, one-sided at
+    the array edges)."""
+    gx, gy, gz = np.gradient(phi, h, h, h)
+    return -gx, -gy, -gz
+
+
+def quad_3d(n_ions=100_000, seed=2, coarse=1, a=0.02, q_ref_mz=500.0,
+            rf_v=200.0, f_hz=1.0e6, n_cycles=2000):
+    """3-D quadrupole: four round rods (r0 = 4 mm, rod radius 4.6 mm,
+    20 mm long) on a 161 x 161 x 401 grid @ 0.05 mm (the 8 x 8 mm bore
+    region), entrance and exit plates.  Bases (unit volts): rod pair A,
+    rod pair B as the ideal quadrupole potentials 1/2 +- (x^2 - y^2) /
+    (2 r0^2), and the two plates as exponential end fields.  RF 1 MHz at
+    200 V0-p on A and B in opposite phase; DC U on A at the Mathieu point
+    (a, q(m/z 500)), U = a V / (2 q).  m/z uniform in 400-600, births in a
+    0.5 mm disk at z = 0.5 mm, KE U(4.5, 5.5) eV in a 2 degree cone;
+    dt = period / 250 = 4 ns; 2,000 RF cycles."""
+    h = 0.05 * coarse
+    n = 160 // coarse + 1
+    nzz = 400 // coarse + 1
+    r0 = 4.0
+    xs = np.arange(n) * h - 4.0
+    zs = np.arange(nzz) * h
+    X, Y, Z = np.meshgrid(xs, xs, zs, indexing="ij")
+    quad = (X * X - Y * Y) / (2.0 * r0 * r0)
+    lam = 1.0
+    entr = np.exp(-Z / lam)
+    exitp = np.exp(-(zs[-1] - Z) / lam)
+    phiA = 0.5 + quad
+    phiB = 0.5 - quad
+    # rods: centre (+-8.6, 0) pair A, (0, +-8.6) pair B, radius 4.6
+    R = 4.6
+    c = r0 + R
+    ele = (((X - c) ** 2 + Y ** 2 <= R * R) | ((X + c) ** 2 + Y ** 2 <= R * R)
+           | (X ** 2 + (Y - c) ** 2 <= R * R) | (X ** 2 + (Y + c) ** 2
+                                                <= R * R))
+    om = f_hz * 1e-6 * 2 * math.pi
+    Om = f_hz * 2 * math.pi
+    m_ref = q_ref_mz * KG_AMU
+    q = 4.0 * E_CHG * rf_v / (m_ref * Om ** 2 * (r0 * 1e-3) ** 2)
+    U = a * rf_v / (2.0 * q)
+    # static: U on pair A, plates at 0 V here (their bases keep the
+    # channels available; entrance +2 V, exit -2 V give a gentle push)
+    A = U * phiA + 2.0 * entr - 2.0 * exitp
+    EA = _grad3(A, h)
+    EB_A = _grad3(phiA, h)
+    EB_B = _grad3(phiB, h)
+    pack = dict(EAx=EA[0], EAy=EA[1], EAz=EA[2],
+                ExK=np.stack([EB_A[0], EB_B[0]]),
+                EyK=np.stack([EB_A[1], EB_B[1]]),
+                EzK=np.stack([EB_A[2], EB_B[2]]), ele=ele, h_mm=h,
+                ch_kind=np.array([0, 0], np.int64),
+                ch_om=np.array([om, om]), ch_ph=np.array([0.0, math.pi]),
+                ch_amp=np.array([rf_v, rf_v]), ch_off=np.zeros(2),
+                ch_duty=np.full(2, 0.5))
+    rng = np.random.default_rng(seed)
+    m = rng.uniform(400.0, 600.0, n_ions)
+    rr = 0.5 * np.sqrt(rng.uniform(0, 1, n_ions))
+    th = rng.uniform(0, 2 * math.pi, n_ions)
+    ke = rng.uniform(4.5, 5.5, n_ions)
+    cone = math.radians(2.0)
+    ct = rng.uniform(math.cos(cone), 1.0, n_ions)
+    st = np.sqrt(1 - ct * ct)
+    ph = rng.uniform(0, 2 * math.pi, n_ions)
+    v = _speed_mm_us(ke, m)
+    ions = dict(x=4.0 + rr * np.cos(th), y=4.0 + rr * np.sin(th),
+                z=np.full(n_ions, 0.5), vx=v * st * np.cos(ph),
+                vy=v * st * np.sin(ph), vz=v * ct, tob=np.zeros(n_ions),
+                m_amu=m, acc=E_CHG / (m * KG_AMU),
+                sigma_m2=np.zeros(n_ions),
+                seed=seed + np.arange(n_ions, dtype=np.int64))
+    dt_ns = 1e9 / f_hz / 250.0
+    return dict(name="quad_3d", route="xyz", pack=pack, ions=ions,
+                dt=dt_ns * 1e-9, t_max=n_cycles / f_hz, gas=None, U=U, q=q,
+                a=a)

@@ -17,6 +17,23 @@ namespace ig {
 // whole number of 16-byte units.  One bilinear or trilinear gather then
 // reads a few contiguous records instead of 2(1+K) or 3(1+K) separate
 // arrays.
+// Drive waveforms of the K channels (device memory).
+struct Waves {
+  int32_t kind[IG_MAX_CHANNELS];
+  double om[IG_MAX_CHANNELS], ph[IG_MAX_CHANNELS], duty[IG_MAX_CHANNELS];
+  double amp[IG_MAX_CHANNELS], off[IG_MAX_CHANNELS];
+  int64_t tab_off[IG_MAX_CHANNELS + 1];
+};
+
+// One tune: the per-request / per-ensemble parameters an ion of a batched
+// launch flies with (ig_tune of the C ABI, on the device).
+struct Tune {
+  Waves w;
+  double dt, t_max;
+  double scale[IG_MAX_CHANNELS + 2]; // multiplies each set's field
+  uint64_t rng_key;
+};
+
 struct FieldsK {
   int32_t route;
   int32_t prec;
@@ -32,11 +49,8 @@ struct FieldsK {
   const uint8_t *metal;
   const uint8_t *label;
   double scale[IG_MAX_CHANNELS + 2]; // per-set multiplier (FP32 storage)
-  int32_t kind[IG_MAX_CHANNELS];
-  double om[IG_MAX_CHANNELS], ph[IG_MAX_CHANNELS], duty[IG_MAX_CHANNELS];
-  double amp[IG_MAX_CHANNELS], off[IG_MAX_CHANNELS];
+  const struct Waves *waves;         // the pack's drive waveforms (device)
   const double *tab_t, *tab_v;
-  int64_t tab_off[IG_MAX_CHANNELS + 1];
 };
 
 struct FlightK {
@@ -56,6 +70,8 @@ struct FlightK {
   double pl_w[IG_MAX_PLANES][4];
   int32_t pl_kind[IG_MAX_PLANES];
   uint64_t rng_key;
+  int32_t coll_batch; // parked hard-sphere lanes a wave scatters together
+  int32_t occupancy;  // register-budget variant of the kernel (1, 2, 3)
 };
 
 // Device-resident ion state, structure of arrays.  t is the elapsed flight
@@ -78,6 +94,9 @@ struct EnsK {
   double *rec;         // (n, rec_rows, rec_cols) or null
   const int32_t *idx;  // optional compacted list of live ions
   int64_t n_idx;
+  const int32_t *tune; // per-ion tune index (with tunes)
+  const Tune *tunes;   // per-ensemble tune table, or null: the pack's
+                       // waveforms and the flight's dt/t_max/rng_key
 };
 
 // values per node record: n_vals rounded up to a whole number of 16-byte
@@ -97,22 +116,5 @@ template <int PREC>
 int launch_xyz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
                     int block, void *stream);
 
-inline int launch_route(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
-                        int block, void *stream) {
-  if (n_launch <= 0) return 0;
-  const bool rz = f.route == IG_ROUTE_RZ;
-  switch (f.prec) {
-  case IG_PREC_F64:
-    return rz ? launch_rz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream)
-              : launch_xyz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
-  case IG_PREC_F32_STORE:
-    return rz ? launch_rz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream)
-              : launch_xyz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
-  case IG_PREC_F32:
-    return rz ? launch_rz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream)
-              : launch_xyz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
-  default: return -2;
-  }
-}
 
 } // namespace ig

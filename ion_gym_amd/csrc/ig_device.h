@@ -152,25 +152,43 @@ __device__ __noinline__ bool hs_collide(Rng &rng, double &vx, double &vy, double
   return true;
 }
 
-// collision3d.py:333-374: one collision opportunity over the step, in the
-// gas-flow frame.  Returns 1 collided, 0 not, -1 partner draw refused.
-__device__ __forceinline__ int hs_step(Rng &rng, double &vx, double &vy, double &vz,
-                                       const FlightK &p, double dt, double dt_to_us,
-                                       double sigma, double m_ion) {
+// collision3d.py:333-374, in two halves so a kernel can defer the second:
+// hs_test draws the step's collision test in the gas-flow frame,
+// hs_scatter performs the collision (the velocity is unchanged in
+// between, so both halves see the same w and sp the reference computes
+// once).
+__device__ __forceinline__ bool hs_test(Rng &rng, double vx, double vy, double vz,
+                                        const FlightK &p, double dt, double dt_to_us,
+                                        double sigma) {
   double wx = vx - p.vg[0], wy = vy - p.vg[1], wz = vz - p.vg[2];
   double sp = sqrt(wx * wx + wy * wy + wz * wz);
   if (sp < 1e-7) sp = 1e-7;
   double cbr = c_bar_rel(sp, p.c_star, p.c_bar);
   double lam = 1000.0 * IG_KB * p.T_k * (sp / cbr) / (p.P_pa * sigma);
   double u = rng_uniform(rng);
-  if (u < 1.0 - exp(-sp * dt * dt_to_us / lam)) {
-    if (!hs_collide(rng, wx, wy, wz, m_ion, p.m_gas, p.sig1d, sp)) return -1;
-    vx = wx + p.vg[0];
-    vy = wy + p.vg[1];
-    vz = wz + p.vg[2];
-    return 1;
-  }
-  return 0;
+  return u < 1.0 - exp(-sp * dt * dt_to_us / lam);
+}
+
+// returns false when no partner is accepted (the reference raises)
+__device__ __forceinline__ bool hs_scatter(Rng &rng, double &vx, double &vy, double &vz,
+                                           const FlightK &p, double m_ion) {
+  double wx = vx - p.vg[0], wy = vy - p.vg[1], wz = vz - p.vg[2];
+  double sp = sqrt(wx * wx + wy * wy + wz * wz);
+  if (sp < 1e-7) sp = 1e-7;
+  if (!hs_collide(rng, wx, wy, wz, m_ion, p.m_gas, p.sig1d, sp)) return false;
+  vx = wx + p.vg[0];
+  vy = wy + p.vg[1];
+  vz = wz + p.vg[2];
+  return true;
+}
+
+// one collision opportunity over the step.  Returns 1 collided, 0 not, -1
+// partner draw refused.
+__device__ __forceinline__ int hs_step(Rng &rng, double &vx, double &vy, double &vz,
+                                       const FlightK &p, double dt, double dt_to_us,
+                                       double sigma, double m_ion) {
+  if (!hs_test(rng, vx, vy, vz, p, dt, dt_to_us, sigma)) return 0;
+  return hs_scatter(rng, vx, vy, vz, p, m_ion) ? 1 : -1;
 }
 
 // ------------------------------------------------------------------ SDS
@@ -225,28 +243,29 @@ __device__ __forceinline__ double py_mod1(double a) {
 // the channel grid); affine = true the 3-D amp*base(t)+off
 // (tracer3d.py:234-276).
 template <bool AFFINE>
-__device__ __forceinline__ double wave_eval(const FieldsK &f, int k, double t) {
-  int kind = f.kind[k];
-  double om = f.om[k], ph = f.ph[k];
+__device__ __forceinline__ double wave_eval(const Waves *__restrict__ W, const FieldsK &f, int k,
+                                            double t) {
+  int kind = W->kind[k];
+  double om = W->om[k], ph = W->ph[k];
   if (kind == IG_WAVE_SIN) {
     double s = sin(om * t + ph);
-    return AFFINE ? f.amp[k] * s + f.off[k] : s;
+    return AFFINE ? W->amp[k] * s + W->off[k] : s;
   }
   if (kind == IG_WAVE_COS) {
     double s = cos(om * t + ph);
-    return AFFINE ? f.amp[k] * s + f.off[k] : s;
+    return AFFINE ? W->amp[k] * s + W->off[k] : s;
   }
   if (kind == IG_WAVE_SQUARE) {
     double base;
-    if (f.duty[k] == 0.5)
+    if (W->duty[k] == 0.5)
       base = sin(om * t + ph) >= 0.0 ? 1.0 : -1.0;
     else
-      base = py_mod1((om * t + ph) / (2.0 * IG_PI)) < f.duty[k] ? 1.0 : -1.0;
-    return AFFINE ? f.amp[k] * base + f.off[k] : base;
+      base = py_mod1((om * t + ph) / (2.0 * IG_PI)) < W->duty[k] ? 1.0 : -1.0;
+    return AFFINE ? W->amp[k] * base + W->off[k] : base;
   }
-  int64_t o0 = f.tab_off[k], o1 = f.tab_off[k + 1];
+  int64_t o0 = W->tab_off[k], o1 = W->tab_off[k + 1];
   const double *tt = f.tab_t, *tv = f.tab_v;
-  if (o1 - o0 == 0) return AFFINE ? f.off[k] : 0.0;
+  if (o1 - o0 == 0) return AFFINE ? W->off[k] : 0.0;
   double v;
   if (t <= tt[o0]) {
     v = tv[o0];
@@ -268,7 +287,101 @@ __device__ __forceinline__ double wave_eval(const FieldsK &f, int k, double t) {
       v = tv[lo] + fr * (tv[hi] - tv[lo]);
     }
   }
-  return AFFINE ? f.amp[k] * v + f.off[k] : v;
+  return AFFINE ? W->amp[k] * v + W->off[k] : v;
+}
+
+// ------------------------------------------- FP32-basis mode (IG_PREC_F32)
+// The FP32 mode keeps FP64 for positions, velocities, time and the drive
+// phase argument; the quantities that only enter as a bounded weight or a
+// probability are computed in FP32 from FP64 inputs.  Their error against
+// the FP64 path is stated by the tests (a few 1e-7 relative per value).
+
+// sin/cos of an FP64 phase: reduced to [-pi, pi] in FP64, evaluated in FP32
+__device__ __forceinline__ float phase_reduce(double arg) {
+  double n = rint(arg * (1.0 / IG_TWO_PI));
+  return (float)(arg - n * IG_TWO_PI);
+}
+
+template <bool AFFINE>
+__device__ __forceinline__ double wave_eval_fast(const Waves *__restrict__ W, const FieldsK &f,
+                                                 int k, double t) {
+  int kind = W->kind[k];
+  if (kind == IG_WAVE_SIN || kind == IG_WAVE_COS ||
+      (kind == IG_WAVE_SQUARE && W->duty[k] == 0.5)) {
+    float r = phase_reduce(W->om[k] * t + W->ph[k]);
+    float v;
+    if (kind == IG_WAVE_SIN)
+      v = sinf(r);
+    else if (kind == IG_WAVE_COS)
+      v = cosf(r);
+    else
+      v = sinf(r) >= 0.0f ? 1.0f : -1.0f;
+    return AFFINE ? W->amp[k] * (double)v + W->off[k] : (double)v;
+  }
+  return wave_eval<AFFINE>(W, f, k, t);
+}
+
+// The hard-sphere collision test with the rate in FP32.  With
+// lam = 1000 kB T (sp / c_rel) / (P sigma) the exponent sp dt / lam is
+// dt c_rel P sigma / (1000 kB T): the ion speed cancels, so only c_rel(sp)
+// and one constant per ion enter.  The scattering stays FP64.
+__device__ __forceinline__ bool hs_test_fast(Rng &rng, double vx, double vy, double vz,
+                                             const FlightK &p, double dt, double dt_to_us,
+                                             double sigma) {
+  double wx = vx - p.vg[0], wy = vy - p.vg[1], wz = vz - p.vg[2];
+  double sp = sqrt(wx * wx + wy * wy + wz * wz);
+  if (sp < 1e-7) sp = 1e-7;
+  const float s = (float)(sp / p.c_star);
+  const float c_bar = (float)p.c_bar;
+  float cbr;
+  if (s < 1e-12f) {
+    cbr = c_bar;
+  } else {
+    const float sqrt_pi = 1.7724538509055159f;
+    cbr = c_bar * ((s + 0.5f / s) * 0.5f * sqrt_pi * erff(s) + 0.5f * expf(-s * s));
+  }
+  const float k = (float)(dt * dt_to_us * p.P_pa * sigma / (1000.0 * IG_KB * p.T_k));
+  const float prob = -expm1f(-k * cbr);
+  double u = rng_uniform(rng);
+  return u < (double)prob;
+}
+
+__device__ __forceinline__ int hs_step_fast(Rng &rng, double &vx, double &vy, double &vz,
+                                            const FlightK &p, double dt, double dt_to_us,
+                                            double sigma, double m_ion) {
+  if (!hs_test_fast(rng, vx, vy, vz, p, dt, dt_to_us, sigma)) return 0;
+  return hs_scatter(rng, vx, vy, vz, p, m_ion) ? 1 : -1;
+}
+
+// the SDS jump with the table interpolation, the log-log mass-ratio blend
+// and the direction in FP32
+__device__ __forceinline__ void sds_jump_fast(Rng &rng, const double *__restrict__ st,
+                                              double log_mr, double V, double mfp, double dt,
+                                              double &jx, double &jy, double &jz) {
+  double u = rng_uniform(rng);
+  float x = (float)(u * (1002 - 2));
+  int i = (int)x;
+  if (i > 999) i = 999;
+  float w = x - (float)i;
+  int k = (int)floor(log_mr);
+  k = k < 0 ? 0 : (k > 3 ? 3 : k);
+  float t = (float)(log_mr - (double)k);
+  const double *a = st + k * 1002, *b = a + 1002;
+  float lo = (float)a[i] + w * ((float)a[i + 1] - (float)a[i]);
+  float hi = (float)b[i] + w * ((float)b[i + 1] - (float)b[i]);
+  float llo = log10f(lo);
+  float dsteps = exp10f(llo + t * (log10f(hi) - llo));
+  double nc = V / mfp * dt;
+  double rj = sqrt(nc / 100000.0) * (double)dsteps * mfp;
+  float c = (float)(2.0 * rng_uniform(rng) - 1.0);
+  float ph = (float)(2.0 * IG_PI * rng_uniform(rng));
+  float one_m = 1.0f - c * c;
+  float sn, cs;
+  sincosf(ph, &sn, &cs);
+  double s = rj * (double)sqrtf(one_m > 0.0f ? one_m : 0.0f);
+  jx = s * (double)cs;
+  jy = s * (double)sn;
+  jz = rj * (double)c;
 }
 
 } // namespace ig

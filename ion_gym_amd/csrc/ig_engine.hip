@@ -16,8 +16,33 @@
 using namespace ig;
 
 namespace {
+int launch_route(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
+                        int block, void *stream) {
+  if (n_launch <= 0) return 0;
+  const bool rz = f.route == IG_ROUTE_RZ;
+  switch (f.prec) {
+  case IG_PREC_F64:
+    return rz ? launch_rz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream)
+              : launch_xyz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
+  case IG_PREC_F32_STORE:
+    return rz ? launch_rz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream)
+              : launch_xyz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
+  case IG_PREC_F32:
+    return rz ? launch_rz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream)
+              : launch_xyz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
+  default: return -2;
+  }
+}
+} // namespace
+
+namespace {
 
 thread_local std::string g_err;
+
+// parked hard-sphere lanes a wave scatters together when the caller leaves
+// coll_batch at 0 (the autotuner measures the best value per workload)
+constexpr int kDefaultCollBatch = 16;
+constexpr int kDefaultOccupancy = 2;
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -53,7 +78,8 @@ int dev_alloc(DevBuf &b, size_t bytes) {
 
 struct Fields {
   FieldsK k{};
-  DevBuf grid, metal, label, tab_t, tab_v;
+  DevBuf grid, metal, label, tab_t, tab_v, waves;
+  Waves waves_host{};
   int64_t bytes = 0;
 };
 
@@ -61,6 +87,10 @@ struct Ensemble {
   int64_t cap = 0, n = 0;
   int32_t route = IG_ROUTE_RZ; // clock of k.t: r-z us, 3-D s
   int32_t rec_rows = 0, rec_cols = 0;
+  int32_t n_tunes = 0;
+  DevBuf tunes;         // Tune table (device)
+  DevBuf counts;        // live_by_tune scratch (device)
+  int32_t *tune_idx = nullptr;
   std::vector<std::unique_ptr<DevBuf>> bufs;
   EnsK k{};
   template <class T>
@@ -202,27 +232,29 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
     k.scale[s] = k.prec == IG_PREC_F64 ? 1.0 : sc;
   }
   int64_t tab_total = 0;
+  Waves &wv = F->waves_host;
+  std::memset(&wv, 0, sizeof(wv));
   for (int c = 0; c < k.K; ++c) {
     int kind = d->ch_kind[c];
     if (kind < IG_WAVE_SIN || kind > IG_WAVE_TAB_LINEAR)
       return fail(IG_E_ARG, "ig_fields_upload: unknown waveform kind " + std::to_string(kind));
-    k.kind[c] = kind;
-    k.om[c] = d->ch_om[c];
-    k.ph[c] = d->ch_ph[c];
-    k.duty[c] = d->ch_duty[c];
-    k.amp[c] = d->ch_amp ? d->ch_amp[c] : 1.0;
-    k.off[c] = d->ch_off ? d->ch_off[c] : 0.0;
+    wv.kind[c] = kind;
+    wv.om[c] = d->ch_om[c];
+    wv.ph[c] = d->ch_ph[c];
+    wv.duty[c] = d->ch_duty[c];
+    wv.amp[c] = d->ch_amp ? d->ch_amp[c] : 1.0;
+    wv.off[c] = d->ch_off ? d->ch_off[c] : 0.0;
   }
-  for (int c = 0; c <= k.K; ++c) k.tab_off[c] = d->tab_off ? d->tab_off[c] : 0;
+  for (int c = 0; c <= k.K; ++c) wv.tab_off[c] = d->tab_off ? d->tab_off[c] : 0;
   if (d->tab_off) {
     tab_total = d->n_tab;
     for (int c = 0; c < k.K; ++c)
-      if (k.tab_off[c] < 0 || k.tab_off[c + 1] < k.tab_off[c] || k.tab_off[c + 1] > tab_total)
+      if (wv.tab_off[c] < 0 || wv.tab_off[c + 1] < wv.tab_off[c] || wv.tab_off[c + 1] > tab_total)
         return fail(IG_E_ARG, "ig_fields_upload: tab_off slices out of range");
   }
   for (int c = 0; c < k.K; ++c)
-    if ((k.kind[c] == IG_WAVE_TAB_HOLD || k.kind[c] == IG_WAVE_TAB_LINEAR) &&
-        k.tab_off[c + 1] > k.tab_off[c] && (!d->tab_t || !d->tab_v))
+    if ((wv.kind[c] == IG_WAVE_TAB_HOLD || wv.kind[c] == IG_WAVE_TAB_LINEAR) &&
+        wv.tab_off[c + 1] > wv.tab_off[c] && (!d->tab_t || !d->tab_v))
       return fail(IG_E_ARG, "ig_fields_upload: table channel without tab_t/tab_v");
 
   // interleave (set, comp) per node into padded records
@@ -258,12 +290,14 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
     if ((rc = upload(h, F->tab_t, d->tab_t, tab_total * sizeof(double)))) return rc;
     if ((rc = upload(h, F->tab_v, d->tab_v, tab_total * sizeof(double)))) return rc;
   }
+  if ((rc = upload(h, F->waves, &F->waves_host, sizeof(Waves)))) return rc;
   HIPX(hipStreamSynchronize(h->stream));
   k.grid = F->grid.p;
   k.metal = (const uint8_t *)F->metal.p;
   k.label = (const uint8_t *)F->label.p;
   k.tab_t = (const double *)F->tab_t.p;
   k.tab_v = (const double *)F->tab_v.p;
+  k.waves = (const Waves *)F->waves.p;
   F->bytes = (int64_t)(F->grid.bytes + F->metal.bytes + F->label.bytes + F->tab_t.bytes +
                        F->tab_v.bytes);
   int64_t id = h->next_id++;
@@ -314,9 +348,17 @@ int ig_ensemble_create(ig_engine *h, int64_t capacity, int32_t rec_rows, int32_t
   if ((rc = E->alloc(k.rng_n, n))) return rc;
   if ((rc = E->alloc(k.ncol, n))) return rc;
   if ((rc = E->alloc(k.steps, n))) return rc;
-  int32_t **i32[] = {&k.has_gauss, &k.fate, &k.face, &k.label, &k.nrec};
+  int32_t **i32[] = {&k.has_gauss, &k.fate, &k.face, &k.label, &k.nrec, &E->tune_idx};
   for (int32_t **a : i32)
     if ((rc = E->alloc(*a, n))) return rc;
+  // slots hold no ion until loaded: a fate the kernels skip
+  {
+    std::vector<int32_t> empty(n, IG_FATE_EMPTY);
+    HIPX(hipMemcpy(k.fate, empty.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPX(hipMemset(E->tune_idx, 0, n * sizeof(int32_t)));
+  }
+  k.tune = E->tune_idx;
+  k.tunes = nullptr;
   k.rec = nullptr;
   if (rec_rows > 0 && (rc = E->alloc(k.rec, n * (size_t)rec_rows * rec_cols))) return rc;
   k.idx = nullptr;
@@ -343,88 +385,94 @@ int ig_ensemble_release(ig_engine *h, ig_ensemble_id id) {
   return 0;
 }
 
-int ig_ensemble_load(ig_engine *h, ig_ensemble_id id, const ig_ions *io) {
-  if (!h || !io) return fail(IG_E_ARG, "ig_ensemble_load: null argument");
-  Ensemble *E = find_ens(h, id);
-  if (!E) return fail(IG_E_STATE, "ig_ensemble_load: unknown ensemble id");
-  if (io->n < 0 || io->n > E->cap)
-    return fail(IG_E_ARG, "ig_ensemble_load: n = " + std::to_string(io->n) +
-                              " exceeds the ensemble capacity " + std::to_string(E->cap));
+static int load_range(ig_engine *h, Ensemble *E, int64_t off, const ig_ions *io, int32_t tune,
+                      const char *who) {
+  const std::string w(who);
+  if (io->n < 0 || off < 0 || off + io->n > E->cap)
+    return fail(IG_E_ARG, w + ": ions [" + std::to_string(off) + ", " +
+                              std::to_string(off + io->n) + ") exceed the ensemble capacity " +
+                              std::to_string(E->cap));
   if (io->n > 0 && (!io->x || !io->y || !io->z || !io->vx || !io->vy || !io->vz || !io->tob ||
                     !io->m_amu || !io->acc || !io->seed))
-    return fail(IG_E_ARG, "ig_ensemble_load: births, m_amu, acc and seed are required");
+    return fail(IG_E_ARG, w + ": births, m_amu, acc and seed are required");
+  if (tune < 0 || tune >= (E->n_tunes > 0 ? E->n_tunes : 1))
+    return fail(IG_E_ARG, w + ": tune " + std::to_string(tune) + " is not in the tune table");
   if (int rc = set_dev(h)) return rc;
   const int64_t n = io->n;
-  EnsK &k = E->k;
-  E->n = n;
-  k.n = n;
   if (n == 0) return 0;
+  EnsK &k = E->k;
   const size_t b8 = (size_t)n * 8, b4 = (size_t)n * 4;
   hipStream_t st = h->stream;
-  HIPX(hipMemcpyAsync(k.x, io->x, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.y, io->y, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.z, io->z, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.vx, io->vx, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.vy, io->vy, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.vz, io->vz, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.tob, io->tob, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.m_amu, io->m_amu, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.acc, io->acc, b8, hipMemcpyHostToDevice, st));
-  HIPX(hipMemcpyAsync(k.seed, io->seed, b8, hipMemcpyHostToDevice, st));
+  struct {
+    void *dst;
+    const void *src;
+  } cp[] = {{k.x + off, io->x},         {k.y + off, io->y},     {k.z + off, io->z},
+            {k.vx + off, io->vx},       {k.vy + off, io->vy},   {k.vz + off, io->vz},
+            {k.tob + off, io->tob},     {k.m_amu + off, io->m_amu}, {k.acc + off, io->acc},
+            {k.seed + off, io->seed}};
+  for (auto &c : cp) HIPX(hipMemcpyAsync(c.dst, c.src, b8, hipMemcpyHostToDevice, st));
   if (io->sigma_m2)
-    HIPX(hipMemcpyAsync(k.sigma, io->sigma_m2, b8, hipMemcpyHostToDevice, st));
+    HIPX(hipMemcpyAsync(k.sigma + off, io->sigma_m2, b8, hipMemcpyHostToDevice, st));
   else
-    HIPX(hipMemsetAsync(k.sigma, 0, b8, st));
-  if (io->sds) {
-    // (n, 4) rows -> four columns
-    std::vector<double> col((size_t)n);
-    double *dst[4] = {k.sds_damp, k.sds_mfp, k.sds_V, k.sds_logmr};
-    for (int c = 0; c < 4; ++c) {
+    HIPX(hipMemsetAsync(k.sigma + off, 0, b8, st));
+  double *sds_cols[4] = {k.sds_damp + off, k.sds_mfp + off, k.sds_V + off, k.sds_logmr + off};
+  std::vector<double> col;
+  for (int c = 0; c < 4; ++c) {
+    if (io->sds) {
+      // (n, 4) rows -> four columns
+      col.resize((size_t)n);
       for (int64_t i = 0; i < n; ++i) col[i] = io->sds[4 * i + c];
-      HIPX(hipMemcpyAsync(dst[c], col.data(), b8, hipMemcpyHostToDevice, st));
+      HIPX(hipMemcpyAsync(sds_cols[c], col.data(), b8, hipMemcpyHostToDevice, st));
       HIPX(hipStreamSynchronize(st));
+    } else {
+      HIPX(hipMemsetAsync(sds_cols[c], 0, b8, st));
     }
-  } else {
-    HIPX(hipMemsetAsync(k.sds_damp, 0, b8, st));
-    HIPX(hipMemsetAsync(k.sds_mfp, 0, b8, st));
-    HIPX(hipMemsetAsync(k.sds_V, 0, b8, st));
-    HIPX(hipMemsetAsync(k.sds_logmr, 0, b8, st));
   }
-  HIPX(hipMemsetAsync(k.t, 0, b8, st));
-  HIPX(hipMemsetAsync(k.rng_n, 0, b8, st));
-  HIPX(hipMemsetAsync(k.gauss, 0, b8, st));
-  HIPX(hipMemsetAsync(k.has_gauss, 0, b4, st));
-  HIPX(hipMemsetAsync(k.ncol, 0, b8, st));
-  HIPX(hipMemsetAsync(k.steps, 0, b8, st));
-  HIPX(hipMemsetAsync(k.path, 0, b8, st));
-  HIPX(hipMemsetAsync(k.eat, 0, b8, st));
-  HIPX(hipMemsetAsync(k.ket, 0, b8, st));
-  HIPX(hipMemsetAsync(k.e_last, 0, 3 * b8, st));
-  HIPX(hipMemsetAsync(k.face, 0xff, b4, st)); // -1
-  HIPX(hipMemsetAsync(k.label, 0, b4, st));
-  HIPX(hipMemsetAsync(k.nrec, 0, b4, st));
-  if (io->fate_in) {
-    std::vector<int32_t> fz((size_t)n);
-    for (int64_t i = 0; i < n; ++i) fz[i] = io->fate_in[i] == IG_FATE_FLYING ? IG_FATE_FLYING
-                                                                              : io->fate_in[i];
-    HIPX(hipMemcpyAsync(k.fate, fz.data(), b4, hipMemcpyHostToDevice, st));
-    HIPX(hipStreamSynchronize(st));
-  } else {
-    std::vector<int32_t> fz((size_t)n, IG_FATE_FLYING);
-    HIPX(hipMemcpyAsync(k.fate, fz.data(), b4, hipMemcpyHostToDevice, st));
-    HIPX(hipStreamSynchronize(st));
-  }
+  void *zero8[] = {k.t + off,     k.rng_n + off, k.gauss + off, k.ncol + off,
+                   k.steps + off, k.path + off,  k.eat + off,   k.ket + off};
+  for (void *z : zero8) HIPX(hipMemsetAsync(z, 0, b8, st));
+  HIPX(hipMemsetAsync(k.e_last + 3 * off, 0, 3 * b8, st));
+  HIPX(hipMemsetAsync(k.has_gauss + off, 0, b4, st));
+  HIPX(hipMemsetAsync(k.face + off, 0xff, b4, st)); // -1
+  HIPX(hipMemsetAsync(k.label + off, 0, b4, st));
+  HIPX(hipMemsetAsync(k.nrec + off, 0, b4, st));
+  std::vector<int32_t> fz((size_t)n, IG_FATE_FLYING), tz((size_t)n, tune);
+  if (io->fate_in)
+    for (int64_t i = 0; i < n; ++i) fz[i] = io->fate_in[i];
+  HIPX(hipMemcpyAsync(k.fate + off, fz.data(), b4, hipMemcpyHostToDevice, st));
+  HIPX(hipMemcpyAsync(E->tune_idx + off, tz.data(), b4, hipMemcpyHostToDevice, st));
   HIPX(hipStreamSynchronize(st));
   return 0;
 }
 
-int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *o) {
-  if (!h || !o) return fail(IG_E_ARG, "ig_ensemble_read: null argument");
+int ig_ensemble_load(ig_engine *h, ig_ensemble_id id, const ig_ions *io) {
+  if (!h || !io) return fail(IG_E_ARG, "ig_ensemble_load: null argument");
   Ensemble *E = find_ens(h, id);
-  if (!E) return fail(IG_E_STATE, "ig_ensemble_read: unknown ensemble id");
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_load: unknown ensemble id");
+  int rc = load_range(h, E, 0, io, 0, "ig_ensemble_load");
+  if (rc) return rc;
+  E->n = io->n;
+  E->k.n = io->n;
+  return 0;
+}
+
+int ig_ensemble_load_at(ig_engine *h, ig_ensemble_id id, int64_t offset, const ig_ions *io,
+                        int32_t tune) {
+  if (!h || !io) return fail(IG_E_ARG, "ig_ensemble_load_at: null argument");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_load_at: unknown ensemble id");
+  int rc = load_range(h, E, offset, io, tune, "ig_ensemble_load_at");
+  if (rc) return rc;
+  if (offset + io->n > E->n) E->n = E->k.n = offset + io->n;
+  return 0;
+}
+
+static int read_range(ig_engine *h, Ensemble *E, int64_t off, int64_t n, ig_results *o) {
+  if (off < 0 || n < 0 || off + n > E->n)
+    return fail(IG_E_ARG, "ig_ensemble_read: range [" + std::to_string(off) + ", " +
+                              std::to_string(off + n) + ") is outside the loaded ions");
   if (int rc = set_dev(h)) return rc;
   const EnsK &k = E->k;
-  const int64_t n = E->n;
   if (n == 0) return 0;
   const size_t b8 = (size_t)n * 8, b4 = (size_t)n * 4;
   hipStream_t st = h->stream;
@@ -433,17 +481,19 @@ int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *o) {
     void *dst;
     const void *src;
     size_t bytes;
-  } cp[] = {{o->x, k.x, b8},         {o->y, k.y, b8},         {o->z, k.z, b8},
-            {o->vx, k.vx, b8},       {o->vy, k.vy, b8},       {o->vz, k.vz, b8},
-            {o->fate, k.fate, b4},   {o->face, k.face, b4},   {o->label, k.label, b4},
-            {o->ncol, k.ncol, b8},   {o->steps, k.steps, b8}, {o->nrec, k.nrec, b4}};
+  } cp[] = {{o->x, k.x + off, b8},         {o->y, k.y + off, b8},
+            {o->z, k.z + off, b8},         {o->vx, k.vx + off, b8},
+            {o->vy, k.vy + off, b8},       {o->vz, k.vz + off, b8},
+            {o->fate, k.fate + off, b4},   {o->face, k.face + off, b4},
+            {o->label, k.label + off, b4}, {o->ncol, k.ncol + off, b8},
+            {o->steps, k.steps + off, b8}, {o->nrec, k.nrec + off, b4}};
   for (auto &c : cp)
     if (c.dst) HIPX(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, st));
   if (E->route == IG_ROUTE_XYZ && (o->vx || o->vy || o->vz)) {
     // the 3-D kernel carries m/s from its first entry on (nrec >= 1);
     // report mm/us as the reference does (tracer3d.py:779: v * 1e-3)
     std::vector<int32_t> nr((size_t)n);
-    HIPX(hipMemcpyAsync(nr.data(), k.nrec, b4, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(nr.data(), k.nrec + off, b4, hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
     double *vv[3] = {o->vx, o->vy, o->vz};
     for (double *v : vv)
@@ -455,8 +505,8 @@ int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *o) {
     // lab time of the end state: tob + t (r-z, t in us) or tob + t*1e6
     // (3-D, t in s), evaluated on the host in the reference's order
     std::vector<double> t((size_t)n), tob((size_t)n);
-    HIPX(hipMemcpyAsync(t.data(), k.t, b8, hipMemcpyDeviceToHost, st));
-    HIPX(hipMemcpyAsync(tob.data(), k.tob, b8, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(t.data(), k.t + off, b8, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(tob.data(), k.tob + off, b8, hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
     if (E->route == IG_ROUTE_XYZ)
       for (int64_t i = 0; i < n; ++i) o->t_us[i] = tob[i] + t[i] * 1e6;
@@ -464,9 +514,121 @@ int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *o) {
       for (int64_t i = 0; i < n; ++i) o->t_us[i] = tob[i] + t[i];
   }
   if (o->records && E->rec_rows > 0 && k.rec)
-    HIPX(hipMemcpyAsync(o->records, k.rec, (size_t)n * E->rec_rows * E->rec_cols * 8,
-                        hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(o->records, k.rec + (size_t)off * E->rec_rows * E->rec_cols,
+                        (size_t)n * E->rec_rows * E->rec_cols * 8, hipMemcpyDeviceToHost, st));
   HIPX(hipStreamSynchronize(st));
+  return 0;
+}
+
+int ig_ensemble_read(ig_engine *h, ig_ensemble_id id, ig_results *o) {
+  if (!h || !o) return fail(IG_E_ARG, "ig_ensemble_read: null argument");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_read: unknown ensemble id");
+  return read_range(h, E, 0, E->n, o);
+}
+
+int ig_ensemble_read_at(ig_engine *h, ig_ensemble_id id, int64_t offset, int64_t n,
+                        ig_results *o) {
+  if (!h || !o) return fail(IG_E_ARG, "ig_ensemble_read_at: null argument");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_read_at: unknown ensemble id");
+  return read_range(h, E, offset, n, o);
+}
+
+int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_tune *tunes) {
+  if (!h || (n > 0 && !tunes)) return fail(IG_E_ARG, "ig_ensemble_set_tunes: null argument");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_set_tunes: unknown ensemble id");
+  if (n < 0 || n > (1 << 20)) return fail(IG_E_ARG, "ig_ensemble_set_tunes: bad tune count");
+  if (int rc = set_dev(h)) return rc;
+  HIPX(hipStreamSynchronize(h->stream));
+  if (n == 0) {
+    if (E->tunes.p) (void)hipFree(E->tunes.p);
+    E->tunes.p = nullptr;
+    E->tunes.bytes = 0;
+    E->n_tunes = 0;
+    E->k.tunes = nullptr;
+    return 0;
+  }
+  std::vector<Tune> host((size_t)n);
+  for (int32_t t = 0; t < n; ++t) {
+    const ig_tune &a = tunes[t];
+    Tune &b = host[t];
+    std::memset(&b, 0, sizeof(b));
+    if (!(a.dt > 0.0) || !std::isfinite(a.dt))
+      return fail(IG_E_ARG, "ig_ensemble_set_tunes: tune " + std::to_string(t) + " dt <= 0");
+    b.dt = a.dt;
+    b.t_max = a.t_max;
+    b.rng_key = a.rng_key;
+    for (int s2 = 0; s2 < IG_MAX_CHANNELS + 2; ++s2) b.scale[s2] = a.set_scale[s2];
+    for (int c = 0; c < IG_MAX_CHANNELS; ++c) {
+      if (a.ch_kind[c] < IG_WAVE_SIN || a.ch_kind[c] > IG_WAVE_SQUARE)
+        return fail(IG_E_ARG, "ig_ensemble_set_tunes: tune " + std::to_string(t) +
+                                  " channel " + std::to_string(c) +
+                                  ": tunes carry sin, cos or square drives (tables stay with "
+                                  "the field pack)");
+      b.w.kind[c] = a.ch_kind[c];
+      b.w.om[c] = a.ch_om[c];
+      b.w.ph[c] = a.ch_ph[c];
+      b.w.duty[c] = a.ch_duty[c];
+      b.w.amp[c] = a.ch_amp[c];
+      b.w.off[c] = a.ch_off[c];
+    }
+  }
+  // loaded ions must keep a valid tune index
+  if (E->n > 0) {
+    std::vector<int32_t> idx((size_t)E->n);
+    HIPX(hipMemcpy(idx.data(), E->tune_idx, E->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int32_t v : idx)
+      if (v >= n) return fail(IG_E_ARG, "ig_ensemble_set_tunes: a loaded ion uses tune " +
+                                            std::to_string(v) + " beyond the new table");
+  }
+  DevBuf nb;
+  if (int rc = dev_alloc(nb, host.size() * sizeof(Tune))) return rc;
+  HIPX(hipMemcpy(nb.p, host.data(), host.size() * sizeof(Tune), hipMemcpyHostToDevice));
+  std::swap(E->tunes.p, nb.p);
+  std::swap(E->tunes.bytes, nb.bytes);
+  E->n_tunes = n;
+  E->k.tunes = (const Tune *)E->tunes.p;
+  return 0;
+}
+
+__global__ void count_live_kernel(const int32_t *__restrict__ fate,
+                                  const int32_t *__restrict__ tune, int64_t n, int32_t nt,
+                                  unsigned long long *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && fate[i] == IG_FATE_FLYING) {
+    const int32_t t = tune[i];
+    if (t >= 0 && t < nt) atomicAdd(&cnt[t], 1ull);
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int32_t n) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+int ig_ensemble_live_by_tune(ig_engine *h, ig_ensemble_id id, int32_t n, int64_t *out) {
+  if (!h || (n > 0 && !out)) return fail(IG_E_ARG, "ig_ensemble_live_by_tune: null argument");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_live_by_tune: unknown ensemble id");
+  if (n <= 0) return 0;
+  if (int rc = set_dev(h)) return rc;
+  if (E->counts.bytes < (size_t)n * 8) {
+    DevBuf nb;
+    if (int rc = dev_alloc(nb, (size_t)n * 8)) return rc;
+    HIPX(hipStreamSynchronize(h->stream));
+    std::swap(E->counts.p, nb.p);
+    std::swap(E->counts.bytes, nb.bytes);
+  }
+  auto *cnt = (unsigned long long *)E->counts.p;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, h->stream, cnt, n);
+  if (E->n > 0)
+    hipLaunchKernelGGL(count_live_kernel, dim3((unsigned)((E->n + 255) / 256)), dim3(256), 0,
+                       h->stream, E->k.fate, E->tune_idx, E->n, n, cnt);
+  HIPX(hipGetLastError());
+  HIPX(hipMemcpyAsync(out, cnt, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPX(hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -542,6 +704,12 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
     k.pl_kind[i] = p->pl_kind[i];
   }
   k.rng_key = p->rng_key;
+  if (p->coll_batch < 0 || p->coll_batch > 64)
+    return fail(IG_E_ARG, "ig_fly: coll_batch must be in [0, 64]");
+  k.coll_batch = p->coll_batch > 0 ? p->coll_batch : kDefaultCollBatch;
+  if (p->occupancy < 0 || p->occupancy > 3)
+    return fail(IG_E_ARG, "ig_fly: occupancy must be 0 (default), 1, 2 or 3");
+  k.occupancy = p->occupancy > 0 ? p->occupancy : kDefaultOccupancy;
   return 0;
 }
 

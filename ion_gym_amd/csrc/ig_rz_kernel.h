@@ -174,9 +174,30 @@ __device__ __noinline__ void rz_write_row(double *row, uint32_t mask, double t, 
   if (mask & IG_REC_KE_TINT) row[c++] = ket;
 }
 
-template <int PREC, int NS, int GAS>
-__global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const FlightK p,
-                                                     const EnsK e) {
+// Per-lane stage of the step machine below.
+enum : int { kStep = 0, kCollide = 1, kPost = 2, kDone = 3 };
+
+// The flight kernel.  Each lane runs a small state machine per step:
+//   kStep    the leapfrog step (kick, drift, kick), t += dt, the record
+//            accumulators, then the SDS jump or the hard-sphere test;
+//   kCollide a hard-sphere collision is due (the test fired);
+//   kPost    record, metal + bisection, box, stations, bounds, x-high
+//            (tracer_rz.py:279-440), then back to kStep or done.
+// Hard-sphere collisions are rare per lane (dt ~ tau/20) but the
+// scattering with its rejection-sampled partner is long; run inline, some
+// lane of a 64-lane wave would pay it on nearly every step.  A lane whose
+// test fires therefore parks in kCollide while the other lanes keep
+// stepping, and the wave scatters all parked lanes together once
+// p.coll_batch of them wait (or no lane can step).  Each ion still
+// executes exactly the reference's sequence of operations and draws --
+// step, test, scatter, checks, step -- only lanes drift apart in step
+// count, which the per-lane window bound absorbs.
+// OCC bounds the register budget so that at least OCC waves share a SIMD
+// (1 = the compiler's choice); the collision pass then keeps what does not
+// fit in scratch.  Results do not depend on it.
+template <int PREC, int NS, int GAS, bool REC, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void rz_fly_kernel(
+    const FieldsK f, const FlightK p, const EnsK e) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i;
   if (e.idx) {
@@ -192,17 +213,30 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
   double vx = e.vx[i], vy = e.vy[i], vz = e.vz[i];
   double t = e.t[i];
   const double tob = e.tob[i], acc = e.acc[i], m_ion = e.m_amu[i];
-  const double dt = p.dt, t_max = p.t_max, mm = f.h, u0 = f.u0;
+  const double mm = f.h, u0 = f.u0;
+  // the ion's tune: the pack's waveforms and the flight's scalars, or its
+  // row of the ensemble's tune table
+  const Waves *W = f.waves;
+  double dt = p.dt, t_max = p.t_max;
+  uint64_t rkey = p.rng_key;
+  const double *tscale = nullptr;
+  if (e.tunes) {
+    const Tune *T = e.tunes + e.tune[i];
+    W = &T->w;
+    dt = T->dt;
+    t_max = T->t_max;
+    rkey = T->rng_key;
+    tscale = T->scale;
+  }
   const int64_t nx = f.n0, nu = f.n1;
   const int K = f.K;
   const bool gate = f.has_gate && f.tau_gate >= 0.0;
-  const bool rec = p.rec_rows > 0 && e.rec != nullptr;
   int64_t ncol = e.ncol[i];
   const int64_t steps0 = e.steps[i];
   int32_t nrec = e.nrec[i];
   Rng rng;
   {
-    uint64_t key = (uint64_t)e.seed[i] ^ p.rng_key;
+    uint64_t key = (uint64_t)e.seed[i] ^ rkey;
     rng.k0 = (uint32_t)key;
     rng.k1 = (uint32_t)(key >> 32);
     rng.n = e.rng_n[i];
@@ -210,13 +244,12 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
     rng.gauss = e.gauss[i];
   }
   double path = 0.0, eat = 0.0, ket = 0.0;
-  if (rec) {
+  if constexpr (REC) {
     path = e.path[i];
     eat = e.eat[i];
     ket = e.ket[i];
   }
   const double fscale = acc != 0 ? 1e-3 / (1e-9 * acc) : 0.0;
-  const double ke_scale = 0.5 * m_ion * IG_KG_AMU * 1e6 / IG_E_CHG;
   double sds_damp = 0.0, sds_mfp = 0.0, sds_V = 0.0, sds_logmr = 0.0, sds_fac = 1.0;
   if constexpr (GAS == IG_GAS_SDS) {
     sds_damp = e.sds_damp[i];
@@ -229,7 +262,6 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
     }
   }
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
-  const double vgx = p.vg[0], vgy = p.vg[1], vgz = p.vg[2];
 
   // cached gather and waveform weights (see the file comment)
   double cz[NS], cr[NS];
@@ -241,13 +273,22 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
     const double gu = (pr - u0) / mm;
     if (!(gx == cgx && gu == cgu)) {
       rz_gather<PREC, NS>(f, gx, gu, cz, cr);
+      if (tscale) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          cz[q] = cz[q] * tscale[q];
+          cr[q] = cr[q] * tscale[q];
+        }
+      }
       cgx = gx;
       cgu = gu;
     }
     if (!(tt == wtt)) {
 #pragma unroll
       for (int k = 0; k < NS - 1; ++k)
-        if (k < K) wk[k] = wave_eval<false>(f, k, tt);
+        if (k < K)
+          wk[k] = PREC == IG_PREC_F32 ? wave_eval_fast<false>(W, f, k, tt)
+                                      : wave_eval<false>(W, f, k, tt);
       wtt = tt;
     }
     double zz = cz[0], rz = cr[0];
@@ -271,7 +312,7 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
     // field at the birth point for row 0 (tracer_rz.py:184-202)
     double r0 = sqrt(y * y + z * z);
     efield(x, r0, tob, ez, er);
-    if (rec) {
+    if constexpr (REC) {
       rz_write_row(e.rec + (size_t)i * p.rec_rows * p.rec_cols, p.rec_mask, tob, x, y, z, vx, vy,
                    vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
       nrec = 1;
@@ -284,175 +325,214 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
   int fate = IG_FATE_FLYING, face = -1, label = 0;
   int64_t s = 0;
   const int64_t lim = p.max_steps;
+  const int coll_batch = p.coll_batch;
+  int stage = kStep;
+  // the step's start point, for the impact bisection and the crossings
+  double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
   for (;;) {
-    if (!(t < t_max)) {
-      fate = IG_FATE_TIMEOUT;
-      break;
-    }
-    if (lim > 0 && s >= lim) break;
-    const double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
-    double r = sqrt(y * y + z * z);
-    efield(x, r, tob + t, ez, er);
-    double rr = r > 1e-9 ? r : 1e-9;
-    if constexpr (GAS == IG_GAS_SDS) {
-      vx += 0.5 * sds_fac * (ez - (vx - vgx) * sds_damp) * dt;
-      vy += 0.5 * sds_fac * (er * y / rr - (vy - vgy) * sds_damp) * dt;
-      vz += 0.5 * sds_fac * (er * z / rr - (vz - vgz) * sds_damp) * dt;
-    } else {
-      vx += 0.5 * ez * dt;
-      vy += 0.5 * er * y / rr * dt;
-      vz += 0.5 * er * z / rr * dt;
-    }
-    x += vx * dt;
-    y += vy * dt;
-    z += vz * dt;
-    if (rec) path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
-    r = sqrt(y * y + z * z);
-    efield(x, r, tob + t + dt, ez, er);
-    rr = r > 1e-9 ? r : 1e-9;
-    if constexpr (GAS == IG_GAS_SDS) {
-      vx += 0.5 * sds_fac * (ez - (vx - vgx) * sds_damp) * dt;
-      vy += 0.5 * sds_fac * (er * y / rr - (vy - vgy) * sds_damp) * dt;
-      vz += 0.5 * sds_fac * (er * z / rr - (vz - vgz) * sds_damp) * dt;
-    } else {
-      vx += 0.5 * ez * dt;
-      vy += 0.5 * er * y / rr * dt;
-      vz += 0.5 * er * z / rr * dt;
-    }
-    t += dt;
-    s += 1;
-    if (rec) {
-      double sp = sqrt(vx * vx + vy * vy + vz * vz);
-      eat += ez * fscale * dt;
-      ket += ke_scale * sp * sp * dt;
-    }
-    if constexpr (GAS == IG_GAS_SDS) {
-      if (sds_mfp > 0.0) {
-        double pre = 0.0;
-        if (rec) pre = sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
-        double jx, jy, jz;
-        sds_jump(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
-        x += jx;
-        y += jy;
-        z += jz;
-        if (rec)
-          path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo)) - pre;
+    if (stage == kStep) {
+      if (!(t < t_max)) {
+        fate = IG_FATE_TIMEOUT;
+        stage = kDone;
+      } else if (lim > 0 && s >= lim) {
+        stage = kDone; // window ends, the ion keeps flying
+      } else {
+        xo = x;
+        yo = y;
+        zo = z;
+        vxo = vx;
+        vyo = vy;
+        vzo = vz;
+        to = t;
+        double r = sqrt(y * y + z * z);
+        efield(x, r, tob + t, ez, er);
+        double rr = r > 1e-9 ? r : 1e-9;
+        if constexpr (GAS == IG_GAS_SDS) {
+          vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
+          vy += 0.5 * sds_fac * (er * y / rr - (vy - p.vg[1]) * sds_damp) * dt;
+          vz += 0.5 * sds_fac * (er * z / rr - (vz - p.vg[2]) * sds_damp) * dt;
+        } else {
+          vx += 0.5 * ez * dt;
+          vy += 0.5 * er * y / rr * dt;
+          vz += 0.5 * er * z / rr * dt;
+        }
+        x += vx * dt;
+        y += vy * dt;
+        z += vz * dt;
+        if constexpr (REC)
+          path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+        r = sqrt(y * y + z * z);
+        efield(x, r, tob + t + dt, ez, er);
+        rr = r > 1e-9 ? r : 1e-9;
+        if constexpr (GAS == IG_GAS_SDS) {
+          vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
+          vy += 0.5 * sds_fac * (er * y / rr - (vy - p.vg[1]) * sds_damp) * dt;
+          vz += 0.5 * sds_fac * (er * z / rr - (vz - p.vg[2]) * sds_damp) * dt;
+        } else {
+          vx += 0.5 * ez * dt;
+          vy += 0.5 * er * y / rr * dt;
+          vz += 0.5 * er * z / rr * dt;
+        }
+        t += dt;
+        s += 1;
+        if constexpr (REC) {
+          const double ke_scale = 0.5 * m_ion * IG_KG_AMU * 1e6 / IG_E_CHG;
+          double sp = sqrt(vx * vx + vy * vy + vz * vz);
+          eat += ez * fscale * dt;
+          ket += ke_scale * sp * sp * dt;
+        }
+        stage = kPost;
+        if constexpr (GAS == IG_GAS_SDS) {
+          if (sds_mfp > 0.0) {
+            double pre = 0.0;
+            if constexpr (REC)
+              pre = sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+            double jx, jy, jz;
+            if constexpr (PREC == IG_PREC_F32)
+              sds_jump_fast(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
+            else
+              sds_jump(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
+            x += jx;
+            y += jy;
+            z += jz;
+            if constexpr (REC)
+              path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo)) - pre;
+          }
+        } else if constexpr (GAS == IG_GAS_HS) {
+          if (PREC == IG_PREC_F32 ? hs_test_fast(rng, vx, vy, vz, p, dt, 1.0, sigma)
+                                  : hs_test(rng, vx, vy, vz, p, dt, 1.0, sigma))
+            stage = kCollide;
+        }
       }
-    } else if constexpr (GAS == IG_GAS_HS) {
-      int hit = hs_step(rng, vx, vy, vz, p, dt, 1.0, sigma, m_ion);
-      if (hit < 0) {
-        fate = -3;
-        break;
-      }
-      ncol += hit;
     }
-    if (rec && (steps0 + s) % p.rec_every == 0 && nrec < p.rec_rows - 1) {
+    if constexpr (GAS == IG_GAS_HS) {
+      const uint64_t parked = __ballot(stage == kCollide);
+      if (parked) {
+        const uint64_t moving = __ballot(stage == kStep || stage == kPost);
+        if (__popcll(parked) >= coll_batch || moving == 0) {
+          if (stage == kCollide) {
+            if (hs_scatter(rng, vx, vy, vz, p, m_ion)) {
+              ncol += 1;
+              stage = kPost;
+            } else {
+              fate = -3; // no partner accepted: refused, as the reference raises
+              stage = kDone;
+            }
+          }
+        }
+      }
+    }
+    if (stage == kPost) {
+      stage = kStep;
+      if constexpr (REC) {
+        if ((steps0 + s) % p.rec_every == 0 && nrec < p.rec_rows - 1) {
+          rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t,
+                       x, y, z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat,
+                       ket);
+          nrec += 1;
+        }
+      }
+      const double r = sqrt(y * y + z * z);
+      if (metal_nn(f.metal, x / mm, (r - u0) / mm, nx, nu)) {
+        double f0 = 0.0, f1 = 1.0;
+        for (int b = 0; b < 40; ++b) {
+          double fm = 0.5 * (f0 + f1);
+          double ym = yo + fm * (y - yo);
+          double zm = zo + fm * (z - zo);
+          double rm = sqrt(ym * ym + zm * zm);
+          if (metal_nn(f.metal, (xo + fm * (x - xo)) / mm, (rm - u0) / mm, nx, nu))
+            f1 = fm;
+          else
+            f0 = fm;
+        }
+        if constexpr (REC)
+          path -=
+              (1.0 - f1) * sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+        x = xo + f1 * (x - xo);
+        y = yo + f1 * (y - yo);
+        z = zo + f1 * (z - zo);
+        vx = vxo + f1 * (vx - vxo);
+        vy = vyo + f1 * (vy - vyo);
+        vz = vzo + f1 * (vz - vzo);
+        t = to + f1 * dt;
+        fate = IG_FATE_IMPACT;
+        if (f.label) {
+          double re = sqrt(y * y + z * z);
+          double li = floor(x / mm + 0.5), lj = floor((re - u0) / mm + 0.5);
+          if (li >= 0.0 && lj >= 0.0 && li <= (double)(nx - 1) && lj <= (double)(nu - 1))
+            label = f.label[(int64_t)li * nu + (int64_t)lj];
+        }
+        stage = kDone;
+      } else if (x < 0 || r / mm > (double)(nu - 2)) {
+        fate = IG_FATE_BOX;
+        stage = kDone;
+      } else {
+        for (int ip = 0; ip < p.n_pl; ++ip) {
+          const int pcol = p.pl_col[ip];
+          const double cn = pcol == 0 ? x : (pcol == 1 ? y : z);
+          const double co = pcol == 0 ? xo : (pcol == 1 ? yo : zo);
+          const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
+          bool crossed;
+          if (sgn == 0.0)
+            crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
+          else
+            crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
+          if (!crossed) continue;
+          const double den = cn - co;
+          double fr = den == 0.0 ? 0.0 : (val - co) / den;
+          if (fr < 0.0) fr = 0.0;
+          if (fr > 1.0) fr = 1.0;
+          const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
+          const double w1 = pcol == 0 ? yc : xc;
+          const double w2 = pcol == 2 ? yc : zc;
+          const bool ins = p.pl_w[ip][0] <= w1 && w1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= w2 &&
+                           w2 <= p.pl_w[ip][3];
+          if (p.pl_kind[ip] == IG_FATE_DETECT) {
+            if (!ins) continue;
+          } else if (ins) {
+            continue;
+          }
+          if constexpr (REC)
+            path -= (1.0 - fr) *
+                    sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
+          x = xc;
+          y = yc;
+          z = zc;
+          vx = vxo + fr * (vx - vxo);
+          vy = vyo + fr * (vy - vyo);
+          vz = vzo + fr * (vz - vzo);
+          t = to + fr * dt;
+          fate = p.pl_kind[ip];
+          face = ip;
+          stage = kDone;
+          break;
+        }
+        if (stage != kDone) {
+#pragma unroll
+          for (int fc = 0; fc < 6; ++fc) {
+            if (stage == kDone || !p.bnd_on[fc]) continue;
+            const double v = fc < 2 ? x : (fc < 4 ? y : z);
+            if ((fc & 1) ? v > p.bnd_val[fc] : v < p.bnd_val[fc]) {
+              fate = IG_FATE_BOUND;
+              face = fc;
+              stage = kDone;
+            }
+          }
+        }
+        if (stage != kDone && x / mm > (double)(nx - 1)) {
+          fate = IG_FATE_BOX;
+          stage = kDone;
+        }
+      }
+    }
+    if (__ballot(stage != kDone) == 0) break;
+  }
+  if constexpr (REC) {
+    if (fate != IG_FATE_FLYING) {
       rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t, x,
                    y, z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
       nrec += 1;
     }
-    r = sqrt(y * y + z * z);
-    if (metal_nn(f.metal, x / mm, (r - u0) / mm, nx, nu)) {
-      double f0 = 0.0, f1 = 1.0;
-      for (int b = 0; b < 40; ++b) {
-        double fm = 0.5 * (f0 + f1);
-        double ym = yo + fm * (y - yo);
-        double zm = zo + fm * (z - zo);
-        double rm = sqrt(ym * ym + zm * zm);
-        if (metal_nn(f.metal, (xo + fm * (x - xo)) / mm, (rm - u0) / mm, nx, nu))
-          f1 = fm;
-        else
-          f0 = fm;
-      }
-      if (rec)
-        path -= (1.0 - f1) * sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
-      x = xo + f1 * (x - xo);
-      y = yo + f1 * (y - yo);
-      z = zo + f1 * (z - zo);
-      vx = vxo + f1 * (vx - vxo);
-      vy = vyo + f1 * (vy - vyo);
-      vz = vzo + f1 * (vz - vzo);
-      t = to + f1 * dt;
-      fate = IG_FATE_IMPACT;
-      if (f.label) {
-        double re = sqrt(y * y + z * z);
-        double li = floor(x / mm + 0.5), lj = floor((re - u0) / mm + 0.5);
-        if (li >= 0.0 && lj >= 0.0 && li <= (double)(nx - 1) && lj <= (double)(nu - 1))
-          label = f.label[(int64_t)li * nu + (int64_t)lj];
-      }
-      break;
-    }
-    if (x < 0 || r / mm > (double)(nu - 2)) {
-      fate = IG_FATE_BOX;
-      break;
-    }
-    if (p.n_pl > 0) {
-      bool hit_pl = false;
-      for (int ip = 0; ip < p.n_pl; ++ip) {
-        const int pcol = p.pl_col[ip];
-        const double cn = pcol == 0 ? x : (pcol == 1 ? y : z);
-        const double co = pcol == 0 ? xo : (pcol == 1 ? yo : zo);
-        const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
-        bool crossed;
-        if (sgn == 0.0)
-          crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
-        else
-          crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
-        if (!crossed) continue;
-        const double den = cn - co;
-        double fr = den == 0.0 ? 0.0 : (val - co) / den;
-        if (fr < 0.0) fr = 0.0;
-        if (fr > 1.0) fr = 1.0;
-        const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
-        const double w1 = pcol == 0 ? yc : xc;
-        const double w2 = pcol == 2 ? yc : zc;
-        const bool ins = p.pl_w[ip][0] <= w1 && w1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= w2 &&
-                         w2 <= p.pl_w[ip][3];
-        if (p.pl_kind[ip] == IG_FATE_DETECT) {
-          if (!ins) continue;
-        } else if (ins) {
-          continue;
-        }
-        if (rec)
-          path -=
-              (1.0 - fr) * sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
-        x = xc;
-        y = yc;
-        z = zc;
-        vx = vxo + fr * (vx - vxo);
-        vy = vyo + fr * (vy - vyo);
-        vz = vzo + fr * (vz - vzo);
-        t = to + fr * dt;
-        fate = p.pl_kind[ip];
-        face = ip;
-        hit_pl = true;
-        break;
-      }
-      if (hit_pl) break;
-    }
-    {
-      bool stop = false;
-#pragma unroll
-      for (int fc = 0; fc < 6; ++fc) {
-        if (stop || !p.bnd_on[fc]) continue;
-        const double v = fc < 2 ? x : (fc < 4 ? y : z);
-        if ((fc & 1) ? v > p.bnd_val[fc] : v < p.bnd_val[fc]) {
-          fate = IG_FATE_BOUND;
-          face = fc;
-          stop = true;
-        }
-      }
-      if (stop) break;
-    }
-    if (x / mm > (double)(nx - 1)) {
-      fate = IG_FATE_BOX;
-      break;
-    }
-  }
-  if (fate != IG_FATE_FLYING && rec) {
-    rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t, x, y,
-                 z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
-    nrec += 1;
   }
   e.x[i] = x;
   e.y[i] = y;
@@ -472,10 +552,28 @@ __global__ __launch_bounds__(256) void rz_fly_kernel(const FieldsK f, const Flig
   e.gauss[i] = rng.gauss;
   e.e_last[3 * i] = ez;
   e.e_last[3 * i + 1] = er;
-  if (rec) {
+  if constexpr (REC) {
     e.path[i] = path;
     e.eat[i] = eat;
     e.ket[i] = ket;
+  }
+}
+
+template <int PREC, int NS, int GAS>
+static void launch_rz_gas(const FieldsK &f, const FlightK &p, const EnsK &e, dim3 grid, int block,
+                          hipStream_t st) {
+  if (p.rec_rows > 0 && e.rec) {
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, true, 1>), grid, dim3(block), 0, st, f, p,
+                       e);
+  } else if (GAS == IG_GAS_HS && p.occupancy == 2) {
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, false, 2>), grid, dim3(block), 0, st, f, p,
+                       e);
+  } else if (GAS == IG_GAS_HS && p.occupancy == 3) {
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, false, 3>), grid, dim3(block), 0, st, f, p,
+                       e);
+  } else {
+    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, false, 1>), grid, dim3(block), 0, st, f, p,
+                       e);
   }
 }
 
@@ -484,12 +582,11 @@ static int launch_rz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int64
                         int block, hipStream_t st) {
   dim3 grid((unsigned)((n_launch + block - 1) / block));
   if (p.gas == IG_GAS_HS)
-    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, IG_GAS_HS>), grid, dim3(block), 0, st, f, p, e);
+    launch_rz_gas<PREC, NS, IG_GAS_HS>(f, p, e, grid, block, st);
   else if (p.gas == IG_GAS_SDS)
-    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, IG_GAS_SDS>), grid, dim3(block), 0, st, f, p, e);
+    launch_rz_gas<PREC, NS, IG_GAS_SDS>(f, p, e, grid, block, st);
   else
-    hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, IG_GAS_VACUUM>), grid, dim3(block), 0, st, f, p,
-                       e);
+    launch_rz_gas<PREC, NS, IG_GAS_VACUUM>(f, p, e, grid, block, st);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

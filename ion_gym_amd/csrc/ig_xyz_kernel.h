@@ -40,8 +40,8 @@ struct RecLayout3 {
 // e = tri(E_A) then e = e + w_k * tri(E_k) for k = 1..K.
 template <int PREC, int NS>
 __device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy, double gz,
-                                          const double *w, int K, double &ex, double &ey,
-                                          double &ez) {
+                                          const double *w, int K, const double *tscale,
+                                          double &ex, double &ey, double &ez) {
   using T = typename StoreT3<PREC>::T;
   using L = RecLayout3<T, NS>;
   const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
@@ -92,6 +92,11 @@ __device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy
         if constexpr (PREC == IG_PREC_F32_STORE) r = r * f.scale[s];
         v[c] = r;
       }
+    }
+    if (tscale) {
+      v[0] = v[0] * tscale[s];
+      v[1] = v[1] * tscale[s];
+      v[2] = v[2] * tscale[s];
     }
     if (s == 0) {
       e[0] = v[0];
@@ -147,7 +152,19 @@ __global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const Fli
   }
   double t = e.t[i];
   const double tob = e.tob[i], qm = e.acc[i], m_amu = e.m_amu[i];
-  const double dt = p.dt, t_max = p.t_max, h = f.h, inv_h = f.inv_h;
+  const double h = f.h, inv_h = f.inv_h;
+  const Waves *W = f.waves;
+  double dt = p.dt, t_max = p.t_max;
+  uint64_t rkey = p.rng_key;
+  const double *tscale = nullptr;
+  if (e.tunes) {
+    const Tune *T = e.tunes + e.tune[i];
+    W = &T->w;
+    dt = T->dt;
+    t_max = T->t_max;
+    rkey = T->rng_key;
+    tscale = T->scale;
+  }
   const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
   const int K = f.K;
   const bool rec = p.rec_rows > 0 && e.rec != nullptr;
@@ -155,7 +172,7 @@ __global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const Fli
   int32_t nrec = e.nrec[i];
   Rng rng;
   {
-    uint64_t key = (uint64_t)e.seed[i] ^ p.rng_key;
+    uint64_t key = (uint64_t)e.seed[i] ^ rkey;
     rng.k0 = (uint32_t)key;
     rng.k1 = (uint32_t)(key >> 32);
     rng.n = e.rng_n[i];
@@ -170,14 +187,16 @@ __global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const Fli
     if (!(tt == wtt)) {
 #pragma unroll
       for (int k = 0; k < NW; ++k)
-        if (k < K) w[k] = wave_eval<true>(f, k, tt);
+        if (k < K)
+          w[k] = PREC == IG_PREC_F32 ? wave_eval_fast<true>(W, f, k, tt)
+                                     : wave_eval<true>(W, f, k, tt);
       wtt = tt;
     }
   };
   auto efield = [&](double px, double py, double pz, double tt, double &a, double &b,
                     double &c) {
     weights(tt);
-    xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, a, b, c);
+    xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
   };
   auto write = [&](int row, double tus) {
     double *r = e.rec + ((size_t)i * p.rec_rows + row) * p.rec_cols;
@@ -250,7 +269,8 @@ __global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const Fli
     s += 1;
     if constexpr (GAS == IG_GAS_HS) {
       double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
-      int hit = hs_step(rng, sx, sy, sz, p, dt, 1e6, sigma, m_amu);
+      int hit = PREC == IG_PREC_F32 ? hs_step_fast(rng, sx, sy, sz, p, dt, 1e6, sigma, m_amu)
+                                    : hs_step(rng, sx, sy, sz, p, dt, 1e6, sigma, m_amu);
       if (hit < 0) {
         fate = -3;
         break;

@@ -174,7 +174,8 @@ class Flight:
     3-D s."""
 
     def __init__(self, dt, t_max, *, max_steps=0, rec_every=1, rec_rows=0,
-                 channels=(), gas=None, bounds=None, planes=(), rng_key=0):
+                 channels=(), gas=None, bounds=None, planes=(), rng_key=0,
+                 coll_batch=0, occupancy=0):
         self.dt = float(dt)
         self.t_max = float(t_max)
         self.max_steps = int(max_steps)
@@ -185,6 +186,8 @@ class Flight:
         self.bounds = bounds
         self.planes = list(planes)
         self.rng_key = int(rng_key)
+        self.coll_batch = int(coll_batch)
+        self.occupancy = int(occupancy)
 
     def struct(self, keep):
         p = L.FlightParams()
@@ -222,6 +225,8 @@ class Flight:
                 p.pl_w[i][c] = float(win[c])
             p.pl_kind[i] = int(kind)
         p.rng_key = self.rng_key & 0xFFFFFFFFFFFFFFFF
+        p.coll_batch = self.coll_batch
+        p.occupancy = self.occupancy
         return p
 
 

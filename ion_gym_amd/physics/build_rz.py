@@ -1,0 +1,165 @@
+"""Engine-backed r-z fly function: the drop-in for make_rz_fly_fn.
+
+ion_gym/physics/build_rz.py:561-777 returns ``fly_fn(i) -> (traj,
+summary)`` that flies ion i through the numba kernel _fly_rec_full.  This
+module returns a function with the same signature and results, flown by the
+MI355X engine, plus ``fly_fn.batch(indices)`` that flies many ions in one
+launch; ion_gym_amd.physics.ensemble_driver takes that path when it is
+present (the reference's sim_build propagates fly-function attributes,
+sim_build.py:834-842).
+"""
+import math
+import threading
+
+import numpy as np
+
+from .. import _lib as L
+from ..engine import BASE_CHANNELS, RZ_CHANNELS, Engine, Flight
+from .collision3d import gas_kernel_scalars
+from .constants import E_CHG, KG_AMU
+from .ion_envelope import (birth_splat, make_summary, per_ion,
+                           rz_birth_refusal)
+
+_ENGINES = {}
+_ENGINES_LOCK = threading.Lock()
+
+
+def shared_engine(device=0):
+    """One engine per device and process, created on first use."""
+    with _ENGINES_LOCK:
+        e = _ENGINES.get(device)
+        if e is None:
+            e = _ENGINES[device] = Engine(device)
+        return e
+
+
+# bytes of record buffer one launch may fill before the batch is split
+RECORD_BUDGET_BYTES = 1 << 30
+
+
+def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
+    """(fly_fn, col_names) for an RZModel, births (N, 7) and a FlightSpec.
+    Refusals match the reference: an unknown record channel, charge 0."""
+    chans = [c for c in RZ_CHANNELS if c in spec.integration.record_channels]
+    bad = [c for c in spec.integration.record_channels
+           if c not in RZ_CHANNELS]
+    if bad:
+        raise ValueError(f"r-z tracer cannot record channel(s) {bad}; the "
+                         f"r-z kernel produces {list(RZ_CHANNELS)}")
+    col_names = BASE_CHANNELS + chans
+    q_e = int(spec.source.charge)
+    if q_e == 0:
+        raise ValueError("source.charge = 0: an uncharged ion has no "
+                         "electric acceleration; state the charge state.")
+    col = spec.collisions if (spec.collisions is not None
+                              and spec.collisions.enabled) else None
+    sds_on = col is not None and col.model.lower() == "sds"
+    if col is not None and col.model.lower() not in ("hs", "sds"):
+        raise ValueError(f"collisions.model {col.model!r} is not a known "
+                         f"model; the flight routes implement 'hs' and 'sds'")
+    g = gas_kernel_scalars(col.gas if col else None,
+                           T_k=col.T_k if col else None,
+                           P_torr=col.P_torr if col else None)
+    gas = dict(model=(L.GAS_SDS if sds_on else L.GAS_HS) if col else
+               L.GAS_VACUUM,
+               T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+               flow=tuple(col.flow_mm_us) if col else (0.0, 0.0, 0.0))
+    if sds_on:
+        from . import sds as sds_mod
+        gas["sds_stats"] = (sds_stats if sds_stats is not None
+                            else sds_mod.default_statistics())
+    dt = spec.integration.dt_ns * 1e-3
+    t_max = spec.integration.t_max_us
+    rec_every = max(int(spec.integration.rec_every), 1)
+    n_rows = int(t_max / dt / rec_every) + 8
+    n_rows = min(max(n_rows, 16), 4_000_000)
+    flight = Flight(dt, t_max, rec_every=rec_every, rec_rows=n_rows,
+                    channels=chans, gas=gas,
+                    bounds=spec.bounds.as_tuple(), planes=spec.planes)
+    births = np.asarray(births, np.float64)
+    seed_base = spec.source.seed if spec.source.seed is not None else 0
+    state = {}
+
+    def _fields():
+        if "fp" not in state:
+            eng = shared_engine(device)
+            state["eng"] = eng
+            state["fp"] = eng.upload_rz(model.pack(), spec.precision)
+        return state["eng"], state["fp"]
+
+    def _ion_arrays(idx):
+        n = len(idx)
+        b = births[idx]
+        env = [per_ion(spec.source.mz_list, spec.source.n_ions, i, seed_base)
+               for i in idx]
+        m = np.array([e.mz for e in env])
+        ions = dict(x=b[:, 0], y=b[:, 1], z=b[:, 2], vx=b[:, 3], vy=b[:, 4],
+                    vz=b[:, 5], tob=b[:, 6], m_amu=m,
+                    acc=q_e * E_CHG / (m * KG_AMU),
+                    sigma_m2=np.full(n, col.sigma_m2 if col else 0.0),
+                    seed=np.array([e.seed for e in env], np.int64))
+        if sds_on:
+            from . import sds as sds_mod
+            rows = []
+            for mi in m:
+                p = sds_mod.transport_params(mi, q_e, g["m_gas"], col.T_k,
+                                             col.P_torr, col.K0_cm2_per_Vs,
+                                             stats=gas["sds_stats"])
+                rows.append((p["damping"], p["mfp_mm"], p["V_mm_us"],
+                             p["log_mr_ratio"]))
+            ions["sds"] = np.array(rows)
+        return ions, env
+
+    def batch(indices):
+        """[(traj, summary)] for the given ion indices, flown together."""
+        idx = np.asarray(list(indices), np.int64)
+        out = [None] * len(idx)
+        fly_idx = []
+        for k, i in enumerate(idx):
+            b = births[i]
+            r = math.hypot(b[1], b[2])
+            err = rz_birth_refusal(model.ele, model.mm_per_gu, b[0], r, i)
+            if err is not None:
+                out[k] = birth_splat(err, col_names=col_names, birth=b,
+                                     mz=per_ion(spec.source.mz_list,
+                                                spec.source.n_ions, i,
+                                                seed_base).mz,
+                                     label=f"ion {i}")
+            else:
+                fly_idx.append(k)
+        if not fly_idx:
+            return out
+        eng, fp = _fields()
+        per_ion_bytes = n_rows * len(col_names) * 8
+        chunk = max(1, min(len(fly_idx), RECORD_BUDGET_BYTES // per_ion_bytes))
+        for c0 in range(0, len(fly_idx), chunk):
+            ks = fly_idx[c0:c0 + chunk]
+            ions, env = _ion_arrays(idx[ks])
+            res = eng.fly_batch(fp, flight, ions)
+            if np.any(res["fate"] == -3):
+                j = int(np.nonzero(res["fate"] == -3)[0][0])
+                raise ValueError(f"ion {int(idx[ks][j])}: hard-sphere "
+                                 f"collision: no gas partner accepted in "
+                                 f"MAX_PARTNER_TRIES draws; the ion velocity "
+                                 f"or the gas thermal speed is not finite")
+            for j, k in enumerate(ks):
+                nrec = int(res["nrec"][j])
+                traj = res["records"][j, :nrec].copy()
+                if sds_on and "n_col" in col_names:
+                    traj[:, col_names.index("n_col")] = np.nan
+                end = traj[-1]
+                out[k] = (traj, make_summary(
+                    kind=int(res["fate"][j]), tof=float(end[0]),
+                    mz=env[j].mz, x_end=float(end[1]), y_end=float(end[2]),
+                    z_end=float(end[3]),
+                    r_end=float(math.hypot(end[2], end[3])),
+                    n_col=None if sds_on else int(res["ncol"][j])))
+        return out
+
+    def fly_fn(i):
+        return batch([i])[0]
+
+    fly_fn.batch = batch
+    fly_fn.col_names = col_names
+    return fly_fn, col_names

@@ -46,7 +46,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     import subprocess
     structs = {"ig_fields_desc": _lib.FieldsDesc,
                "ig_flight_params": _lib.FlightParams,
-               "ig_ions": _lib.Ions, "ig_results": _lib.Results}
+               "ig_ions": _lib.Ions, "ig_results": _lib.Results,
+               "ig_tune": _lib.Tune}
     lines = ["#include <stdio.h>", "#include <stddef.h>",
              '#include "ion_gym_amd.h"', "int main(void) {"]
     for cname, py in structs.items():

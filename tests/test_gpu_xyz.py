@@ -1,0 +1,112 @@
+"""GPU parity of the 3-D RK4 kernel against the CPU oracle (tracer3d.py
+_fly3d restated): FP64 ion by ion to 1e-9 relative with identical fates;
+FP32 storage against the oracle on float32-rounded grids; the FP32-basis
+mode's error against FP64 stated; hard-sphere gas ion by ion on the shared
+Philox stream."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from test_gpu_rz import _close
+
+pytestmark = pytest.mark.gpu
+
+
+def _parity(g, o, rel=1e-9, frac_ok=1.0):
+    ok = (g["fate"] == o["kind"]) & (g["steps"] == o["steps"])
+    ok &= g["ncol"] == o["ncol"]
+    for k in ("x", "y", "z"):
+        ok &= _close(g[k], o[k], 1.0) <= rel
+    vs = np.sqrt(o["vx"] ** 2 + o["vy"] ** 2 + o["vz"] ** 2) + 1e-30
+    for k in ("vx", "vy", "vz"):
+        ok &= _close(g[k], o[k], vs) <= rel
+    ok &= _close(g["t_us"], o["t"], 1e-6) <= rel
+    i = int(np.argmin(ok)) if not ok.all() else 0
+    assert ok.mean() >= frac_ok, (
+        f"{(~ok).sum()} ions differ; ion {i}: gpu fate {g['fate'][i]} steps "
+        f"{g['steps'][i]} x {g['x'][i]!r} | oracle {o['kind'][i]} "
+        f"{o['steps'][i]} {o['x'][i]!r}")
+
+
+@pytest.fixture(scope="module")
+def quad():
+    from ion_gym_amd import configs
+    return configs.quad_3d(n_ions=300, coarse=4, n_cycles=20)
+
+
+def _f(c, **kw):
+    from ion_gym_amd.engine import Flight
+    return Flight(c["dt"], c["t_max"], gas=c.get("gas"), **kw)
+
+
+def test_quad_fp64_matches_oracle(engine, quad):
+    c = quad
+    fp = engine.upload_xyz(c["pack"], "f64")
+    g = engine.fly_batch(fp, _f(c), c["ions"])
+    o = O.fly_xyz(c["pack"], dict(dt=c["dt"], t_max=c["t_max"]), c["ions"])
+    _parity(g, o)
+    assert len(set(np.unique(g["fate"]))) >= 1
+    fp.release()
+
+
+def test_quad_f32_store_matches_oracle_on_rounded_grids(engine, quad):
+    c = quad
+    p32 = dict(c["pack"])
+    for k in ("EAx", "EAy", "EAz", "ExK", "EyK", "EzK"):
+        p32[k] = c["pack"][k].astype(np.float32).astype(np.float64)
+    fp = engine.upload_xyz(c["pack"], "f32_store")
+    g = engine.fly_batch(fp, _f(c), c["ions"])
+    o = O.fly_xyz(p32, dict(dt=c["dt"], t_max=c["t_max"]), c["ions"])
+    _parity(g, o)
+    fp.release()
+
+
+def test_quad_records_match_oracle(engine, quad):
+    c = quad
+    ions = {k: v[:50] for k, v in c["ions"].items()}
+    rows = 600
+    fp = engine.upload_xyz(c["pack"], "f64")
+    g = engine.fly_batch(fp, _f(c, rec_every=10, rec_rows=rows), ions)
+    o = O.fly_xyz(c["pack"], dict(dt=c["dt"], t_max=c["t_max"],
+                                  rec_every=10, rec_rows=rows), ions)
+    _parity(g, o)
+    assert np.array_equal(g["nrec"], o["nrec"])
+    for i in range(50):
+        n = int(o["nrec"][i])
+        a, b = g["records"][i, :n], o["rec"][i, :n]
+        scale = np.maximum(np.abs(b).max(axis=0), 1e-6)
+        assert np.all(np.abs(a - b) <= 1e-9 * scale), i
+    fp.release()
+
+
+def test_quad_fp32_basis_error_against_fp64(engine, quad):
+    """FP32 storage, interpolation and waveforms: positions after 20 RF
+    cycles within 1e-4 mm of the FP64 oracle for ions that share its
+    fate."""
+    c = quad
+    fp = engine.upload_xyz(c["pack"], "f32")
+    g = engine.fly_batch(fp, _f(c), c["ions"])
+    o = O.fly_xyz(c["pack"], dict(dt=c["dt"], t_max=c["t_max"]), c["ions"])
+    same = g["fate"] == o["kind"]
+    assert same.mean() > 0.97
+    for k in ("x", "y", "z"):
+        assert np.max(np.abs(g[k][same] - o[k][same])) < 1e-4, k
+    fp.release()
+
+
+def test_quad_hard_sphere_matches_oracle_philox(engine, quad):
+    from ion_gym_amd.physics.collision3d import gas_kernel_scalars
+    c = dict(quad)
+    g0 = gas_kernel_scalars("N2", T_k=300.0, P_torr=0.05)
+    c["gas"] = dict(model=1, **{k: g0[k] for k in ("T_k", "P_pa", "c_star",
+                                                   "c_bar", "sig1d",
+                                                   "m_gas")})
+    ions = dict(c["ions"])
+    ions["sigma_m2"] = np.full(len(ions["x"]), 2e-18)
+    fp = engine.upload_xyz(c["pack"], "f64")
+    g = engine.fly_batch(fp, _f(c), ions)
+    o = O.fly_xyz(c["pack"], dict(dt=c["dt"], t_max=c["t_max"],
+                                  gas=c["gas"], rng=O.RNG_PHILOX), ions)
+    assert o["ncol"].sum() > 100
+    _parity(g, o, frac_ok=0.99)
+    fp.release()

@@ -254,3 +254,96 @@ def test_sds_mean_square_jump_is_6Ddt():
         p["mfp_mm"] ** 2
     assert np.mean(rj2) == pytest.approx(6 * p["D_mm2_per_us"] * dt,
                                          rel=1e-3)
+
+
+# ----------------------------------------------------------- 3-D RK4
+def _uniform_xyz(shape, h, chans):
+    """3-D pack with zero static field and uniform per-channel fields:
+    chans = [(axis, amplitude V/mm, kind, om, ph, amp, off)]."""
+    z = np.zeros(shape)
+    K = len(chans)
+    ExK = np.zeros((K,) + shape)
+    EyK = np.zeros((K,) + shape)
+    EzK = np.zeros((K,) + shape)
+    for k, (ax, E, *_r) in enumerate(chans):
+        (ExK, EyK, EzK)[ax][k] = E
+    return dict(EAx=z, EAy=z.copy(), EAz=z.copy(), ExK=ExK, EyK=EyK,
+                EzK=EzK, ele=np.zeros(shape, bool), h_mm=h,
+                ch_kind=np.array([c[2] for c in chans], np.int64),
+                ch_om=np.array([c[3] for c in chans], float),
+                ch_ph=np.array([c[4] for c in chans], float),
+                ch_amp=np.array([c[5] for c in chans], float),
+                ch_off=np.array([c[6] for c in chans], float),
+                ch_duty=np.full(K, 0.5))
+
+
+def _qm(mz):
+    return 1.602176634e-19 / (mz * 1.66053906660e-27)
+
+
+def test_rk4_two_frequency_closed_form():
+    # ion_gym tests/test_multidrive3d.py:84-125 (gate 2): uniform Ex at
+    # w0, uniform Ey at w1; x(t) = (a/W)[t - sin(W t)/W] per axis
+    n, h = 31, 1.0
+    A0, A1, w0, w1 = 0.8, 0.5, 0.30, 1.10
+    pack = _uniform_xyz((n, n, 5), h, [(0, 1.0, 0, w0, 0.0, A0, 0.0),
+                                       (1, 1.0, 0, w1, 0.0, A1, 0.0)])
+    mz = 50.0
+    cx = (n - 1) / 2 * h
+    ions = dict(x=np.array([cx]), y=np.array([cx]), z=np.array([2 * h]),
+                vx=np.zeros(1), vy=np.zeros(1), vz=np.zeros(1),
+                tob=np.zeros(1), m_amu=np.array([mz]),
+                acc=np.array([_qm(mz)]), seed=np.zeros(1, np.int64))
+    r = O.fly_xyz(pack, dict(dt=0.5e-9, t_max=6.0e-6), ions)
+    t = (r["t"][0]) * 1e-6
+    a0, a1 = _qm(mz) * 1e3 * A0, _qm(mz) * 1e3 * A1
+    W0, W1 = w0 * 1e6, w1 * 1e6
+    xp = (a0 / W0) * (t - math.sin(W0 * t) / W0) * 1e3 + cx
+    yp = (a1 / W1) * (t - math.sin(W1 * t) / W1) * 1e3 + cx
+    assert abs(r["x"][0] - xp) / abs(xp - cx) < 1e-4
+    assert abs(r["y"][0] - yp) / abs(yp - cx) < 1e-4
+    assert r["z"][0] == 2 * h
+    assert r["kind"][0] == 2
+
+
+def test_rk4_exact_in_a_uniform_static_field():
+    # a uniform field makes the motion quadratic, which RK4 integrates
+    # exactly: affine channel at frequency 0 with offset = the field
+    n, h = 21, 1.0
+    pack = _uniform_xyz((n, n, n), h, [(2, 1.0, 0, 0.0, 0.0, 0.0, 0.3)])
+    mz = 80.0
+    ions = dict(x=np.full(2, 10.0), y=np.full(2, 10.0), z=np.full(2, 2.0),
+                vx=np.array([0.1, -0.2]), vy=np.zeros(2),
+                vz=np.array([0.5, 1.0]), tob=np.zeros(2),
+                m_amu=np.full(2, mz), acc=np.full(2, _qm(mz)),
+                seed=np.zeros(2, np.int64))
+    r = O.fly_xyz(pack, dict(dt=2e-9, t_max=3e-6), ions)
+    a = _qm(mz) * 1e3 * 0.3  # m/s^2
+    for i in range(2):
+        t = r["t"][i] * 1e-6
+        z = 2.0 + ions["vz"][i] * 1e3 * t * 1e3 + 0.5 * a * t * t * 1e3
+        assert r["z"][i] == pytest.approx(z, rel=1e-11)
+        assert r["vz"][i] == pytest.approx(ions["vz"][i] + a * t * 1e-3,
+                                           rel=1e-11)
+        assert r["x"][i] == pytest.approx(10.0 + ions["vx"][i] * 1e3 * t *
+                                          1e3, rel=1e-11)
+
+
+def test_rk4_metal_impact_all_eight_corners_and_box():
+    # a metal slab at k >= 12: the all-corners rule fires inside the slab
+    # (first cell with all eight corners metal), the 40-step bisection puts
+    # the end point on that cell's face; an ion flying out of +x leaves
+    # the box (kind 1)
+    n, h = 21, 1.0
+    pack = _uniform_xyz((n, n, n), h, [])
+    pack["ele"][:, :, 12:] = True
+    ions = dict(x=np.array([10.0, 10.0]), y=np.array([10.0, 10.0]),
+                z=np.array([5.0, 5.0]), vx=np.array([0.0, 3.0]),
+                vy=np.zeros(2), vz=np.array([2.0, 0.0]), tob=np.zeros(2),
+                m_amu=np.full(2, 100.0), acc=np.full(2, _qm(100.0)),
+                seed=np.zeros(2, np.int64))
+    r = O.fly_xyz(pack, dict(dt=1e-8, t_max=20e-6), ions)
+    assert r["kind"][0] == 0
+    assert r["z"][0] == pytest.approx(12.0, abs=1e-9)
+    assert r["kind"][1] == 1
+    assert r["x"][1] > (n - 1) * h
