@@ -222,3 +222,93 @@ def quad_3d(n_ions=100_000, seed=2, coarse=1, a=0.02, q_ref_mz=500.0,
     return dict(name="quad_3d", route="xyz", pack=pack, ions=ions,
                 dt=dt_ns * 1e-9, t_max=n_cycles / f_hz, gas=None, U=U, q=q,
                 a=a)
+
+
+def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
+             K0=1.5, P_torr=4.0, T_k=300.0, dt_frac=0.1):
+    """Drift tube with SDS: a 100-ring stack (0.4 mm rings at 0.8 mm pitch,
+    inner radius 8 mm) on an 801 x 101 r-z grid @ 0.1 mm whose linear
+    ladder gives a uniform 1 V/mm base field; N2 at 4 Torr and 300 K; the
+    ion's declared K0 = 1.5 cm^2/(V s); births on the axis region at
+    x = 5 mm with 300 K velocities; end plates at 80 V and 0 V close the
+    stack (the exit plate at x = 80 mm is the detector).  The field
+    strength E/N (Td) sets the
+    static set's scale (``set_scale``); dt = dt_frac / gamma."""
+    from .physics import sds as sds_mod
+    h = 0.1 * coarse
+    nz, nr = 800 // coarse + 1, 100 // coarse + 1
+    elec = []
+    for i in range(100):
+        z0 = 0.2 + 0.8 * i
+        elec.append((i + 1, z0, z0 + 0.4, 8.0, 10.0))
+    # entrance and exit plates close the stack, so the 1 V/mm ladder field
+    # is uniform from end to end; the exit plate is the detector
+    elec.append((101, 0.0, 0.0, 0.0, 10.0))
+    elec.append((102, 80.0, 80.0, 0.0, 10.0))
+    lab = _grid_labels(nz, nr, h, elec)
+    # 1 V/mm: V(z) = 80 - z
+    dc = {i + 1: 80.0 - (elec[i][1] + 0.2) for i in range(100)}
+    dc[101] = 80.0
+    dc[102] = 0.0
+    (A,) = solve_rz(lab, [dc], h)
+    model = _model_from_potentials(lab, h, A, [], [], [], [], [])
+    n_gas = P_torr * (101325.0 / 760.0) / (K_B * T_k)   # 1/m^3
+    E_vpm = e_td * 1e-21 * n_gas                         # V/m
+    scale = E_vpm * 1e-3 / 1.0                           # base is 1 V/mm
+    g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
+    stats = sds_mod.default_statistics()
+    p = sds_mod.transport_params(mz, 1, g["m_gas"], T_k, P_torr, K0,
+                                 stats=stats)
+    dt = dt_frac / p["damping"]
+    rng = np.random.default_rng(seed)
+    m = np.full(n_ions, mz)
+    rr = 0.5 * np.sqrt(rng.uniform(0, 1, n_ions))
+    th = rng.uniform(0, 2 * math.pi, n_ions)
+    s1 = np.sqrt(K_B * T_k / (m * KG_AMU)) * 1e-3
+    v = rng.normal(0.0, 1.0, (3, n_ions)) * s1
+    ions = dict(x=np.full(n_ions, 5.0), y=rr * np.cos(th), z=rr * np.sin(th),
+                vx=v[0], vy=v[1], vz=v[2], tob=np.zeros(n_ions), m_amu=m,
+                acc=E_CHG / (m * KG_AMU), sigma_m2=np.zeros(n_ions),
+                seed=seed + np.arange(n_ions, dtype=np.int64),
+                sds=np.tile([p["damping"], p["mfp_mm"], p["V_mm_us"],
+                             p["log_mr_ratio"]], (n_ions, 1)))
+    gas = dict(model=2, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+               flow=(0.0, 0.0, 0.0), sds_stats=stats)
+    return dict(name="drift_rz", route="rz", model=model, ions=ions, dt=dt,
+                t_max=2000.0, gas=gas, scale=scale, E_vpmm=E_vpm * 1e-3,
+                K_mm2_per_V_us=p["K_mm2_per_V_us"], D=p["D_mm2_per_us"],
+                transport=p)
+
+
+def hs_drift_rz(n_ions=2000, seed=5000, coarse=2, e_vpmm=1.0, mz=500.0,
+                gas="He", P_torr=1.0, T_k=300.0, sigma_m2=2.0e-18,
+                x_birth=76.0, dt_ns=None):
+    """The drift tube of drift_rz with hard-sphere gas: uniform field
+    e_vpmm V/mm (set_scale of the 1 V/mm ladder), births on the axis region
+    at x_birth with thermal velocities, the exit plate at x = 80 mm as the
+    detector (arrival = impact on it, fate 0).  dt = tau_collision / 20
+    unless given."""
+    base = drift_rz(n_ions=1, coarse=coarse)
+    model = base["model"]
+    g = gas_mod.gas_kernel_scalars(gas, T_k=T_k, P_torr=P_torr)
+    tau_ns = gas_mod.collision_time_ns(gas, T_k, P_torr, sigma_m2)
+    if dt_ns is None:
+        dt_ns = tau_ns / gas_mod.HS_DT_PER_COLLISION_TIME
+    rng = np.random.default_rng(seed)
+    m = np.full(n_ions, mz)
+    rr = 0.3 * np.sqrt(rng.uniform(0, 1, n_ions))
+    th = rng.uniform(0, 2 * math.pi, n_ions)
+    s1 = np.sqrt(K_B * T_k / (m * KG_AMU)) * 1e-3
+    v = rng.normal(0.0, 1.0, (3, n_ions)) * s1
+    ions = dict(x=np.full(n_ions, x_birth), y=rr * np.cos(th),
+                z=rr * np.sin(th), vx=v[0], vy=v[1], vz=v[2],
+                tob=np.zeros(n_ions), m_amu=m, acc=E_CHG / (m * KG_AMU),
+                sigma_m2=np.full(n_ions, sigma_m2),
+                seed=seed + np.arange(n_ions, dtype=np.int64))
+    gas_d = dict(model=1, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+                 c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+                 flow=(0.0, 0.0, 0.0))
+    return dict(name="hs_drift_rz", route="rz", model=model, ions=ions,
+                dt=dt_ns * 1e-3, t_max=200.0, gas=gas_d, scale=e_vpmm,
+                tau_ns=tau_ns)

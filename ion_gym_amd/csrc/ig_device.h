@@ -48,6 +48,10 @@ struct Rng {
   uint64_t n;
   int32_t has_gauss;
   double gauss;
+  // words (2, 3) of block (n - 1) / 2 when n is odd: the next draw's pair,
+  // kept from the block that produced the previous draw (cached = true)
+  uint32_t w2, w3;
+  bool cached;
 };
 
 // a double in [0, 1) from two 32-bit words, 27 + 26 bits (the construction
@@ -57,12 +61,20 @@ __device__ __forceinline__ double words_to_u01(uint32_t a, uint32_t b) {
 }
 
 __device__ __forceinline__ double rng_uniform(Rng &r) {
-  uint64_t blk = r.n >> 1;
-  bool hi = (r.n & 1u) != 0;
+  const uint64_t blk = r.n >> 1;
+  const bool hi = (r.n & 1u) != 0;
   r.n += 1;
+  if (hi && r.cached) {
+    r.cached = false;
+    return words_to_u01(r.w2, r.w3);
+  }
   uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), 0u, 0u};
   philox4x32_10(c, r.k0, r.k1);
-  return hi ? words_to_u01(c[2], c[3]) : words_to_u01(c[0], c[1]);
+  if (hi) return words_to_u01(c[2], c[3]);
+  r.w2 = c[2];
+  r.w3 = c[3];
+  r.cached = true;
+  return words_to_u01(c[0], c[1]);
 }
 
 // polar-method normal with the second value of each pair cached, the
@@ -351,6 +363,198 @@ __device__ __forceinline__ int hs_step_fast(Rng &rng, double &vx, double &vy, do
                                             double sigma, double m_ion) {
   if (!hs_test_fast(rng, vx, vy, vz, p, dt, dt_to_us, sigma)) return 0;
   return hs_scatter(rng, vx, vy, vz, p, m_ion) ? 1 : -1;
+}
+
+// The hard-sphere collision of the FP32-basis mode: the reference's
+// algorithm (collision3d.py:281-330: rejection-biased Maxwell partner from
+// polar-method normals, impact angle asin(sqrt(U)), azimuth 2 pi U,
+// line-of-centres attenuation) with the partner draw, the acceptance test,
+// the angles and the unit vectors in FP32; the random stream, its draw
+// order and the final velocity update stay as in FP64.
+__device__ __forceinline__ float rng_normal_f(Rng &r) {
+  if (r.has_gauss) {
+    r.has_gauss = 0;
+    return (float)r.gauss;
+  }
+  float x1, x2, r2;
+  int tries = 0;
+  do {
+    x1 = (float)(2.0 * rng_uniform(r) - 1.0);
+    x2 = (float)(2.0 * rng_uniform(r) - 1.0);
+    r2 = x1 * x1 + x2 * x2;
+  } while ((r2 >= 1.0f || r2 == 0.0f) && ++tries < 200);
+  if (r2 >= 1.0f || r2 == 0.0f) r2 = 0.5f;
+  const float f = sqrtf(-2.0f * logf(r2) / r2);
+  r.gauss = (double)(f * x1);
+  r.has_gauss = 1;
+  return f * x2;
+}
+
+__device__ __forceinline__ bool hs_scatter_fast(Rng &rng, double &vx, double &vy, double &vz,
+                                                const FlightK &p, double m_ion) {
+  const double wx = vx - p.vg[0], wy = vy - p.vg[1], wz = vz - p.vg[2];
+  double sp = sqrt(wx * wx + wy * wy + wz * wz);
+  if (sp < 1e-7) sp = 1e-7;
+  const float sig = (float)p.sig1d;
+  const float scale = (float)sp + sig * 5.196152422706632f;
+  const float fwx = (float)wx, fwy = (float)wy, fwz = (float)wz;
+  float rx = 0.f, ry = 0.f, rz = 0.f, rlen = 0.f;
+  bool accepted = false;
+  for (int tr = 0; tr < 10000; ++tr) {
+    const float gx = rng_normal_f(rng) * sig;
+    const float gy = rng_normal_f(rng) * sig;
+    const float gz = rng_normal_f(rng) * sig;
+    rx = fwx - gx;
+    ry = fwy - gy;
+    rz = fwz - gz;
+    rlen = sqrtf(rx * rx + ry * ry + rz * rz);
+    if (rng_uniform(rng) < (double)(rlen / scale)) {
+      accepted = true;
+      break;
+    }
+  }
+  if (!accepted) return false;
+  if (rlen < 1e-12f) return true;
+  // sin(asin(sqrt(a U))) = sqrt(a U), cos = sqrt(1 - a U)
+  const float au = (float)(0.999999999 * rng_uniform(rng));
+  const float sa = sqrtf(au), ca = sqrtf(1.0f - au);
+  float st, ct;
+  sincospif((float)(2.0 * rng_uniform(rng)), &st, &ct);
+  const float inv = 1.0f / rlen;
+  const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
+  float px, py, pz;
+  if (fabsf(ux) < 0.9f) {
+    px = 0.0f;
+    py = -uz;
+    pz = uy;
+  } else {
+    px = -uz;
+    py = 0.0f;
+    pz = ux;
+  }
+  const float pinv = 1.0f / sqrtf(px * px + py * py + pz * pz);
+  px *= pinv;
+  py *= pinv;
+  pz *= pinv;
+  const float qx = uy * pz - uz * py, qy = uz * px - ux * pz, qz = ux * py - uy * px;
+  const float rhx = ca * ux + sa * (ct * px + st * qx);
+  const float rhy = ca * uy + sa * (ct * py + st * qy);
+  const float rhz = ca * uz + sa * (ct * pz + st * qz);
+  const double f = 2.0 * p.m_gas / (m_ion + p.m_gas) *
+                   (double)(rx * rhx + ry * rhy + rz * rhz);
+  vx = (wx - f * (double)rhx) + p.vg[0];
+  vy = (wy - f * (double)rhy) + p.vg[1];
+  vz = (wz - f * (double)rhz) + p.vg[2];
+  return true;
+}
+
+// ---- hard sphere, FP32-basis mode, flattened into the lane state machine
+// The FP32 mode keeps the reference's collision model (collision3d.py:
+// 281-374: exact-Maxwell collision test, flux-weighted Maxwell partner by
+// rejection with acceptance |w - g| / (sp + 3 sqrt(3) sig1d), impact angle
+// asin(sqrt(U)), uniform azimuth, line-of-centres attenuation) and draws
+// the partner normals by Box-Muller instead of the polar method, so a
+// rejection try is straight-line code: a lane performs one try per pass of
+// its kernel loop while the other lanes keep stepping, instead of the
+// whole wave looping until its unluckiest lane accepts.
+struct HsTry {
+  float wx, wy, wz; // ion velocity in the gas frame
+  float scale;      // sp + 3 sqrt(3) sig1d
+  int tries;
+};
+
+// per-ion constant of the collision exponent: dt P sigma / (1000 kB T)
+__device__ __forceinline__ float hs_rate_const(const FlightK &p, double dt, double dt_to_us,
+                                               double sigma) {
+  return (float)(dt * dt_to_us * p.P_pa * sigma / (1000.0 * IG_KB * p.T_k));
+}
+
+// the collision test (FP32 rate); on a hit fills the try state
+__device__ __forceinline__ bool hs_test_flat(Rng &rng, double vx, double vy, double vz,
+                                             const FlightK &p, float kconst, float inv_c_star,
+                                             HsTry &st) {
+  const double wx = vx - p.vg[0], wy = vy - p.vg[1], wz = vz - p.vg[2];
+  double sp = sqrt(wx * wx + wy * wy + wz * wz);
+  if (sp < 1e-7) sp = 1e-7;
+  const float s = (float)sp * inv_c_star;
+  const float c_bar = (float)p.c_bar;
+  float cbr;
+  if (s < 1e-12f) {
+    cbr = c_bar;
+  } else {
+    const float sqrt_pi = 1.7724538509055159f;
+    cbr = c_bar * ((s + 0.5f / s) * 0.5f * sqrt_pi * erff(s) + 0.5f * expf(-s * s));
+  }
+  const float prob = -expm1f(-kconst * cbr);
+  if (!(rng_uniform(rng) < (double)prob)) return false;
+  st.wx = (float)wx;
+  st.wy = (float)wy;
+  st.wz = (float)wz;
+  st.scale = (float)sp + (float)p.sig1d * 5.196152422706632f;
+  st.tries = 0;
+  return true;
+}
+
+// Box-Muller normal with the second value of each pair cached
+__device__ __forceinline__ float rng_normal_bm(Rng &r) {
+  if (r.has_gauss) {
+    r.has_gauss = 0;
+    return (float)r.gauss;
+  }
+  const float u1 = (float)(1.0 - rng_uniform(r)); // (0, 1]
+  const float u2 = (float)rng_uniform(r);
+  const float rad = sqrtf(-2.0f * logf(u1));
+  float sn, cs;
+  sincospif(2.0f * u2, &sn, &cs);
+  r.gauss = (double)(rad * sn);
+  r.has_gauss = 1;
+  return rad * cs;
+}
+
+// One partner try.  Returns 1 when the collision is complete (velocity
+// updated), 0 to try again, -1 when MAX_PARTNER_TRIES ran out.
+__device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, double &vx, double &vy,
+                                           double &vz, const FlightK &p, double m_ion) {
+  const float sig = (float)p.sig1d;
+  const float gx = rng_normal_bm(rng) * sig;
+  const float gy = rng_normal_bm(rng) * sig;
+  const float gz = rng_normal_bm(rng) * sig;
+  const float rx = st.wx - gx, ry = st.wy - gy, rz = st.wz - gz;
+  const float rlen = sqrtf(rx * rx + ry * ry + rz * rz);
+  if (!(rng_uniform(rng) < (double)(rlen / st.scale))) {
+    return ++st.tries >= 10000 ? -1 : 0;
+  }
+  if (rlen < 1e-12f) return 1;
+  const float au = (float)(0.999999999 * rng_uniform(rng));
+  const float sa = sqrtf(au), ca = sqrtf(1.0f - au);
+  float sth, cth;
+  sincospif((float)(2.0 * rng_uniform(rng)), &sth, &cth);
+  const float inv = 1.0f / rlen;
+  const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
+  float px, py, pz;
+  if (fabsf(ux) < 0.9f) {
+    px = 0.0f;
+    py = -uz;
+    pz = uy;
+  } else {
+    px = -uz;
+    py = 0.0f;
+    pz = ux;
+  }
+  const float pinv = 1.0f / sqrtf(px * px + py * py + pz * pz);
+  px *= pinv;
+  py *= pinv;
+  pz *= pinv;
+  const float qx = uy * pz - uz * py, qy = uz * px - ux * pz, qz = ux * py - uy * px;
+  const float rhx = ca * ux + sa * (cth * px + sth * qx);
+  const float rhy = ca * uy + sa * (cth * py + sth * qy);
+  const float rhz = ca * uz + sa * (cth * pz + sth * qz);
+  const double f = 2.0 * p.m_gas / (m_ion + p.m_gas) *
+                   (double)(rx * rhx + ry * rhy + rz * rhz);
+  vx = (vx - p.vg[0] - f * (double)rhx) + p.vg[0];
+  vy = (vy - p.vg[1] - f * (double)rhy) + p.vg[1];
+  vz = (vz - p.vg[2] - f * (double)rhz) + p.vg[2];
+  return 1;
 }
 
 // the SDS jump with the table interpolation, the log-log mass-ratio blend

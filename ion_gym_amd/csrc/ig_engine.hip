@@ -42,7 +42,10 @@ thread_local std::string g_err;
 // parked hard-sphere lanes a wave scatters together when the caller leaves
 // coll_batch at 0 (the autotuner measures the best value per workload)
 constexpr int kDefaultCollBatch = 16;
-constexpr int kDefaultOccupancy = 2;
+// default register-budget variant of the hard-sphere kernel per precision
+// (measured on the funnel workload: FP32 best at 3 waves/SIMD, FP64 and
+// FP32-storage at the compiler's own budget)
+inline int default_occupancy(int prec) { return prec == IG_PREC_F32 ? 3 : 1; }
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -709,7 +712,7 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
   k.coll_batch = p->coll_batch > 0 ? p->coll_batch : kDefaultCollBatch;
   if (p->occupancy < 0 || p->occupancy > 3)
     return fail(IG_E_ARG, "ig_fly: occupancy must be 0 (default), 1, 2 or 3");
-  k.occupancy = p->occupancy > 0 ? p->occupancy : kDefaultOccupancy;
+  k.occupancy = p->occupancy;
   return 0;
 }
 
@@ -743,6 +746,7 @@ int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensembl
   }
   const FieldsK &f = fit->second->k;
   E->route = f.route;
+  if (k.occupancy == 0) k.occupancy = default_occupancy(f.prec);
   HIPX(hipEventRecord(h->ev0, h->stream));
   int rc = launch_route(f, k, E->k, E->n, h->block, h->stream);
   HIPX(hipEventRecord(h->ev1, h->stream));

@@ -242,6 +242,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     rng.n = e.rng_n[i];
     rng.has_gauss = e.has_gauss[i];
     rng.gauss = e.gauss[i];
+    rng.cached = false;
   }
   double path = 0.0, eat = 0.0, ket = 0.0;
   if constexpr (REC) {
@@ -262,15 +263,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     }
   }
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
+  // FP32-basis hard sphere: collision exponent constant and the
+  // flattened partner-try state
+  constexpr bool kFlat = PREC == IG_PREC_F32 && GAS == IG_GAS_HS;
+  float hs_k = 0.0f, hs_inv_cstar = 0.0f;
+  HsTry hst{};
+  if constexpr (kFlat) {
+    hs_k = hs_rate_const(p, dt, 1.0, sigma);
+    hs_inv_cstar = (float)(1.0 / p.c_star);
+  }
 
   // cached gather and waveform weights (see the file comment)
   double cz[NS], cr[NS];
   double cgx = __builtin_nan(""), cgu = __builtin_nan("");
   double wk[NS > 1 ? NS - 1 : 1];
   double wtt = __builtin_nan("");
+  // FP32-basis mode: node coordinates by the reciprocal pitch (the FP64
+  // modes divide, as the reference does)
+  const double inv_mm = 1.0 / mm;
   auto efield = [&](double px, double pr, double tt, double &ez_o, double &er_o) {
-    const double gx = px / mm;
-    const double gu = (pr - u0) / mm;
+    const double gx = PREC == IG_PREC_F32 ? px * inv_mm : px / mm;
+    const double gu = PREC == IG_PREC_F32 ? (pr - u0) * inv_mm : (pr - u0) / mm;
     if (!(gx == cgx && gu == cgu)) {
       rz_gather<PREC, NS>(f, gx, gu, cz, cr);
       if (tscale) {
@@ -327,6 +340,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   const int64_t lim = p.max_steps;
   const int coll_batch = p.coll_batch;
   int stage = kStep;
+  // sqrt(y^2 + z^2) of the current position while it stays valid (the
+  // reference takes it three times per step at two positions; only the
+  // drift, the SDS jump and the terminal interpolations move the ion)
+  bool rvalid = false;
+  double rcur = 0.0;
   // the step's start point, for the impact bisection and the crossings
   double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
   for (;;) {
@@ -344,7 +362,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         vyo = vy;
         vzo = vz;
         to = t;
-        double r = sqrt(y * y + z * z);
+        double r = rvalid ? rcur : sqrt(y * y + z * z);
         efield(x, r, tob + t, ez, er);
         double rr = r > 1e-9 ? r : 1e-9;
         if constexpr (GAS == IG_GAS_SDS) {
@@ -362,6 +380,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         if constexpr (REC)
           path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
         r = sqrt(y * y + z * z);
+        rcur = r;
+        rvalid = true;
         efield(x, r, tob + t + dt, ez, er);
         rr = r > 1e-9 ? r : 1e-9;
         if constexpr (GAS == IG_GAS_SDS) {
@@ -395,17 +415,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
             x += jx;
             y += jy;
             z += jz;
+            rvalid = false;
             if constexpr (REC)
               path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo)) - pre;
           }
+        } else if constexpr (kFlat) {
+          if (hs_test_flat(rng, vx, vy, vz, p, hs_k, hs_inv_cstar, hst)) stage = kCollide;
         } else if constexpr (GAS == IG_GAS_HS) {
-          if (PREC == IG_PREC_F32 ? hs_test_fast(rng, vx, vy, vz, p, dt, 1.0, sigma)
-                                  : hs_test(rng, vx, vy, vz, p, dt, 1.0, sigma))
-            stage = kCollide;
+          if (hs_test(rng, vx, vy, vz, p, dt, 1.0, sigma)) stage = kCollide;
         }
       }
     }
-    if constexpr (GAS == IG_GAS_HS) {
+    if constexpr (kFlat) {
+      // one partner try per pass for every lane whose collision is due
+      if (stage == kCollide) {
+        const int rc = hs_try_flat(rng, hst, vx, vy, vz, p, m_ion);
+        if (rc > 0) {
+          ncol += 1;
+          stage = kPost;
+        } else if (rc < 0) {
+          fate = -3; // no partner accepted: refused, as the reference raises
+          stage = kDone;
+        }
+      }
+    } else if constexpr (GAS == IG_GAS_HS) {
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
@@ -432,7 +465,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
           nrec += 1;
         }
       }
-      const double r = sqrt(y * y + z * z);
+      const double r = rvalid ? rcur : sqrt(y * y + z * z);
       if (metal_nn(f.metal, x / mm, (r - u0) / mm, nx, nu)) {
         double f0 = 0.0, f1 = 1.0;
         for (int b = 0; b < 40; ++b) {

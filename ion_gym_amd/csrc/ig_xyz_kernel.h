@@ -178,6 +178,7 @@ __global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const Fli
     rng.n = e.rng_n[i];
     rng.has_gauss = e.has_gauss[i];
     rng.gauss = e.gauss[i];
+    rng.cached = false;
   }
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
   constexpr int NW = NS > 1 ? NS - 1 : 1;

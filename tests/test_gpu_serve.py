@@ -1,0 +1,81 @@
+"""The continuous-batching flight service: requests of different tune
+arriving while others fly are admitted as slots free, and each request's
+result is the same flown alone and flown in the mix."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(c, k_req, per, seed0):
+    from ion_gym_amd.serve import FlightRequest, make_tune
+    out = []
+    for k in range(k_req):
+        sl = slice(k * per, (k + 1) * per)
+        ions = {a: np.asarray(v)[sl].copy() for a, v in c["ions"].items()}
+        ions["seed"] = ions["seed"] + 1000 * k
+        # each request its own RF amplitude (drive-set scales) and key
+        tune = make_tune(c["model"].pack(), dt=c["dt"], t_max=0.4,
+                         rng_key=seed0 + k,
+                         set_scale=[1.0, 0.6 + 0.2 * k, 0.6 + 0.2 * k])
+        out.append(FlightRequest(ions=ions, tune=tune))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_result_is_the_same_alone_and_in_the_mix(engine, prec):
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    from ion_gym_amd.serve import FlightServer, latency_stats
+    c = configs.funnel_rz(n_ions=6 * 2000, seed=3, coarse=2)
+    fp = engine.upload_rz(c["model"].pack(), prec)
+    fl = Flight(c["dt"], c["t_max"], gas=c["gas"])
+    # alone: each request on its own fresh server
+    alone = []
+    for req in _requests(c, 6, 2000, 77):
+        srv = FlightServer(engine, fp, fl, capacity=2000, window=300)
+        srv.submit(req)
+        srv.drain()
+        alone.append(req.result)
+        srv.release()
+    # in the mix: a slab for three at a time, windows of a different
+    # length, requests submitted while others fly
+    srv = FlightServer(engine, fp, fl, capacity=6000, window=170)
+    reqs = _requests(c, 6, 2000, 77)
+    for r in reqs[:3]:
+        srv.submit(r)
+    srv.step()
+    srv.step()
+    for r in reqs[3:]:
+        srv.submit(r)
+    srv.drain()
+    assert len(srv.done) == 6
+    for r, a in zip(reqs, alone):
+        for key in ("x", "y", "z", "vx", "vy", "vz", "t_us", "fate", "ncol",
+                    "steps"):
+            assert np.array_equal(r.result[key], a[key]), (r.rid, key)
+        assert r.result["ncol"].sum() > 0
+    st = latency_stats(srv.done)
+    assert st["n"] == 6 and st["p99_s"] >= st["p50_s"] > 0
+    srv.release()
+    fp.release()
+
+
+def test_admission_waits_for_free_slots(engine):
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    from ion_gym_amd.serve import FlightServer
+    c = configs.funnel_rz(n_ions=3 * 1500, seed=3, coarse=2)
+    fp = engine.upload_rz(c["model"].pack(), "f32")
+    srv = FlightServer(engine, fp, Flight(c["dt"], c["t_max"], gas=c["gas"]),
+                       capacity=3000, window=200)
+    reqs = _requests(c, 3, 1500, 5)
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    assert len(srv.active) == 2 and len(srv.pending) == 1
+    srv.drain()
+    assert all(r.result is not None for r in reqs)
+    assert reqs[2].t_admit >= min(reqs[0].t_done, reqs[1].t_done)
+    srv.release()
+    fp.release()
