@@ -13,6 +13,9 @@
 
 namespace ig {
 
+// per-lane stage of the flight kernels' step machines
+enum : int { kStep = 0, kCollide = 1, kPost = 2, kDone = 3 };
+
 #define IG_KB 1.380649e-23         // ion_gym/physics/constants.py:19
 #define IG_E_CHG 1.602176634e-19   // ion_gym/physics/constants.py:18
 #define IG_KG_AMU 1.66053906660e-27 // ion_gym/physics/constants.py:23

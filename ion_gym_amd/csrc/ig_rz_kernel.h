@@ -174,8 +174,6 @@ __device__ __noinline__ void rz_write_row(double *row, uint32_t mask, double t, 
   if (mask & IG_REC_KE_TINT) row[c++] = ket;
 }
 
-// Per-lane stage of the step machine below.
-enum : int { kStep = 0, kCollide = 1, kPost = 2, kDone = 3 };
 
 // The flight kernel.  Each lane runs a small state machine per step:
 //   kStep    the leapfrog step (kick, drift, kick), t += dt, the record

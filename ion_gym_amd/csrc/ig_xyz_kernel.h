@@ -127,9 +127,9 @@ __device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, doubl
   return a[0] && a[di] && a[dj] && a[di + dj] && a[1] && a[di + 1] && a[dj + 1] && a[di + dj + 1];
 }
 
-template <int PREC, int NS, int GAS>
-__global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const FlightK p,
-                                                      const EnsK e) {
+template <int PREC, int NS, int GAS, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void xyz_fly_kernel(
+    const FieldsK f, const FlightK p, const EnsK e) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i;
   if (e.idx) {
@@ -223,145 +223,200 @@ __global__ __launch_bounds__(256) void xyz_fly_kernel(const FieldsK f, const Fli
   int fate = IG_FATE_FLYING, hit_ip = -1;
   int64_t s = 0;
   const int64_t lim = p.max_steps;
+  // the step machine of the r-z kernel (kStep / kCollide / kPost): hard-
+  // sphere collisions are scattered together by the wave (FP64, the
+  // reference's algorithm) or one partner try per pass (FP32-basis mode)
+  constexpr bool kFlat = PREC == IG_PREC_F32 && GAS == IG_GAS_HS;
+  float hs_k = 0.0f, hs_inv_cstar = 0.0f;
+  HsTry hst{};
+  if constexpr (kFlat) {
+    hs_k = hs_rate_const(p, dt, 1e6, sigma);
+    hs_inv_cstar = (float)(1.0 / p.c_star);
+  }
+  const int coll_batch = p.coll_batch;
+  int stage = kStep;
+  double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
   for (;;) {
-    if (!(t < t_max)) {
-      fate = IG_FATE_TIMEOUT;
-      break;
-    }
-    if (lim > 0 && s >= lim) break;
-    const double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
-    const double t_us = tob + t * 1e6;
-    const double h_us = dt * 1e6;
-    double ax1, ay1, az1, ax2, ay2, az2, ax3, ay3, az3, ax4, ay4, az4, ea, eb, ec;
-    efield(x, y, z, t_us, ea, eb, ec);
-    ax1 = qm * 1e3 * ea;
-    ay1 = qm * 1e3 * eb;
-    az1 = qm * 1e3 * ec;
-    const double k1x = vx, k1y = vy, k1z = vz;
-    double px = x + 0.5 * dt * k1x * c, py = y + 0.5 * dt * k1y * c, pz = z + 0.5 * dt * k1z * c;
-    efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
-    ax2 = qm * 1e3 * ea;
-    ay2 = qm * 1e3 * eb;
-    az2 = qm * 1e3 * ec;
-    const double k2x = vx + 0.5 * dt * ax1, k2y = vy + 0.5 * dt * ay1, k2z = vz + 0.5 * dt * az1;
-    px = x + 0.5 * dt * k2x * c;
-    py = y + 0.5 * dt * k2y * c;
-    pz = z + 0.5 * dt * k2z * c;
-    efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
-    ax3 = qm * 1e3 * ea;
-    ay3 = qm * 1e3 * eb;
-    az3 = qm * 1e3 * ec;
-    const double k3x = vx + 0.5 * dt * ax2, k3y = vy + 0.5 * dt * ay2, k3z = vz + 0.5 * dt * az2;
-    px = x + dt * k3x * c;
-    py = y + dt * k3y * c;
-    pz = z + dt * k3z * c;
-    efield(px, py, pz, t_us + h_us, ea, eb, ec);
-    ax4 = qm * 1e3 * ea;
-    ay4 = qm * 1e3 * eb;
-    az4 = qm * 1e3 * ec;
-    const double k4x = vx + dt * ax3, k4y = vy + dt * ay3, k4z = vz + dt * az3;
-    x = x + dt / 6.0 * (k1x + 2 * k2x + 2 * k3x + k4x) * c;
-    y = y + dt / 6.0 * (k1y + 2 * k2y + 2 * k3y + k4y) * c;
-    z = z + dt / 6.0 * (k1z + 2 * k2z + 2 * k3z + k4z) * c;
-    vx = vx + dt / 6.0 * (ax1 + 2 * ax2 + 2 * ax3 + ax4);
-    vy = vy + dt / 6.0 * (ay1 + 2 * ay2 + 2 * ay3 + ay4);
-    vz = vz + dt / 6.0 * (az1 + 2 * az2 + 2 * az3 + az4);
-    t += dt;
-    s += 1;
-    if constexpr (GAS == IG_GAS_HS) {
-      double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
-      int hit = PREC == IG_PREC_F32 ? hs_step_fast(rng, sx, sy, sz, p, dt, 1e6, sigma, m_amu)
-                                    : hs_step(rng, sx, sy, sz, p, dt, 1e6, sigma, m_amu);
-      if (hit < 0) {
-        fate = -3;
-        break;
-      }
-      if (hit) {
-        vx = sx * 1e3;
-        vy = sy * 1e3;
-        vz = sz * 1e3;
-        ncol += 1;
-      }
-    }
-    if (in_metal_3d(f.metal, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
-      double f0 = 0.0, f1 = 1.0;
-      for (int b = 0; b < 40; ++b) {
-        double fm = 0.5 * (f0 + f1);
-        if (in_metal_3d(f.metal, (xo + fm * (x - xo)) * inv_h, (yo + fm * (y - yo)) * inv_h,
-                        (zo + fm * (z - zo)) * inv_h, nx, ny, nz))
-          f1 = fm;
-        else
-          f0 = fm;
-      }
-      const double fr = f1;
-      x = xo + fr * (x - xo);
-      y = yo + fr * (y - yo);
-      z = zo + fr * (z - zo);
-      vx = vxo + fr * (vx - vxo);
-      vy = vyo + fr * (vy - vyo);
-      vz = vzo + fr * (vz - vzo);
-      t = to + fr * dt;
-      fate = IG_FATE_IMPACT;
-      if (nrec < R) write(nrec, tob + t * 1e6);
-      if (rec && nrec < R) nrec++;
-      break;
-    }
-    if (p.n_pl > 0) {
-      bool hit_pl = false;
-      for (int ip = 0; ip < p.n_pl; ++ip) {
-        const int col = p.pl_col[ip];
-        const double cn = col == 0 ? x : (col == 1 ? y : z);
-        const double co = col == 0 ? xo : (col == 1 ? yo : zo);
-        const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
-        bool crossed;
-        if (sgn == 0.0)
-          crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
-        else
-          crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
-        if (!crossed) continue;
-        const double den = cn - co;
-        double fr = den == 0.0 ? 0.0 : (val - co) / den;
-        if (fr < 0.0) fr = 0.0;
-        if (fr > 1.0) fr = 1.0;
-        const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
-        const double t1 = col == 0 ? yc : xc;
-        const double t2 = col == 2 ? yc : zc;
-        const bool ins = p.pl_w[ip][0] <= t1 && t1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= t2 &&
-                         t2 <= p.pl_w[ip][3];
-        if (p.pl_kind[ip] == IG_FATE_DETECT) {
-          if (!ins) continue;
-        } else if (ins) {
-          continue;
+    if (stage == kStep) {
+      if (!(t < t_max)) {
+        fate = IG_FATE_TIMEOUT;
+        stage = kDone;
+      } else if (lim > 0 && s >= lim) {
+        stage = kDone;
+      } else {
+        xo = x;
+        yo = y;
+        zo = z;
+        vxo = vx;
+        vyo = vy;
+        vzo = vz;
+        to = t;
+        const double t_us = tob + t * 1e6;
+        const double h_us = dt * 1e6;
+        double ax1, ay1, az1, ax2, ay2, az2, ax3, ay3, az3, ax4, ay4, az4, ea, eb, ec;
+        efield(x, y, z, t_us, ea, eb, ec);
+        ax1 = qm * 1e3 * ea;
+        ay1 = qm * 1e3 * eb;
+        az1 = qm * 1e3 * ec;
+        const double k1x = vx, k1y = vy, k1z = vz;
+        double px = x + 0.5 * dt * k1x * c, py = y + 0.5 * dt * k1y * c,
+               pz = z + 0.5 * dt * k1z * c;
+        efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
+        ax2 = qm * 1e3 * ea;
+        ay2 = qm * 1e3 * eb;
+        az2 = qm * 1e3 * ec;
+        const double k2x = vx + 0.5 * dt * ax1, k2y = vy + 0.5 * dt * ay1,
+                     k2z = vz + 0.5 * dt * az1;
+        px = x + 0.5 * dt * k2x * c;
+        py = y + 0.5 * dt * k2y * c;
+        pz = z + 0.5 * dt * k2z * c;
+        efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
+        ax3 = qm * 1e3 * ea;
+        ay3 = qm * 1e3 * eb;
+        az3 = qm * 1e3 * ec;
+        const double k3x = vx + 0.5 * dt * ax2, k3y = vy + 0.5 * dt * ay2,
+                     k3z = vz + 0.5 * dt * az2;
+        px = x + dt * k3x * c;
+        py = y + dt * k3y * c;
+        pz = z + dt * k3z * c;
+        efield(px, py, pz, t_us + h_us, ea, eb, ec);
+        ax4 = qm * 1e3 * ea;
+        ay4 = qm * 1e3 * eb;
+        az4 = qm * 1e3 * ec;
+        const double k4x = vx + dt * ax3, k4y = vy + dt * ay3, k4z = vz + dt * az3;
+        x = x + dt / 6.0 * (k1x + 2 * k2x + 2 * k3x + k4x) * c;
+        y = y + dt / 6.0 * (k1y + 2 * k2y + 2 * k3y + k4y) * c;
+        z = z + dt / 6.0 * (k1z + 2 * k2z + 2 * k3z + k4z) * c;
+        vx = vx + dt / 6.0 * (ax1 + 2 * ax2 + 2 * ax3 + ax4);
+        vy = vy + dt / 6.0 * (ay1 + 2 * ay2 + 2 * ay3 + ay4);
+        vz = vz + dt / 6.0 * (az1 + 2 * az2 + 2 * az3 + az4);
+        t += dt;
+        s += 1;
+        stage = kPost;
+        // HS in mm/us (tracer3d.py:424-431): the kernel carries m/s
+        if constexpr (kFlat) {
+          if (hs_test_flat(rng, vx * 1e-3, vy * 1e-3, vz * 1e-3, p, hs_k, hs_inv_cstar, hst))
+            stage = kCollide;
+        } else if constexpr (GAS == IG_GAS_HS) {
+          if (hs_test(rng, vx * 1e-3, vy * 1e-3, vz * 1e-3, p, dt, 1e6, sigma)) stage = kCollide;
         }
-        x = xc;
-        y = yc;
-        z = zc;
+      }
+    }
+    if constexpr (kFlat) {
+      if (stage == kCollide) {
+        double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
+        const int rc = hs_try_flat(rng, hst, sx, sy, sz, p, m_amu);
+        if (rc > 0) {
+          vx = sx * 1e3;
+          vy = sy * 1e3;
+          vz = sz * 1e3;
+          ncol += 1;
+          stage = kPost;
+        } else if (rc < 0) {
+          fate = -3;
+          stage = kDone;
+        }
+      }
+    } else if constexpr (GAS == IG_GAS_HS) {
+      const uint64_t parked = __ballot(stage == kCollide);
+      if (parked) {
+        const uint64_t moving = __ballot(stage == kStep || stage == kPost);
+        if (__popcll(parked) >= coll_batch || moving == 0) {
+          if (stage == kCollide) {
+            double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
+            if (hs_scatter(rng, sx, sy, sz, p, m_amu)) {
+              vx = sx * 1e3;
+              vy = sy * 1e3;
+              vz = sz * 1e3;
+              ncol += 1;
+              stage = kPost;
+            } else {
+              fate = -3;
+              stage = kDone;
+            }
+          }
+        }
+      }
+    }
+    if (stage == kPost) {
+      stage = kStep;
+      if (in_metal_3d(f.metal, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
+        double f0 = 0.0, f1 = 1.0;
+        for (int b = 0; b < 40; ++b) {
+          double fm = 0.5 * (f0 + f1);
+          if (in_metal_3d(f.metal, (xo + fm * (x - xo)) * inv_h, (yo + fm * (y - yo)) * inv_h,
+                          (zo + fm * (z - zo)) * inv_h, nx, ny, nz))
+            f1 = fm;
+          else
+            f0 = fm;
+        }
+        const double fr = f1;
+        x = xo + fr * (x - xo);
+        y = yo + fr * (y - yo);
+        z = zo + fr * (z - zo);
         vx = vxo + fr * (vx - vxo);
         vy = vyo + fr * (vy - vyo);
         vz = vzo + fr * (vz - vzo);
         t = to + fr * dt;
-        fate = p.pl_kind[ip];
-        hit_ip = ip;
-        hit_pl = true;
-        break;
-      }
-      if (hit_pl) {
+        fate = IG_FATE_IMPACT;
         if (nrec < R) write(nrec, tob + t * 1e6);
         if (rec && nrec < R) nrec++;
-        break;
+        stage = kDone;
+      }
+      if (stage != kDone) {
+        for (int ip = 0; ip < p.n_pl; ++ip) {
+          const int col = p.pl_col[ip];
+          const double cn = col == 0 ? x : (col == 1 ? y : z);
+          const double co = col == 0 ? xo : (col == 1 ? yo : zo);
+          const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
+          bool crossed;
+          if (sgn == 0.0)
+            crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
+          else
+            crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
+          if (!crossed) continue;
+          const double den = cn - co;
+          double fr = den == 0.0 ? 0.0 : (val - co) / den;
+          if (fr < 0.0) fr = 0.0;
+          if (fr > 1.0) fr = 1.0;
+          const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
+          const double t1 = col == 0 ? yc : xc;
+          const double t2 = col == 2 ? yc : zc;
+          const bool ins = p.pl_w[ip][0] <= t1 && t1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= t2 &&
+                           t2 <= p.pl_w[ip][3];
+          if (p.pl_kind[ip] == IG_FATE_DETECT) {
+            if (!ins) continue;
+          } else if (ins) {
+            continue;
+          }
+          x = xc;
+          y = yc;
+          z = zc;
+          vx = vxo + fr * (vx - vxo);
+          vy = vyo + fr * (vy - vyo);
+          vz = vzo + fr * (vz - vzo);
+          t = to + fr * dt;
+          fate = p.pl_kind[ip];
+          hit_ip = ip;
+          if (nrec < R) write(nrec, tob + t * 1e6);
+          if (rec && nrec < R) nrec++;
+          stage = kDone;
+          break;
+        }
+      }
+      if (stage != kDone && (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h ||
+                             y > (ny - 1) * h || z > (nz - 1) * h)) {
+        fate = IG_FATE_BOX;
+        if (nrec < R) write(nrec, tob + t * 1e6);
+        if (rec && nrec < R) nrec++;
+        stage = kDone;
+      }
+      if (stage != kDone && (steps0 + s) % p.rec_every == 0 && nrec < R) {
+        write(nrec, tob + t * 1e6);
+        nrec++;
       }
     }
-    if (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h || y > (ny - 1) * h ||
-        z > (nz - 1) * h) {
-      fate = IG_FATE_BOX;
-      if (nrec < R) write(nrec, tob + t * 1e6);
-      if (rec && nrec < R) nrec++;
-      break;
-    }
-    if ((steps0 + s) % p.rec_every == 0 && nrec < R) {
-      write(nrec, tob + t * 1e6);
-      nrec++;
-    }
+    if (__ballot(stage != kDone) == 0) break;
   }
   e.x[i] = x;
   e.y[i] = y;
@@ -384,13 +439,22 @@ template <int PREC, int NS>
 static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
                          int block, hipStream_t st) {
   dim3 grid((unsigned)((n_launch + block - 1) / block));
-  if (p.gas == IG_GAS_HS)
-    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS>), grid, dim3(block), 0, st, f, p, e);
-  else if (p.gas == IG_GAS_VACUUM)
-    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM>), grid, dim3(block), 0, st, f, p,
-                       e);
-  else
+  if (p.gas == IG_GAS_HS) {
+    if (p.occupancy == 2)
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 2>), grid, dim3(block), 0, st, f, p,
+                         e);
+    else if (p.occupancy == 3)
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 3>), grid, dim3(block), 0, st, f, p,
+                         e);
+    else
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 1>), grid, dim3(block), 0, st, f, p,
+                         e);
+  } else if (p.gas == IG_GAS_VACUUM) {
+    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 1>), grid, dim3(block), 0, st, f,
+                       p, e);
+  } else {
     return -2; // 3-D SDS is its own kernel (tracer3d._fly3d_sds), not this one
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
