@@ -17,6 +17,7 @@ launch / HIP-event launch time, against 8 TB/s HBM) and the CPU baseline
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -97,8 +98,8 @@ def main():
                     choices=["f32", "f32_store", "f64"])
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the plain FP64 comparison run")
-    ap.add_argument("--cpu-sample", type=int, default=16000)
-    ap.add_argument("--cpu-window", type=int, default=20000)
+    ap.add_argument("--cpu-sample", type=int, default=32000)
+    ap.add_argument("--cpu-window", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -137,11 +138,15 @@ def main():
     mean_ms = float(np.mean(main_run["kernel_ms"]))
     achieved = alg_bytes / (mean_ms * 1e-3) / 1e9
     traffic = None
+    # HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs
+    # (scaled by a plain copy's readings) kept beside the profiles
     tpath = os.path.join(HERE, "profiles",
-                         f"traffic_funnel_rz_{args.precision}.json")
-    if os.path.exists(tpath):
+                         f"traffic_funnel_rz_{args.precision}.md")
+    if os.path.exists(tpath) and args.window == 2000 and \
+            args.ions == 1_000_000:
         with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            m = re.search(r"^bytes_per_launch = (\d+)", fh.read(), re.M)
+        traffic = int(m.group(1)) if m else None
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())

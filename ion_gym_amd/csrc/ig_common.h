@@ -87,6 +87,8 @@ struct EnsK {
   uint64_t *rng_n;     // draws consumed
   double *gauss;       // polar-method cache value
   int32_t *has_gauss;
+  float *hz;           // FP32-basis hard sphere: remaining Exp(1) hazard
+                       // (NaN until the first draw)
   int32_t *fate, *face, *label, *nrec;
   int64_t *ncol, *steps;
   double *path, *eat, *ket; // record accumulators

@@ -351,6 +351,7 @@ int ig_ensemble_create(ig_engine *h, int64_t capacity, int32_t rec_rows, int32_t
   if ((rc = E->alloc(k.rng_n, n))) return rc;
   if ((rc = E->alloc(k.ncol, n))) return rc;
   if ((rc = E->alloc(k.steps, n))) return rc;
+  if ((rc = E->alloc(k.hz, n))) return rc;
   int32_t **i32[] = {&k.has_gauss, &k.fate, &k.face, &k.label, &k.nrec, &E->tune_idx};
   for (int32_t **a : i32)
     if ((rc = E->alloc(*a, n))) return rc;
@@ -436,6 +437,7 @@ static int load_range(ig_engine *h, Ensemble *E, int64_t off, const ig_ions *io,
   for (void *z : zero8) HIPX(hipMemsetAsync(z, 0, b8, st));
   HIPX(hipMemsetAsync(k.e_last + 3 * off, 0, 3 * b8, st));
   HIPX(hipMemsetAsync(k.has_gauss + off, 0, b4, st));
+  HIPX(hipMemsetAsync(k.hz + off, 0xff, b4, st)); // NaN: no hazard drawn yet
   HIPX(hipMemsetAsync(k.face + off, 0xff, b4, st)); // -1
   HIPX(hipMemsetAsync(k.label + off, 0, b4, st));
   HIPX(hipMemsetAsync(k.nrec + off, 0, b4, st));

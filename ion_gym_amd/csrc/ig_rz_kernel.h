@@ -264,11 +264,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // FP32-basis hard sphere: collision exponent constant and the
   // flattened partner-try state
   constexpr bool kFlat = PREC == IG_PREC_F32 && GAS == IG_GAS_HS;
-  float hs_k = 0.0f, hs_inv_cstar = 0.0f;
+  float hs_k = 0.0f, hs_inv_cstar = 0.0f, hz = 0.0f;
   HsTry hst{};
   if constexpr (kFlat) {
     hs_k = hs_rate_const(p, dt, 1.0, sigma);
     hs_inv_cstar = (float)(1.0 / p.c_star);
+    hz = e.hz[i];
   }
 
   // cached gather and waveform weights (see the file comment)
@@ -276,12 +277,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   double cgx = __builtin_nan(""), cgu = __builtin_nan("");
   double wk[NS > 1 ? NS - 1 : 1];
   double wtt = __builtin_nan("");
-  // FP32-basis mode: node coordinates by the reciprocal pitch (the FP64
-  // modes divide, as the reference does)
+  // Node coordinates, the radius and the radial direction: the FP64 modes
+  // divide and take FP64 square roots in the reference's order; the
+  // FP32-basis mode multiplies by the reciprocal pitch and takes the radius
+  // and its reciprocal in FP32 (relative error ~1e-7, the size of the FP32
+  // field values they weight)
   const double inv_mm = 1.0 / mm;
+  auto node = [&](double v) { return PREC == IG_PREC_F32 ? v * inv_mm : v / mm; };
+  auto radius = [&](double a, double b) {
+    if constexpr (PREC == IG_PREC_F32)
+      return (double)sqrtf((float)(a * a + b * b));
+    else
+      return sqrt(a * a + b * b);
+  };
+  // the radial kick components er * y / rr and er * z / rr
+  auto radial = [&](double er_, double rr_, double &ky, double &kz) {
+    if constexpr (PREC == IG_PREC_F32) {
+      const double q = er_ * (double)__builtin_amdgcn_rcpf((float)rr_);
+      ky = q * y;
+      kz = q * z;
+    } else {
+      ky = er_ * y / rr_;
+      kz = er_ * z / rr_;
+    }
+  };
   auto efield = [&](double px, double pr, double tt, double &ez_o, double &er_o) {
-    const double gx = PREC == IG_PREC_F32 ? px * inv_mm : px / mm;
-    const double gu = PREC == IG_PREC_F32 ? (pr - u0) * inv_mm : (pr - u0) / mm;
+    const double gx = node(px);
+    const double gu = node(pr - u0);
     if (!(gx == cgx && gu == cgu)) {
       rz_gather<PREC, NS>(f, gx, gu, cz, cr);
       if (tscale) {
@@ -321,7 +343,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   double ez, er;
   if (steps0 == 0) {
     // field at the birth point for row 0 (tracer_rz.py:184-202)
-    double r0 = sqrt(y * y + z * z);
+    double r0 = radius(y, z);
     efield(x, r0, tob, ez, er);
     if constexpr (REC) {
       rz_write_row(e.rec + (size_t)i * p.rec_rows * p.rec_cols, p.rec_mask, tob, x, y, z, vx, vy,
@@ -360,36 +382,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         vyo = vy;
         vzo = vz;
         to = t;
-        double r = rvalid ? rcur : sqrt(y * y + z * z);
+        double r = rvalid ? rcur : radius(y, z);
         efield(x, r, tob + t, ez, er);
         double rr = r > 1e-9 ? r : 1e-9;
+        double ky, kz;
+        radial(er, rr, ky, kz);
         if constexpr (GAS == IG_GAS_SDS) {
           vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
-          vy += 0.5 * sds_fac * (er * y / rr - (vy - p.vg[1]) * sds_damp) * dt;
-          vz += 0.5 * sds_fac * (er * z / rr - (vz - p.vg[2]) * sds_damp) * dt;
+          vy += 0.5 * sds_fac * (ky - (vy - p.vg[1]) * sds_damp) * dt;
+          vz += 0.5 * sds_fac * (kz - (vz - p.vg[2]) * sds_damp) * dt;
         } else {
           vx += 0.5 * ez * dt;
-          vy += 0.5 * er * y / rr * dt;
-          vz += 0.5 * er * z / rr * dt;
+          vy += 0.5 * ky * dt;
+          vz += 0.5 * kz * dt;
         }
         x += vx * dt;
         y += vy * dt;
         z += vz * dt;
         if constexpr (REC)
           path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
-        r = sqrt(y * y + z * z);
+        r = radius(y, z);
         rcur = r;
         rvalid = true;
         efield(x, r, tob + t + dt, ez, er);
         rr = r > 1e-9 ? r : 1e-9;
+        radial(er, rr, ky, kz);
         if constexpr (GAS == IG_GAS_SDS) {
           vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
-          vy += 0.5 * sds_fac * (er * y / rr - (vy - p.vg[1]) * sds_damp) * dt;
-          vz += 0.5 * sds_fac * (er * z / rr - (vz - p.vg[2]) * sds_damp) * dt;
+          vy += 0.5 * sds_fac * (ky - (vy - p.vg[1]) * sds_damp) * dt;
+          vz += 0.5 * sds_fac * (kz - (vz - p.vg[2]) * sds_damp) * dt;
         } else {
           vx += 0.5 * ez * dt;
-          vy += 0.5 * er * y / rr * dt;
-          vz += 0.5 * er * z / rr * dt;
+          vy += 0.5 * ky * dt;
+          vz += 0.5 * kz * dt;
         }
         t += dt;
         s += 1;
@@ -418,7 +443,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
               path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo)) - pre;
           }
         } else if constexpr (kFlat) {
-          if (hs_test_flat(rng, vx, vy, vz, p, hs_k, hs_inv_cstar, hst)) stage = kCollide;
+          if (hs_hazard_step(rng, hz, vx, vy, vz, p, hs_k, hs_inv_cstar, hst)) stage = kCollide;
         } else if constexpr (GAS == IG_GAS_HS) {
           if (hs_test(rng, vx, vy, vz, p, dt, 1.0, sigma)) stage = kCollide;
         }
@@ -427,7 +452,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     if constexpr (kFlat) {
       // one partner try per pass for every lane whose collision is due
       if (stage == kCollide) {
-        const int rc = hs_try_flat(rng, hst, vx, vy, vz, p, m_ion);
+        const int rc = hs_try_flat(rng, hst, hz, vx, vy, vz, p, m_ion);
         if (rc > 0) {
           ncol += 1;
           stage = kPost;
@@ -463,15 +488,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
           nrec += 1;
         }
       }
-      const double r = rvalid ? rcur : sqrt(y * y + z * z);
-      if (metal_nn(f.metal, x / mm, (r - u0) / mm, nx, nu)) {
+      const double r = rvalid ? rcur : radius(y, z);
+      if (metal_nn(f.metal, node(x), node(r - u0), nx, nu)) {
         double f0 = 0.0, f1 = 1.0;
         for (int b = 0; b < 40; ++b) {
           double fm = 0.5 * (f0 + f1);
           double ym = yo + fm * (y - yo);
           double zm = zo + fm * (z - zo);
-          double rm = sqrt(ym * ym + zm * zm);
-          if (metal_nn(f.metal, (xo + fm * (x - xo)) / mm, (rm - u0) / mm, nx, nu))
+          double rm = radius(ym, zm);
+          if (metal_nn(f.metal, node(xo + fm * (x - xo)), node(rm - u0), nx, nu))
             f1 = fm;
           else
             f0 = fm;
@@ -488,13 +513,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         t = to + f1 * dt;
         fate = IG_FATE_IMPACT;
         if (f.label) {
-          double re = sqrt(y * y + z * z);
-          double li = floor(x / mm + 0.5), lj = floor((re - u0) / mm + 0.5);
+          double re = radius(y, z);
+          double li = floor(node(x) + 0.5), lj = floor(node(re - u0) + 0.5);
           if (li >= 0.0 && lj >= 0.0 && li <= (double)(nx - 1) && lj <= (double)(nu - 1))
             label = f.label[(int64_t)li * nu + (int64_t)lj];
         }
         stage = kDone;
-      } else if (x < 0 || r / mm > (double)(nu - 2)) {
+      } else if (x < 0 || node(r) > (double)(nu - 2)) {
         fate = IG_FATE_BOX;
         stage = kDone;
       } else {
@@ -550,7 +575,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
             }
           }
         }
-        if (stage != kDone && x / mm > (double)(nx - 1)) {
+        if (stage != kDone && node(x) > (double)(nx - 1)) {
           fate = IG_FATE_BOX;
           stage = kDone;
         }
@@ -588,6 +613,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     e.eat[i] = eat;
     e.ket[i] = ket;
   }
+  if constexpr (kFlat) e.hz[i] = hz;
 }
 
 template <int PREC, int NS, int GAS>

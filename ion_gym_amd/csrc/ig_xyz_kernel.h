@@ -227,11 +227,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // sphere collisions are scattered together by the wave (FP64, the
   // reference's algorithm) or one partner try per pass (FP32-basis mode)
   constexpr bool kFlat = PREC == IG_PREC_F32 && GAS == IG_GAS_HS;
-  float hs_k = 0.0f, hs_inv_cstar = 0.0f;
+  float hs_k = 0.0f, hs_inv_cstar = 0.0f, hz = 0.0f;
   HsTry hst{};
   if constexpr (kFlat) {
     hs_k = hs_rate_const(p, dt, 1e6, sigma);
     hs_inv_cstar = (float)(1.0 / p.c_star);
+    hz = e.hz[i];
   }
   const int coll_batch = p.coll_batch;
   int stage = kStep;
@@ -295,7 +296,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         stage = kPost;
         // HS in mm/us (tracer3d.py:424-431): the kernel carries m/s
         if constexpr (kFlat) {
-          if (hs_test_flat(rng, vx * 1e-3, vy * 1e-3, vz * 1e-3, p, hs_k, hs_inv_cstar, hst))
+          if (hs_hazard_step(rng, hz, vx * 1e-3, vy * 1e-3, vz * 1e-3, p, hs_k, hs_inv_cstar,
+                             hst))
             stage = kCollide;
         } else if constexpr (GAS == IG_GAS_HS) {
           if (hs_test(rng, vx * 1e-3, vy * 1e-3, vz * 1e-3, p, dt, 1e6, sigma)) stage = kCollide;
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     if constexpr (kFlat) {
       if (stage == kCollide) {
         double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
-        const int rc = hs_try_flat(rng, hst, sx, sy, sz, p, m_amu);
+        const int rc = hs_try_flat(rng, hst, hz, sx, sy, sz, p, m_amu);
         if (rc > 0) {
           vx = sx * 1e3;
           vy = sy * 1e3;
@@ -433,6 +435,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   e.rng_n[i] = rng.n;
   e.has_gauss[i] = rng.has_gauss;
   e.gauss[i] = rng.gauss;
+  if constexpr (kFlat) e.hz[i] = hz;
 }
 
 template <int PREC, int NS>
