@@ -11,11 +11,18 @@ ig_fly launch), with the ion state and the field resident in HBM.
 Prints ONE JSON line.  ``value`` is ion-steps/s (live ion-steps, whole
 job).  Beside it: the plain FP64 kernel's ion-steps/s on the same
 workload, the roofline of the flight kernel (algorithmic gather bytes per
-launch / HIP-event launch time, against 8 TB/s HBM) and the CPU baseline
-(the oracle's C FP64 port on the host cores, timed on a bounded sample).
+launch / HIP-event launch time, against 8 TB/s HBM), the CPU baseline
+(the oracle's C FP64 port on the host cores, timed on a bounded sample)
+and the serving leg: flight requests on the 3-D SLIM traveling-wave board
+(201 x 81 x 801 @ 0.05 mm, 11 FP32 field sets, N2 4 Torr hard sphere,
+1e3-5e5 ions per request with their own TW amplitude, speed and m/z)
+arriving as a Poisson stream at a set fraction of the engine's solo
+throughput, served by continuous batching; p50/p99 request latency and
+the sustained ion-steps/s.
 """
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -86,6 +93,76 @@ def cpu_baseline(cfg, window, n_sample, threads):
     return int(r["steps"].sum()), el
 
 
+def run_serving(eng, n_req, load, flight_us, window, capacity, seed):
+    """The serving leg: requests on the SLIM traveling-wave board (11 FP32
+    field sets, N2 4 Torr hard sphere) arrive as a Poisson stream and are
+    served by one FlightServer (continuous batching over one resident
+    slab).  The arrival rate is set to ``load`` times the engine's measured
+    solo throughput divided by the mean request work.  Returns p50/p99
+    request latency (submit -> result on the host) and the sustained
+    ion-steps/s over the run."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    from ion_gym_amd.serve import (FlightRequest, FlightServer, latency_stats,
+                                   make_tune)
+    import ion_gym_amd._lib as L
+    from ion_gym_amd.serve import _fill_tune
+    t0 = time.perf_counter()
+    c = configs.slim_3d(coarse=1)
+    fp = eng.upload_xyz(c["pack"], "f32")
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    reqs, work = [], []
+    for k in range(n_req):
+        ions, tune = configs.slim_request(c, rng, t_flight_us=flight_us,
+                                          rng_key=k + 1)
+        t = make_tune(c["pack"], **tune)
+        reqs.append(FlightRequest(ions=ions, tune=t))
+        work.append(len(ions["x"]) * math.ceil(t["t_max"] / t["dt"]))
+    # solo throughput: one 200,000-ion request, two windows
+    cal_ions, cal_tune = configs.slim_request(c, np.random.default_rng(seed + 1),
+                                              n_ions=200_000,
+                                              t_flight_us=flight_us)
+    ens = eng.ensemble(200_000)
+    tunes = (L.Tune * 1)()
+    _fill_tune(tunes[0], make_tune(c["pack"], **cal_tune))
+    L.check(L.lib().ig_ensemble_set_tunes(eng.h, ens.id, 1, tunes))
+    ens.load(cal_ions)
+    fl = Flight(cal_tune["dt"], cal_tune["t_max"], gas=c["gas"],
+                max_steps=window)
+    eng.fly(fp, fl, ens)
+    s0 = ens.steps()
+    t1 = time.perf_counter()
+    eng.fly(fp, fl, ens)
+    eng.sync()
+    solo = (ens.steps() - s0) / (time.perf_counter() - t1)
+    ens.release()
+    mean_work = float(np.mean(work))
+    lam = load * solo / mean_work
+    gaps = np.random.default_rng(seed + 2).exponential(1.0 / lam, n_req)
+    arrivals = list(zip(np.cumsum(gaps) - gaps[0], reqs))
+    srv = FlightServer(eng, fp, Flight(cal_tune["dt"], cal_tune["t_max"],
+                                       gas=c["gas"]),
+                       capacity=capacity, window=window)
+    t2 = time.perf_counter()
+    done = srv.run_arrivals(arrivals)
+    wall = time.perf_counter() - t2
+    st = latency_stats(done)
+    out = {"workload": "slim_3d (SLIM TW board 201x81x801 @ 0.05 mm, 11 "
+                       "FP32 sets, N2 4 Torr HS)",
+           "requests": len(done), "ions_per_request": "log-U(1e3, 5e5)",
+           "flight_us": flight_us, "offered_load": load,
+           "arrival_rate_per_s": lam, "window_steps": window,
+           "slab_capacity": capacity,
+           "p50_s": st["p50_s"], "p99_s": st["p99_s"], "mean_s": st["mean_s"],
+           "sustained_ion_steps_per_s": srv.ion_steps / wall,
+           "solo_ion_steps_per_s": solo, "wall_s": wall,
+           "field_bytes": fp.nbytes, "build_s": build_s}
+    srv.release()
+    fp.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +178,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32000)
     ap.add_argument("--cpu-window", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg (SLIM board, Poisson arrivals)")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-load", type=float, default=0.6)
+    ap.add_argument("--serve-flight-us", type=float, default=2.0)
+    ap.add_argument("--serve-window", type=int, default=500)
+    ap.add_argument("--serve-capacity", type=int, default=2_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
@@ -159,6 +243,13 @@ def main():
                "sample": f"{args.cpu_sample} ions of the same workload x "
                          f"{args.cpu_window} steps (FP64, HS, Philox), "
                          f"{el:.1f} s"}
+    serving = None
+    if not args.no_serve:
+        print(f"[bench] serving: {args.serve_requests} SLIM requests at "
+              f"offered load {args.serve_load}", file=sys.stderr, flush=True)
+        serving = run_serving(eng, args.serve_requests, args.serve_load,
+                              args.serve_flight_us, args.serve_window,
+                              args.serve_capacity, seed=11)
     line = {
         "metric": "ion-steps/s at 1M ions (3-D funnel+HS), FP32 vs FP64 "
                   "plain; p50/p99 latency",
@@ -189,6 +280,7 @@ def main():
                      "alg_bytes_per_launch": alg_bytes,
                      "bytes_per_ion_step": per_step},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     eng.close()

@@ -312,3 +312,183 @@ def hs_drift_rz(n_ions=2000, seed=5000, coarse=2, e_vpmm=1.0, mz=500.0,
     return dict(name="hs_drift_rz", route="rz", model=model, ions=ions,
                 dt=dt_ns * 1e-3, t_max=200.0, gas=gas_d, scale=e_vpmm,
                 tau_ns=tau_ns)
+
+
+# ----------------------------------------------------------------- SLIM
+def _slab_fields(B, h, gap, ny, workers=None):
+    """E = -grad(phi) of the Laplace solution between two parallel boards
+    y = 0 and y = gap that both carry the board potential B(x, z)
+    (periodic in x and z over B's padded extent): per Fourier mode k,
+    phi_k(y) = B_k cosh(k (y - gap/2)) / cosh(k gap/2).  Returns Ex, Ey,
+    Ez of shape (ny, Px, Pz) in V/mm per volt."""
+    import scipy.fft as sfft
+    Px, Pz = B.shape
+    kx = 2 * np.pi * sfft.fftfreq(Px, d=h)[:, None]
+    kz = 2 * np.pi * sfft.rfftfreq(Pz, d=h)[None, :]
+    K = np.sqrt(kx * kx + kz * kz)
+    Bh = sfft.rfft2(B, workers=workers)
+    yc = np.arange(ny)[:, None, None] * h - 0.5 * gap
+    ay = np.abs(yc)
+    # cosh(K yc) / cosh(K gap/2) and sinh(K yc) / cosh(K gap/2) without
+    # overflow
+    e0 = np.exp(K[None] * (ay - 0.5 * gap)) / (1.0 + np.exp(-K[None] * gap))
+    e2 = np.exp(-2.0 * K[None] * ay)
+    rc = e0 * (1.0 + e2)
+    rs = np.sign(yc) * e0 * (1.0 - e2)
+    ph = Bh[None] * rc
+    Ex = sfft.irfft2(-1j * kx[None] * ph, s=(Px, Pz), workers=workers)
+    Ez = sfft.irfft2(-1j * kz[None] * ph, s=(Px, Pz), workers=workers)
+    del ph
+    Ey = sfft.irfft2(-(K[None] * Bh[None]) * rs, s=(Px, Pz), workers=workers)
+    return Ex, Ey, Ez
+
+
+def slim_3d(coarse=1, gap_mm=4.0, width_mm=10.0, length_mm=40.0, n_tw=48,
+            n_phase=8, rf_v=100.0, rf_hz=1.0e6, dc_v=10.0, P_torr=4.0,
+            T_k=300.0, workers=None):
+    """3-D SLIM traveling-wave board: two parallel boards gap_mm apart
+    (201 x 81 x 801 nodes @ 0.05 mm: x across the board, y across the gap,
+    z along the path).  Each board carries, from the centre line out:
+    48 TW electrodes in an 8-phase pattern across the central band
+    x in [2.6, 7.4] mm (pitch length/48, 0.2 mm gaps), RF guard strips
+    x in [1.6, 2.4] and [7.6, 8.4] (RF A) and [0.6, 1.4], [8.6, 9.4]
+    (RF B, opposite phase), and DC guards x in [0, 0.4] and [9.6, 10].
+    Field sets: the static set (the DC guards at dc_v) and K = 10 drive
+    channels (8 TW phases: square, duty 0.5, unipolar amp*base + amp via
+    ch_amp = ch_off = A/2; 2 RF: sin at rf_hz, 0 and pi), 11 in all.  The
+    bases are the exact Laplace solution between the boards for the board
+    potential (1 on a channel's electrodes, 0 on the others, linear across
+    the 0.2 mm gaps), periodic over a grounded margin (FFT in x and z).
+    N2 at P_torr, T_k with hard-sphere collisions; requests carry their own
+    TW amplitude, speed, m/z and time step (slim_request)."""
+    h = 0.05 * coarse
+    nx = int(round(width_mm / h)) + 1
+    ny = int(round(gap_mm / h)) + 1
+    nz = int(round(length_mm / h)) + 1
+    Px = 1 << int(math.ceil(math.log2(nx * 1.25)))
+    Pz = 1 << int(math.ceil(math.log2(nz * 1.25)))
+    ox = (Px - nx) // 2          # board origin inside the padded plane
+    oz = (Pz - nz) // 2
+    X = (np.arange(Px) - ox) * h
+    Z = (np.arange(Pz) - oz) * h
+    gap = 0.2
+    pitch = length_mm / n_tw
+
+    def band(lo, hi, v):
+        return ((v >= lo) & (v <= hi)).astype(np.float64)
+
+    zin = band(0.0, length_mm, Z)
+    tw_x = band(2.6, 7.4, X)
+    rfa_x = band(1.6, 2.4, X) + band(7.6, 8.4, X)
+    rfb_x = band(0.6, 1.4, X) + band(8.6, 9.4, X)
+    dc_x = band(-1e-9, 0.4, X) + band(9.6, width_mm + 1e-9, X)
+    zz = Z - 0.0
+    n_of_z = np.floor(zz / pitch).astype(int)
+    in_el = np.abs(zz - (n_of_z + 0.5) * pitch) <= 0.5 * (pitch - gap)
+    boards = []
+    boards.append(dc_x[:, None] * zin[None, :])                # static
+    for ph_i in range(n_phase):
+        sel = (in_el & (n_of_z % n_phase == ph_i) & (zz >= 0)
+               & (zz <= length_mm)).astype(np.float64)
+        boards.append(tw_x[:, None] * sel[None, :])
+    boards.append(rfa_x[:, None] * zin[None, :])
+    boards.append(rfb_x[:, None] * zin[None, :])
+    # linear ramps across the gaps: a box filter of the gap width in x and z
+    w = max(1, int(round(gap / h)))
+    kern = np.ones(w) / w
+
+    def smooth(b):
+        b = np.apply_along_axis(lambda r: np.convolve(r, kern, "same"), 0, b)
+        return np.apply_along_axis(lambda r: np.convolve(r, kern, "same"),
+                                   1, b)
+
+    grids = []
+    for b in boards:
+        Ex, Ey, Ez = _slab_fields(smooth(b), h, gap_mm, ny, workers)
+        crop = (slice(None), slice(ox, ox + nx), slice(oz, oz + nz))
+        # (ny, nx, nz) -> (nx, ny, nz)
+        grids.append([np.ascontiguousarray(E[crop].transpose(1, 0, 2))
+                      for E in (Ex, Ey, Ez)])
+        del Ex, Ey, Ez
+    K = len(grids) - 1
+    ele = np.zeros((nx, ny, nz), bool)
+    # the two boards' electrode faces (a single node plane: the reference's
+    # all-eight-corners rule never calls it metal, ions leave through the
+    # box faces y = 0 and y = gap)
+    face = (sum(boards[1:]) + boards[0])[ox:ox + nx, oz:oz + nz] > 0.5
+    ele[:, 0, :] = face
+    ele[:, -1, :] = face
+    om_rf = rf_hz * 1e-6 * 2 * math.pi
+    pack = dict(EAx=grids[0][0] * dc_v, EAy=grids[0][1] * dc_v,
+                EAz=grids[0][2] * dc_v,
+                ExK=np.stack([g[0] for g in grids[1:]]),
+                EyK=np.stack([g[1] for g in grids[1:]]),
+                EzK=np.stack([g[2] for g in grids[1:]]),
+                ele=ele, h_mm=h,
+                ch_kind=np.array([2] * n_phase + [0, 0], np.int64),
+                ch_om=np.array([0.0] * n_phase + [om_rf, om_rf]),
+                ch_ph=np.array([-2 * math.pi * q / n_phase
+                                for q in range(n_phase)] + [0.0, math.pi]),
+                ch_amp=np.array([10.0] * n_phase + [rf_v, rf_v]),
+                ch_off=np.array([10.0] * n_phase + [0.0, 0.0]),
+                ch_duty=np.full(K, 0.5))
+    g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
+    gas_d = dict(model=1, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+                 c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+                 flow=(0.0, 0.0, 0.0))
+    return dict(name="slim_3d", route="xyz", pack=pack, gas=gas_d,
+                pitch_mm=pitch, n_phase=n_phase, shape=(nx, ny, nz), h=h,
+                P_torr=P_torr, T_k=T_k, width_mm=width_mm, gap_mm=gap_mm,
+                length_mm=length_mm)
+
+
+def slim_sigma_m2(mz):
+    """Hard-sphere cross section of a singly charged ion of mass mz amu:
+    2e-18 m^2 at 500 amu, scaling as m^(2/3)."""
+    return 2.0e-18 * (np.asarray(mz, float) / 500.0) ** (2.0 / 3.0)
+
+
+def slim_request(c, rng, n_ions=None, t_flight_us=2.0, rng_key=0):
+    """One serving request on a slim_3d board: n ions (log-uniform in
+    1e3-5e5 unless given) of one m/z (uniform 200-3000) born in a 1 mm box
+    at the board centre 2 mm downstream of the entrance with 300 K
+    velocities; the tune carries a TW amplitude A ~ U(10, 40) V (unipolar
+    squares: amp = off = A/2), a TW speed v ~ U(50, 300) m/s (frequency
+    v / (8 pitch)), the time step tau/20 for the ion's hard-sphere
+    collision time and a flight time of t_flight_us.  Returns (ions,
+    tune dict for serve.make_tune)."""
+    n = int(n_ions) if n_ions is not None else int(
+        round(math.exp(rng.uniform(math.log(1e3), math.log(5e5)))))
+    mz = float(rng.uniform(200.0, 3000.0))
+    A = float(rng.uniform(10.0, 40.0))
+    v = float(rng.uniform(50.0, 300.0))
+    f_per_us = v * 1e-3 / (c["n_phase"] * c["pitch_mm"])
+    sig = float(slim_sigma_m2(mz))
+    tau_ns = gas_mod.collision_time_ns("N2", c["T_k"], c["P_torr"], sig)
+    dt_s = tau_ns / gas_mod.HS_DT_PER_COLLISION_TIME * 1e-9
+    s1 = math.sqrt(K_B * c["T_k"] / (mz * KG_AMU)) * 1e-3
+    vel = rng.normal(0.0, s1, (3, n))
+    xc = 0.5 * c["width_mm"]
+    yc = 0.5 * c["gap_mm"]
+    ions = dict(x=xc + rng.uniform(-0.5, 0.5, n),
+                y=yc + rng.uniform(-0.5, 0.5, n),
+                z=2.0 + rng.uniform(-0.5, 0.5, n),
+                vx=vel[0], vy=vel[1], vz=vel[2], tob=np.zeros(n),
+                m_amu=np.full(n, mz), acc=np.full(n, E_CHG / (mz * KG_AMU)),
+                sigma_m2=np.full(n, sig),
+                seed=rng.integers(0, 2**40) + np.arange(n, dtype=np.int64))
+    pk = c["pack"]
+    K = len(pk["ch_kind"])
+    nph = c["n_phase"]
+    om = np.array(pk["ch_om"], float)
+    om[:nph] = 2 * math.pi * f_per_us
+    amp = np.array(pk["ch_amp"], float)
+    off = np.array(pk["ch_off"], float)
+    amp[:nph] = 0.5 * A
+    off[:nph] = 0.5 * A
+    tune = dict(dt=dt_s, t_max=t_flight_us * 1e-6, rng_key=int(rng_key),
+                set_scale=np.ones(K + 1), ch_kind=np.array(pk["ch_kind"]),
+                ch_om=om, ch_ph=np.array(pk["ch_ph"], float),
+                ch_duty=np.array(pk["ch_duty"], float), ch_amp=amp,
+                ch_off=off)
+    return ions, tune

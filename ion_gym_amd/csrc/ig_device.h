@@ -317,23 +317,55 @@ __device__ __forceinline__ float phase_reduce(double arg) {
   return (float)(arg - n * IG_TWO_PI);
 }
 
+// sin(r) for r in [-pi, pi] (as phase_reduce leaves it) in FP32: reflected
+// into [-pi/2, pi/2] by sin(r) = sin(+-pi - r), then the Taylor polynomial
+// through r^11, whose truncation error there is below (pi/2)^13 / 13! =
+// 6e-8; the reflection adds the FP32 rounding of pi (9e-8).
+__device__ __forceinline__ float sin_pi_range(float r) {
+  const float kHalfPi = 1.57079632679489662f, kPi = 3.14159265358979324f;
+  if (r > kHalfPi)
+    r = kPi - r;
+  else if (r < -kHalfPi)
+    r = -kPi - r;
+  const float r2 = r * r;
+  float q = -2.50521083854417188e-08f;             // -1/11!
+  q = __builtin_fmaf(q, r2, 2.75573192239858907e-06f);  //  1/9!
+  q = __builtin_fmaf(q, r2, -1.98412698412698413e-04f); // -1/7!
+  q = __builtin_fmaf(q, r2, 8.33333333333333333e-03f);  //  1/5!
+  q = __builtin_fmaf(q, r2, -1.66666666666666667e-01f); // -1/3!
+  return __builtin_fmaf(q * r2, r, r);
+}
+
+// the general waveform, out of line: tables and square waves of other
+// duty are rare, and inlined per channel they would crowd the step loop
+template <bool AFFINE>
+__device__ __noinline__ double wave_eval_slow(const Waves *__restrict__ W, const FieldsK &f, int k,
+                                              double t) {
+  return wave_eval<AFFINE>(W, f, k, t);
+}
+
+// FP32-basis waveform: sin, cos and square at duty 0.5 (sin >= 0, i.e. the
+// reduced phase >= 0) from the FP64-reduced phase; the rest as in FP64
 template <bool AFFINE>
 __device__ __forceinline__ double wave_eval_fast(const Waves *__restrict__ W, const FieldsK &f,
                                                  int k, double t) {
-  int kind = W->kind[k];
-  if (kind == IG_WAVE_SIN || kind == IG_WAVE_COS ||
-      (kind == IG_WAVE_SQUARE && W->duty[k] == 0.5)) {
+  const int kind = W->kind[k];
+  const bool sq = kind == IG_WAVE_SQUARE && W->duty[k] == 0.5;
+  if (kind == IG_WAVE_SIN || kind == IG_WAVE_COS || sq) {
     float r = phase_reduce(W->om[k] * t + W->ph[k]);
     float v;
-    if (kind == IG_WAVE_SIN)
-      v = sinf(r);
-    else if (kind == IG_WAVE_COS)
-      v = cosf(r);
-    else
-      v = sinf(r) >= 0.0f ? 1.0f : -1.0f;
+    if (sq) {
+      v = r >= 0.0f ? 1.0f : -1.0f;
+    } else {
+      if (kind == IG_WAVE_COS) { // cos(r) = sin(r + pi/2), back into [-pi, pi]
+        r = r + 1.57079632679489662f;
+        if (r > 3.14159265358979324f) r = r - 6.28318530717958648f;
+      }
+      v = sin_pi_range(r);
+    }
     return AFFINE ? W->amp[k] * (double)v + W->off[k] : (double)v;
   }
-  return wave_eval<AFFINE>(W, f, k, t);
+  return wave_eval_slow<AFFINE>(W, f, k, t);
 }
 
 // ---- hard sphere, FP32-basis mode, flattened into the lane state machine

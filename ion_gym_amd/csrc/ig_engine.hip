@@ -2,6 +2,7 @@
 // field packs, device-resident ensembles and the flight launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -93,6 +94,7 @@ struct Ensemble {
   int32_t n_tunes = 0;
   DevBuf tunes;         // Tune table (device)
   DevBuf counts;        // live_by_tune scratch (device)
+  DevBuf sums;          // live/steps reduction (device)
   int32_t *tune_idx = nullptr;
   std::vector<std::unique_ptr<DevBuf>> bufs;
   EnsK k{};
@@ -637,36 +639,68 @@ int ig_ensemble_live_by_tune(ig_engine *h, ig_ensemble_id id, int32_t n, int64_t
   return 0;
 }
 
+// live ions and the step total of an ensemble, reduced on the device:
+// out[0] += live count, out[1] += sum of steps (grid-stride, one atomic
+// pair per block)
+__global__ void live_steps_kernel(const int32_t *__restrict__ fate,
+                                  const int64_t *__restrict__ steps, int64_t n,
+                                  unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long sl[256], ss[256];
+  unsigned long long l = 0, st = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    l += fate[i] == IG_FATE_FLYING;
+    st += (unsigned long long)steps[i];
+  }
+  sl[threadIdx.x] = l;
+  ss[threadIdx.x] = st;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sl[threadIdx.x] += sl[threadIdx.x + w];
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], sl[0]);
+    atomicAdd(&out[1], ss[0]);
+  }
+}
+
+static int live_and_steps(ig_engine *h, Ensemble *E, int64_t *live, int64_t *steps) {
+  if (int rc = set_dev(h)) return rc;
+  if (E->sums.bytes < 16) {
+    if (int rc = dev_alloc(E->sums, 16)) return rc;
+  }
+  auto *o = (unsigned long long *)E->sums.p;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, h->stream, o, 2);
+  if (E->n > 0) {
+    const int64_t blocks = std::min<int64_t>((E->n + 255) / 256, 1024);
+    hipLaunchKernelGGL(live_steps_kernel, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                       E->k.fate, E->k.steps, E->n, o);
+  }
+  HIPX(hipGetLastError());
+  unsigned long long r[2] = {0, 0};
+  HIPX(hipMemcpyAsync(r, o, 16, hipMemcpyDeviceToHost, h->stream));
+  HIPX(hipStreamSynchronize(h->stream));
+  if (live) *live = (int64_t)r[0];
+  if (steps) *steps = (int64_t)r[1];
+  return 0;
+}
+
 int ig_ensemble_live(ig_engine *h, ig_ensemble_id id, int64_t *out) {
   if (!h || !out) return fail(IG_E_ARG, "ig_ensemble_live: null argument");
   Ensemble *E = find_ens(h, id);
   if (!E) return fail(IG_E_STATE, "ig_ensemble_live: unknown ensemble id");
-  if (int rc = set_dev(h)) return rc;
-  std::vector<int32_t> f((size_t)E->n);
-  if (E->n) {
-    HIPX(hipMemcpyAsync(f.data(), E->k.fate, (size_t)E->n * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPX(hipStreamSynchronize(h->stream));
-  }
-  int64_t live = 0;
-  for (int32_t v : f) live += v == IG_FATE_FLYING;
-  *out = live;
-  return 0;
+  return live_and_steps(h, E, out, nullptr);
 }
 
 int ig_ensemble_steps(ig_engine *h, ig_ensemble_id id, int64_t *out) {
   if (!h || !out) return fail(IG_E_ARG, "ig_ensemble_steps: null argument");
   Ensemble *E = find_ens(h, id);
   if (!E) return fail(IG_E_STATE, "ig_ensemble_steps: unknown ensemble id");
-  if (int rc = set_dev(h)) return rc;
-  std::vector<int64_t> s((size_t)E->n);
-  if (E->n) {
-    HIPX(hipMemcpyAsync(s.data(), E->k.steps, (size_t)E->n * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPX(hipStreamSynchronize(h->stream));
-  }
-  int64_t tot = 0;
-  for (int64_t v : s) tot += v;
-  *out = tot;
-  return 0;
+  return live_and_steps(h, E, nullptr, out);
 }
 
 static int build_flight(const ig_flight_params *p, FlightK &k) {

@@ -20,6 +20,7 @@
 #pragma once
 #pragma clang fp contract(off)
 #include "ig_device.h"
+#include <type_traits>
 
 namespace ig {
 
@@ -70,9 +71,9 @@ __device__ __forceinline__ void load_vals(const T *__restrict__ p, T (&v)[N]) {
 
 // interp._bilin (interp.py:19-36) for every set of the records around
 // (gx, gu): index clamped to the interior, fraction clamped to [0, 1].
-template <int PREC, int NS>
-__device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu, double (&cz)[NS],
-                                          double (&cr)[NS]) {
+template <int PREC, int NS, typename OT>
+__device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu, OT (&cz)[NS],
+                                          OT (&cr)[NS]) {
   using T = typename StoreT<PREC>::T;
   using L = RecLayout<T, NS, 2>;
   const int64_t nx = f.n0, nu = f.n1;
@@ -110,8 +111,8 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
           a11[2 * s + 1], w11,
           __builtin_fmaf(a01[2 * s + 1], w01,
                          __builtin_fmaf(a10[2 * s + 1], w10, a00[2 * s + 1] * w00)));
-      cz[s] = (double)vz * f.scale[s];
-      cr[s] = (double)vr * f.scale[s];
+      cz[s] = (OT)(vz * (float)f.scale[s]);
+      cr[s] = (OT)(vr * (float)f.scale[s]);
     }
   } else {
     const double ax = 1 - fx, au = 1 - fu;
@@ -272,10 +273,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     hz = e.hz[i];
   }
 
-  // cached gather and waveform weights (see the file comment)
-  double cz[NS], cr[NS];
+  // cached gather and waveform weights (see the file comment); the
+  // FP32-basis mode keeps and composes them in FP32
+  using CT = typename std::conditional<PREC == IG_PREC_F32, float, double>::type;
+  CT cz[NS], cr[NS];
   double cgx = __builtin_nan(""), cgu = __builtin_nan("");
-  double wk[NS > 1 ? NS - 1 : 1];
+  CT wk[NS > 1 ? NS - 1 : 1];
+  const double acc9 = 1e-9 * acc;
   double wtt = __builtin_nan("");
   // Node coordinates, the radius and the radial direction: the FP64 modes
   // divide and take FP64 square roots in the reference's order; the
@@ -309,8 +313,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       if (tscale) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-          cz[q] = cz[q] * tscale[q];
-          cr[q] = cr[q] * tscale[q];
+          cz[q] = cz[q] * (CT)tscale[q];
+          cr[q] = cr[q] * (CT)tscale[q];
         }
       }
       cgx = gx;
@@ -320,24 +324,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
 #pragma unroll
       for (int k = 0; k < NS - 1; ++k)
         if (k < K)
-          wk[k] = PREC == IG_PREC_F32 ? wave_eval_fast<false>(W, f, k, tt)
-                                      : wave_eval<false>(W, f, k, tt);
+          wk[k] = PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(W, f, k, tt)
+                                      : (CT)wave_eval<false>(W, f, k, tt);
       wtt = tt;
     }
-    double zz = cz[0], rz = cr[0];
+    CT zz = cz[0], rz = cr[0];
 #pragma unroll
     for (int k = 0; k < NS - 1; ++k) {
       if (k < K) {
-        zz = zz + wk[k] * cz[1 + k];
-        rz = rz + wk[k] * cr[1 + k];
+        if constexpr (PREC == IG_PREC_F32) {
+          zz = __builtin_fmaf(wk[k], cz[1 + k], zz);
+          rz = __builtin_fmaf(wk[k], cr[1 + k], rz);
+        } else {
+          zz = zz + wk[k] * cz[1 + k];
+          rz = rz + wk[k] * cr[1 + k];
+        }
       }
     }
     if (gate && tt >= f.tau_gate) {
       zz = zz + cz[NS - 1];
       rz = rz + cr[NS - 1];
     }
-    ez_o = zz * 1e-9 * acc;
-    er_o = rz * 1e-9 * acc;
+    if constexpr (PREC == IG_PREC_F32) {
+      ez_o = (double)zz * acc9;
+      er_o = (double)rz * acc9;
+    } else {
+      ez_o = zz * 1e-9 * acc;
+      er_o = rz * 1e-9 * acc;
+    }
   };
 
   double ez, er;

@@ -113,6 +113,73 @@ __device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy
   ez = e[2];
 }
 
+// The FP32-basis field: each corner record is read as float4s once and
+// its sets are composed with the per-set weights wt (waveform x scale)
+// before the trilinear blend, so an evaluation is 8 x 3 NS FMAs over
+// 8 x S/4 vector loads whatever the number of sets.  The clamps and cell
+// index are those of _tri in FP64; the sums are FP32 in a different order
+// than the reference's, within the FP32 mode's stated error.
+template <int NS>
+__device__ __forceinline__ void xyz_field_f32(const FieldsK &f, double gx, double gy, double gz,
+                                              const float (&wt)[NS], double &ex, double &ey,
+                                              double &ez) {
+  constexpr int S = RecLayout3<float, NS>::kStride, Q = S / 4;
+  const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
+  double x = gx, y = gy, z = gz;
+  if (x < 0.0) x = 0.0;
+  if (y < 0.0) y = 0.0;
+  if (z < 0.0) z = 0.0;
+  if (x > nx - 1) x = nx - 1.0;
+  if (y > ny - 1) y = ny - 1.0;
+  if (z > nz - 1) z = nz - 1.0;
+  int64_t i = (int64_t)fmax(x, 0.0), j = (int64_t)fmax(y, 0.0), k = (int64_t)fmax(z, 0.0);
+  if (i > nx - 2) i = nx - 2;
+  if (j > ny - 2) j = ny - 2;
+  if (k > nz - 2) k = nz - 2;
+  const float tx = (float)(x - i), ty = (float)(y - j), tz = (float)(z - k);
+  const float4 *g = reinterpret_cast<const float4 *>(f.grid);
+  const int64_t r00 = ((i * ny + j) * nz + k) * Q, dI = ny * nz * Q, dJ = nz * Q;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  // the four (i, j) columns one after another, each as its two z-adjacent
+  // records (contiguous, 2 S floats): the loads of one column are in flight
+  // together without holding all eight corners in registers
+#pragma unroll 1
+  for (int col = 0; col < 4; ++col) {
+#pragma unroll
+  for (int dk = 0; dk < 2; ++dk) {
+    const int di = (col >> 1) & 1, dj = col & 1;
+    const float wc =
+        (di ? tx : 1.0f - tx) * (dj ? ty : 1.0f - ty) * (dk ? tz : 1.0f - tz);
+    const float4 *rec = g + r00 + di * dI + dj * dJ + dk * Q;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+#pragma unroll
+    for (int q4 = 0; q4 < Q; ++q4) {
+      const float4 v = rec[q4];
+      const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int q = 4 * q4 + l;
+        if (q < 3 * NS) {
+          const int sidx = q / 3, comp = q % 3;
+          if (comp == 0)
+            cx = __builtin_fmaf(wt[sidx], vv[l], cx);
+          else if (comp == 1)
+            cy = __builtin_fmaf(wt[sidx], vv[l], cy);
+          else
+            cz = __builtin_fmaf(wt[sidx], vv[l], cz);
+        }
+      }
+    }
+    ax = __builtin_fmaf(wc, cx, ax);
+    ay = __builtin_fmaf(wc, cy, ay);
+    az = __builtin_fmaf(wc, cz, az);
+  }
+  }
+  ex = ax;
+  ey = ay;
+  ez = az;
+}
+
 // tracer3d._in_metal_3d (tracer3d.py:174-189)
 __device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, double x, double y,
                                             double z, int64_t nx, int64_t ny, int64_t nz) {
@@ -183,21 +250,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
   constexpr int NW = NS > 1 ? NS - 1 : 1;
   double w[NW];
+  float wt[NS]; // FP32 mode: per-set composition weights
   double wtt = __builtin_nan("");
   auto weights = [&](double tt) {
     if (!(tt == wtt)) {
+      if constexpr (PREC == IG_PREC_F32) {
 #pragma unroll
-      for (int k = 0; k < NW; ++k)
-        if (k < K)
-          w[k] = PREC == IG_PREC_F32 ? wave_eval_fast<true>(W, f, k, tt)
-                                     : wave_eval<true>(W, f, k, tt);
+        for (int q = 0; q < NS; ++q) {
+          double v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+          v = v * f.scale[q];
+          if (tscale) v = v * tscale[q];
+          wt[q] = (float)v;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+          if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
+      }
       wtt = tt;
     }
   };
   auto efield = [&](double px, double py, double pz, double tt, double &a, double &b,
                     double &c) {
     weights(tt);
-    xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
+    if constexpr (PREC == IG_PREC_F32)
+      xyz_field_f32<NS>(f, px * inv_h, py * inv_h, pz * inv_h, wt, a, b, c);
+    else
+      xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
   };
   auto write = [&](int row, double tus) {
     double *r = e.rec + ((size_t)i * p.rec_rows + row) * p.rec_cols;
@@ -254,43 +333,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         to = t;
         const double t_us = tob + t * 1e6;
         const double h_us = dt * 1e6;
-        double ax1, ay1, az1, ax2, ay2, az2, ax3, ay3, az3, ax4, ay4, az4, ea, eb, ec;
-        efield(x, y, z, t_us, ea, eb, ec);
-        ax1 = qm * 1e3 * ea;
-        ay1 = qm * 1e3 * eb;
-        az1 = qm * 1e3 * ec;
-        const double k1x = vx, k1y = vy, k1z = vz;
-        double px = x + 0.5 * dt * k1x * c, py = y + 0.5 * dt * k1y * c,
-               pz = z + 0.5 * dt * k1z * c;
-        efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
-        ax2 = qm * 1e3 * ea;
-        ay2 = qm * 1e3 * eb;
-        az2 = qm * 1e3 * ec;
-        const double k2x = vx + 0.5 * dt * ax1, k2y = vy + 0.5 * dt * ay1,
-                     k2z = vz + 0.5 * dt * az1;
-        px = x + 0.5 * dt * k2x * c;
-        py = y + 0.5 * dt * k2y * c;
-        pz = z + 0.5 * dt * k2z * c;
-        efield(px, py, pz, t_us + 0.5 * h_us, ea, eb, ec);
-        ax3 = qm * 1e3 * ea;
-        ay3 = qm * 1e3 * eb;
-        az3 = qm * 1e3 * ec;
-        const double k3x = vx + 0.5 * dt * ax2, k3y = vy + 0.5 * dt * ay2,
-                     k3z = vz + 0.5 * dt * az2;
-        px = x + dt * k3x * c;
-        py = y + dt * k3y * c;
-        pz = z + dt * k3z * c;
-        efield(px, py, pz, t_us + h_us, ea, eb, ec);
-        ax4 = qm * 1e3 * ea;
-        ay4 = qm * 1e3 * eb;
-        az4 = qm * 1e3 * ec;
-        const double k4x = vx + dt * ax3, k4y = vy + dt * ay3, k4z = vz + dt * az3;
-        x = x + dt / 6.0 * (k1x + 2 * k2x + 2 * k3x + k4x) * c;
-        y = y + dt / 6.0 * (k1y + 2 * k2y + 2 * k3y + k4y) * c;
-        z = z + dt / 6.0 * (k1z + 2 * k2z + 2 * k3z + k4z) * c;
-        vx = vx + dt / 6.0 * (ax1 + 2 * ax2 + 2 * ax3 + ax4);
-        vy = vy + dt / 6.0 * (ay1 + 2 * ay2 + 2 * ay3 + ay4);
-        vz = vz + dt / 6.0 * (az1 + 2 * az2 + 2 * az3 + az4);
+        // the four RK4 stages as one loop around one field evaluation
+        // (tracer3d.py:395-417): stage s is evaluated at point p_s and lab
+        // time t_s; k_1 = v, p_{s+1} = x + cf_s dt k_s c and
+        // k_{s+1} = v + cf_s dt a_s with cf = 1/2, 1/2, 1; the weighted sums
+        // k_1 + 2 k_2 + 2 k_3 + k_4 accumulate left to right, the
+        // reference's order
+        double px = x, py = y, pz = z;
+        double kx = vx, ky = vy, kz = vz;
+        double skx = 0.0, sky = 0.0, skz = 0.0, sax = 0.0, say = 0.0, saz = 0.0;
+#pragma unroll 1
+        for (int st = 0; st < 4; ++st) {
+          const double tt = st == 0 ? t_us : (st == 3 ? t_us + h_us : t_us + 0.5 * h_us);
+          double ea, eb, ec;
+          efield(px, py, pz, tt, ea, eb, ec);
+          const double ax = qm * 1e3 * ea, ay = qm * 1e3 * eb, az = qm * 1e3 * ec;
+          if (st == 0) {
+            skx = kx;
+            sky = ky;
+            skz = kz;
+            sax = ax;
+            say = ay;
+            saz = az;
+          } else if (st == 3) {
+            skx = skx + kx;
+            sky = sky + ky;
+            skz = skz + kz;
+            sax = sax + ax;
+            say = say + ay;
+            saz = saz + az;
+          } else {
+            skx = skx + 2 * kx;
+            sky = sky + 2 * ky;
+            skz = skz + 2 * kz;
+            sax = sax + 2 * ax;
+            say = say + 2 * ay;
+            saz = saz + 2 * az;
+          }
+          if (st < 3) {
+            const double cf = st < 2 ? 0.5 : 1.0;
+            px = x + cf * dt * kx * c;
+            py = y + cf * dt * ky * c;
+            pz = z + cf * dt * kz * c;
+            kx = vx + cf * dt * ax;
+            ky = vy + cf * dt * ay;
+            kz = vz + cf * dt * az;
+          }
+        }
+        x = x + dt / 6.0 * skx * c;
+        y = y + dt / 6.0 * sky * c;
+        z = z + dt / 6.0 * skz * c;
+        vx = vx + dt / 6.0 * sax;
+        vy = vy + dt / 6.0 * say;
+        vz = vz + dt / 6.0 * saz;
         t += dt;
         s += 1;
         stage = kPost;

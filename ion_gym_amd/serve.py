@@ -215,15 +215,15 @@ class FlightServer:
             return []
         import ctypes as C
         t0 = time.perf_counter()
-        steps0 = self.slab.steps() if self.windows == 0 else self._steps
+        # step total after admission (loading resets the new slots' counts)
+        steps0 = self.slab.steps()
         self.engine.fly(self.fields, self.flight, self.slab)
         live = (C.c_int64 * self.max_requests)()
         L.check(L.lib().ig_ensemble_live_by_tune(self.engine.h, self.slab.id,
                                                  self.max_requests, live),
                 "ig_ensemble_live_by_tune")
         self.device_s += time.perf_counter() - t0
-        self._steps = self.slab.steps()
-        self.ion_steps += self._steps - steps0
+        self.ion_steps += self.slab.steps() - steps0
         self.windows += 1
         finished = []
         for row, req in list(self.active.items()):

@@ -79,3 +79,51 @@ def test_admission_waits_for_free_slots(engine):
     assert reqs[2].t_admit >= min(reqs[0].t_done, reqs[1].t_done)
     srv.release()
     fp.release()
+
+
+def test_slim_requests_are_the_same_alone_and_in_the_mix(engine):
+    """3-D route, 11 FP32 sets, hard-sphere N2: four requests of
+    different TW amplitude, speed and m/z give bit-identical results flown
+    alone and flown together with windows of another length."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    from ion_gym_amd.serve import FlightRequest, FlightServer, make_tune
+    c = configs.slim_3d(coarse=4)
+    fp = engine.upload_xyz(c["pack"], "f32")
+
+    def reqs():
+        rng = np.random.default_rng(8)
+        out = []
+        for k in range(4):
+            ions, tune = configs.slim_request(c, rng, n_ions=1500,
+                                              t_flight_us=0.3, rng_key=k + 1)
+            out.append(FlightRequest(ions=ions,
+                                     tune=make_tune(c["pack"], **tune)))
+        return out
+
+    first = reqs()[0].tune
+    fl = Flight(first["dt"], first["t_max"], gas=c["gas"])
+    alone = []
+    for r in reqs():
+        srv = FlightServer(engine, fp, fl, capacity=1500, window=400)
+        srv.submit(r)
+        srv.drain()
+        alone.append(r.result)
+        srv.release()
+    srv = FlightServer(engine, fp, fl, capacity=3000, window=170)
+    mix = reqs()
+    for r in mix[:2]:
+        srv.submit(r)
+    srv.step()
+    srv.step()
+    for r in mix[2:]:
+        srv.submit(r)
+    srv.drain()
+    assert len(srv.done) == 4
+    for r, a in zip(mix, alone):
+        for key in ("x", "y", "z", "vx", "vy", "vz", "t_us", "fate", "ncol",
+                    "steps"):
+            assert np.array_equal(r.result[key], a[key]), (r.rid, key)
+        assert r.result["ncol"].sum() > 0
+    srv.release()
+    fp.release()

@@ -161,3 +161,54 @@ def test_3d_hard_sphere_thermalises_to_three_halves_kT(engine, prec):
         assert 0.5 * m * np.mean(g[k] ** 2) * 1e6 / (0.5 * kT) == \
             pytest.approx(1.0, abs=0.015), k
     assert g["ncol"].mean() > 300
+
+
+# ------------------------------------------------------------ SLIM board
+@pytest.fixture(scope="module")
+def slim():
+    from ion_gym_amd import configs
+    return configs.slim_3d(coarse=4)
+
+
+def _tuned_pack(c, tune):
+    """The board pack with one request's waveforms baked in."""
+    pk = dict(c["pack"])
+    for k in ("ch_om", "ch_ph", "ch_amp", "ch_off", "ch_duty"):
+        pk[k] = np.asarray(tune[k], float)
+    return pk
+
+
+def test_slim_fp64_matches_oracle(engine, slim):
+    """11 field sets (8 square TW phases, 2 RF, static), vacuum: ion by
+    ion against the oracle to 1e-9 relative."""
+    from ion_gym_amd import configs
+    rng = np.random.default_rng(21)
+    ions, tune = configs.slim_request(slim, rng, n_ions=300, t_flight_us=0.5)
+    pk = _tuned_pack(slim, tune)
+    fp = engine.upload_xyz(pk, "f64")
+    from ion_gym_amd.engine import Flight
+    g = engine.fly_batch(fp, Flight(tune["dt"], tune["t_max"]), ions)
+    o = O.fly_xyz(pk, dict(dt=tune["dt"], t_max=tune["t_max"]), ions)
+    _parity(g, o)
+    fp.release()
+
+
+def test_slim_fp32_basis_error_against_fp64(engine, slim):
+    """The FP32-basis composition (per corner, FP32) against FP64 over
+    1 us of flight in the traveling wave: positions within 1e-4 mm."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    rng = np.random.default_rng(22)
+    ions, tune = configs.slim_request(slim, rng, n_ions=4000, t_flight_us=1.0)
+    pk = _tuned_pack(slim, tune)
+    out = {}
+    for prec in ("f64", "f32"):
+        fp = engine.upload_xyz(pk, prec)
+        out[prec] = engine.fly_batch(fp, Flight(tune["dt"], tune["t_max"]),
+                                     ions)
+        fp.release()
+    a, b = out["f32"], out["f64"]
+    same = a["fate"] == b["fate"]
+    assert same.mean() > 0.99
+    for k in ("x", "y", "z"):
+        assert np.max(np.abs(a[k][same] - b[k][same])) < 1e-4, k
