@@ -189,6 +189,9 @@ typedef struct {
   int32_t occupancy;   /* register-budget variant of the hard-sphere
                           kernel: waves per SIMD 1, 2 or 3 (0 = engine
                           default); results do not depend on it */
+  int32_t block;       /* lanes per workgroup: 64, 128, 192 or 256
+                          (0 = engine default 256); results do not depend
+                          on it */
 } ig_flight_params;
 
 /* One tune: the parameters one request or ensemble of a batched launch
@@ -246,6 +249,9 @@ typedef struct {
 const char *ig_last_error(void);
 int ig_abi_version(void);
 int ig_device_count(int *out);
+/* The device's marketing name and compute-unit count (the autotuner keys
+ * its measurements on them; no counterpart in the reference). */
+int ig_device_info(int device, char *name, int32_t name_len, int32_t *cu_count);
 int ig_engine_create(int device, ig_engine **out);
 int ig_engine_destroy(ig_engine *h);
 int ig_sync(ig_engine *h);
@@ -298,6 +304,17 @@ int ig_fly(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
  * fly_fn(i) over a list of ions (ion_gym/physics/build_rz.py:675). */
 int ig_fly_batch(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
                  const ig_ions *ions, ig_results *out);
+
+/* n_windows consecutive ig_fly windows captured once into a HIP graph and
+ * replayed: the first call with given kernel arguments (field pack, flight
+ * parameters, ensemble state pointers, size, block, n_windows) captures,
+ * later calls replay the cached graph; a changed argument captures anew.
+ * Equivalent to n_windows ig_fly calls with no host work between them
+ * (the window loop of a long flight), with one launch instead of
+ * n_windows.  The graph holds only the flight kernels.  Replaces the
+ * host loop over windows (ion_gym/physics/ensemble_driver.py:381-529). */
+int ig_fly_graph(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
+                 ig_ensemble_id e, int32_t n_windows);
 
 /* Kernel timing of the last ig_fly on the engine's stream (HIP events
  * recorded around the launches), in milliseconds. */

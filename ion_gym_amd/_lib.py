@@ -57,7 +57,7 @@ class FlightParams(C.Structure):
                 ("pl_w", (C.c_double * 4) * MAX_PLANES),
                 ("pl_kind", C.c_int32 * MAX_PLANES),
                 ("rng_key", C.c_uint64), ("coll_batch", C.c_int32),
-                ("occupancy", C.c_int32)]
+                ("occupancy", C.c_int32), ("block", C.c_int32)]
 
 
 class Tune(C.Structure):
@@ -93,6 +93,8 @@ SIGNATURES = {
     "ig_last_error": (C.c_char_p, []),
     "ig_abi_version": (C.c_int, []),
     "ig_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "ig_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_int32,
+                                 C.POINTER(C.c_int32)]),
     "ig_engine_create": (C.c_int, [C.c_int, C.POINTER(_VP)]),
     "ig_engine_destroy": (C.c_int, [_VP]),
     "ig_sync": (C.c_int, [_VP]),
@@ -119,6 +121,8 @@ SIGNATURES = {
                          C.c_int64]),
     "ig_fly_batch": (C.c_int, [_VP, C.c_int64, C.POINTER(FlightParams),
                                C.POINTER(Ions), C.POINTER(Results)]),
+    "ig_fly_graph": (C.c_int, [_VP, C.c_int64, C.POINTER(FlightParams),
+                               C.c_int64, C.c_int32]),
     "ig_last_fly_ms": (C.c_int, [_VP, C.POINTER(C.c_double)]),
 }
 

@@ -224,17 +224,16 @@ def quad_3d(n_ions=100_000, seed=2, coarse=1, a=0.02, q_ref_mz=500.0,
                 a=a)
 
 
-def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
-             K0=1.5, P_torr=4.0, T_k=300.0, dt_frac=0.1):
-    """Drift tube with SDS: a 100-ring stack (0.4 mm rings at 0.8 mm pitch,
-    inner radius 8 mm) on an 801 x 101 r-z grid @ 0.1 mm whose linear
-    ladder gives a uniform 1 V/mm base field; N2 at 4 Torr and 300 K; the
-    ion's declared K0 = 1.5 cm^2/(V s); births on the axis region at
-    x = 5 mm with 300 K velocities; end plates at 80 V and 0 V close the
-    stack (the exit plate at x = 80 mm is the detector).  The field
-    strength E/N (Td) sets the
-    static set's scale (``set_scale``); dt = dt_frac / gamma."""
-    from .physics import sds as sds_mod
+_DRIFT_MODELS = {}
+
+
+def _drift_model(coarse):
+    """The drift tube's field model (cached per grid coarsening): a
+    100-ring stack (0.4 mm rings at 0.8 mm pitch, inner radius 8 mm) on an
+    801 x 101 r-z grid @ 0.1 mm whose linear ladder gives a uniform 1 V/mm
+    base field, closed by end plates at 80 V and 0 V (labels 101, 102)."""
+    if coarse in _DRIFT_MODELS:
+        return _DRIFT_MODELS[coarse]
     h = 0.1 * coarse
     nz, nr = 800 // coarse + 1, 100 // coarse + 1
     elec = []
@@ -252,13 +251,20 @@ def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
     dc[102] = 0.0
     (A,) = solve_rz(lab, [dc], h)
     model = _model_from_potentials(lab, h, A, [], [], [], [], [])
+    _DRIFT_MODELS[coarse] = model
+    return model
+
+
+def _drift_ions(n_ions, seed, mz, e_td, K0, P_torr, T_k, dt_frac, stats,
+                m_gas):
+    """Births on the axis region at x = 5 mm with thermal velocities and the
+    ions' SDS numbers; the field strength E/N (Td) as the 1 V/mm base
+    field's scale; dt = dt_frac / gamma."""
+    from .physics import sds as sds_mod
     n_gas = P_torr * (101325.0 / 760.0) / (K_B * T_k)   # 1/m^3
     E_vpm = e_td * 1e-21 * n_gas                         # V/m
     scale = E_vpm * 1e-3 / 1.0                           # base is 1 V/mm
-    g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
-    stats = sds_mod.default_statistics()
-    p = sds_mod.transport_params(mz, 1, g["m_gas"], T_k, P_torr, K0,
-                                 stats=stats)
+    p = sds_mod.transport_params(mz, 1, m_gas, T_k, P_torr, K0, stats=stats)
     dt = dt_frac / p["damping"]
     rng = np.random.default_rng(seed)
     m = np.full(n_ions, mz)
@@ -272,6 +278,24 @@ def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
                 seed=seed + np.arange(n_ions, dtype=np.int64),
                 sds=np.tile([p["damping"], p["mfp_mm"], p["V_mm_us"],
                              p["log_mr_ratio"]], (n_ions, 1)))
+    return ions, dt, scale, E_vpm, p
+
+
+def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
+             K0=1.5, P_torr=4.0, T_k=300.0, dt_frac=0.1):
+    """Drift tube with SDS: the 100-ring stack of _drift_model (uniform
+    1 V/mm base field); N2 at 4 Torr and 300 K; the ion's declared
+    K0 = 1.5 cm^2/(V s); births on the axis region at x = 5 mm with 300 K
+    velocities; the exit plate at x = 80 mm is the detector.  The field
+    strength E/N (Td) sets the static set's scale (``set_scale``);
+    dt = dt_frac / gamma."""
+    from .physics import sds as sds_mod
+    model = _drift_model(coarse)
+    g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
+    stats = sds_mod.default_statistics()
+    ions, dt, scale, E_vpm, p = _drift_ions(n_ions, seed, mz, e_td, K0,
+                                            P_torr, T_k, dt_frac, stats,
+                                            g["m_gas"])
     gas = dict(model=2, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
                c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
                flow=(0.0, 0.0, 0.0), sds_stats=stats)
@@ -279,6 +303,42 @@ def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
                 t_max=2000.0, gas=gas, scale=scale, E_vpmm=E_vpm * 1e-3,
                 K_mm2_per_V_us=p["K_mm2_per_V_us"], D=p["D_mm2_per_us"],
                 transport=p)
+
+
+def drift_batch_rz(n_ens=64, seed=4000, coarse=1, size_lo=1e3, size_hi=2e5,
+                   td_lo=5.0, td_hi=20.0, mz_lo=100.0, mz_hi=1000.0, K0=1.5,
+                   P_torr=4.0, T_k=300.0, dt_frac=0.1, t_max=2000.0):
+    """Batched drift tube: n_ens ensembles for one launch on the drift_rz
+    stack, each with its own size (log-uniform in size_lo-size_hi), field
+    strength (uniform td_lo-td_hi Td, as the static set's scale) and m/z
+    (uniform mz_lo-mz_hi), hence its own SDS numbers and dt = dt_frac /
+    gamma; seeds seed + 1000 e.  Returns the model, the gas and, per
+    ensemble, its births, tune (dt, t_max, set_scale) and the transport
+    numbers (K, D) the drift and diffusion are checked against."""
+    from .physics import sds as sds_mod
+    model = _drift_model(coarse)
+    g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
+    stats = sds_mod.default_statistics()
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(math.log(size_lo), math.log(size_hi),
+                                        n_ens))).astype(int)
+    tds = rng.uniform(td_lo, td_hi, n_ens)
+    mzs = rng.uniform(mz_lo, mz_hi, n_ens)
+    ens = []
+    for e in range(n_ens):
+        ions, dt, scale, E_vpm, p = _drift_ions(
+            int(sizes[e]), seed + 1000 * e, float(mzs[e]), float(tds[e]), K0,
+            P_torr, T_k, dt_frac, stats, g["m_gas"])
+        ens.append(dict(ions=ions, n=int(sizes[e]), td=float(tds[e]),
+                        mz=float(mzs[e]),
+                        tune=dict(dt=dt, t_max=t_max, set_scale=[scale]),
+                        E_vpmm=E_vpm * 1e-3, K_mm2_per_V_us=p["K_mm2_per_V_us"],
+                        D=p["D_mm2_per_us"]))
+    gas = dict(model=2, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+               flow=(0.0, 0.0, 0.0), sds_stats=stats)
+    return dict(name="drift_batch_rz", route="rz", model=model, gas=gas,
+                ensembles=ens)
 
 
 def hs_drift_rz(n_ions=2000, seed=5000, coarse=2, e_vpmm=1.0, mz=500.0,

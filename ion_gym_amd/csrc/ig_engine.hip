@@ -87,6 +87,13 @@ struct Fields {
   int64_t bytes = 0;
 };
 
+// a captured loop of flight windows and the kernel arguments it replays
+struct GraphEntry {
+  std::vector<uint8_t> key;
+  hipGraphExec_t exec;
+};
+constexpr size_t kMaxGraphs = 16;
+
 struct Ensemble {
   int64_t cap = 0, n = 0;
   int32_t route = IG_ROUTE_RZ; // clock of k.t: r-z us, 3-D s
@@ -120,7 +127,15 @@ struct ig_engine {
   std::map<int64_t, std::unique_ptr<Fields>> fields;
   std::map<int64_t, std::unique_ptr<Ensemble>> ens;
   int block = 256;
+  std::vector<GraphEntry> graphs; // captured window loops (ig_fly_graph)
 };
+
+namespace {
+void drop_graphs(ig_engine *h) {
+  for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+}
+} // namespace
 
 namespace {
 
@@ -154,6 +169,15 @@ int ig_device_count(int *out) {
   return 0;
 }
 
+int ig_device_info(int device, char *name, int32_t name_len, int32_t *cu_count) {
+  if (!name || name_len < 1 || !cu_count) return fail(IG_E_ARG, "ig_device_info: null argument");
+  hipDeviceProp_t pr;
+  HIPX(hipGetDeviceProperties(&pr, device));
+  std::snprintf(name, (size_t)name_len, "%s %s", pr.name, pr.gcnArchName);
+  *cu_count = pr.multiProcessorCount;
+  return 0;
+}
+
 int ig_engine_create(int device, ig_engine **out) {
   if (!out) return fail(IG_E_ARG, "ig_engine_create: null out");
   *out = nullptr;
@@ -177,6 +201,7 @@ int ig_engine_destroy(ig_engine *h) {
   if (!h) return 0;
   (void)hipSetDevice(h->device);
   (void)hipStreamSynchronize(h->stream);
+  drop_graphs(h);
   h->fields.clear();
   h->ens.clear();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -317,6 +342,7 @@ int ig_fields_release(ig_engine *h, ig_fields_id id) {
   if (it == h->fields.end()) return fail(IG_E_STATE, "ig_fields_release: unknown field id");
   if (int rc = set_dev(h)) return rc;
   HIPX(hipStreamSynchronize(h->stream));
+  drop_graphs(h);
   h->fields.erase(it);
   return 0;
 }
@@ -387,6 +413,7 @@ int ig_ensemble_release(ig_engine *h, ig_ensemble_id id) {
   if (it == h->ens.end()) return fail(IG_E_STATE, "ig_ensemble_release: unknown ensemble id");
   if (int rc = set_dev(h)) return rc;
   HIPX(hipStreamSynchronize(h->stream));
+  drop_graphs(h);
   h->ens.erase(it);
   return 0;
 }
@@ -749,19 +776,25 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
   if (p->occupancy < 0 || p->occupancy > 3)
     return fail(IG_E_ARG, "ig_fly: occupancy must be 0 (default), 1, 2 or 3");
   k.occupancy = p->occupancy;
+  if (p->block != 0 && (p->block < 64 || p->block > 256 || p->block % 64 != 0))
+    return fail(IG_E_ARG, "ig_fly: block must be 0 (default), 64, 128, 192 or 256");
   return 0;
 }
 
-int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensemble_id eid) {
-  if (!h || !p) return fail(IG_E_ARG, "ig_fly: null argument");
+// Validation and device-side setup shared by ig_fly and ig_fly_graph:
+// the kernel arguments for one window of (fid, p, eid).
+static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params *p,
+                          ig_ensemble_id eid, const char *who, const FieldsK *&fk,
+                          FlightK &k, Ensemble *&E) {
+  const std::string w(who);
+  if (!h || !p) return fail(IG_E_ARG, w + ": null argument");
   auto fit = h->fields.find(fid);
-  if (fit == h->fields.end()) return fail(IG_E_STATE, "ig_fly: unknown field id");
-  Ensemble *E = find_ens(h, eid);
-  if (!E) return fail(IG_E_STATE, "ig_fly: unknown ensemble id");
-  FlightK k;
+  if (fit == h->fields.end()) return fail(IG_E_STATE, w + ": unknown field id");
+  E = find_ens(h, eid);
+  if (!E) return fail(IG_E_STATE, w + ": unknown ensemble id");
   if (int rc = build_flight(p, k)) return rc;
   if (p->rec_rows > 0 && (p->rec_rows != E->rec_rows || E->k.rec == nullptr))
-    return fail(IG_E_ARG, "ig_fly: rec_rows differs from the ensemble's record buffer");
+    return fail(IG_E_ARG, w + ": rec_rows differs from the ensemble's record buffer");
   k.rec_cols = E->rec_cols;
   if (int rc = set_dev(h)) return rc;
   if (k.gas == IG_GAS_SDS) {
@@ -780,18 +813,84 @@ int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensembl
     }
     k.sds_stats = (const double *)h->sds.p;
   }
-  const FieldsK &f = fit->second->k;
-  E->route = f.route;
-  if (k.occupancy == 0) k.occupancy = default_occupancy(f.prec);
+  fk = &fit->second->k;
+  E->route = fk->route;
+  if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec);
+  return 0;
+}
+
+static int launch_error(int rc, const char *who) {
+  if (rc == -2) return fail(IG_E_ARG, std::string(who) + ": no kernel instantiated for this field pack");
+  hipError_t e = hipGetLastError();
+  return fail(IG_E_HIP, std::string(who) + ": launch failed: " + hipGetErrorString(e));
+}
+
+int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensemble_id eid) {
+  const FieldsK *f = nullptr;
+  FlightK k;
+  Ensemble *E = nullptr;
+  if (int rc = prepare_flight(h, fid, p, eid, "ig_fly", f, k, E)) return rc;
   HIPX(hipEventRecord(h->ev0, h->stream));
-  int rc = launch_route(f, k, E->k, E->n, h->block, h->stream);
+  int rc = launch_route(*f, k, E->k, E->n, p->block > 0 ? p->block : h->block, h->stream);
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
-  if (rc == -2) return fail(IG_E_ARG, "ig_fly: no kernel instantiated for this field pack");
-  if (rc) {
-    hipError_t e = hipGetLastError();
-    return fail(IG_E_HIP, std::string("ig_fly: launch failed: ") + hipGetErrorString(e));
+  if (rc) return launch_error(rc, "ig_fly");
+  return 0;
+}
+
+int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensemble_id eid,
+                 int32_t n_windows) {
+  if (n_windows < 1 || n_windows > 4096)
+    return fail(IG_E_ARG, "ig_fly_graph: n_windows must be in [1, 4096]");
+  const FieldsK *f = nullptr;
+  FlightK k;
+  Ensemble *E = nullptr;
+  if (int rc = prepare_flight(h, fid, p, eid, "ig_fly_graph", f, k, E)) return rc;
+  const int block = p->block > 0 ? p->block : h->block;
+  // the graph is a function of the exact kernel arguments and launch shape
+  std::vector<uint8_t> key(sizeof(FieldsK) + sizeof(FlightK) + sizeof(EnsK) + 3 * 8);
+  {
+    uint8_t *q = key.data();
+    std::memcpy(q, f, sizeof(FieldsK));
+    q += sizeof(FieldsK);
+    std::memcpy(q, &k, sizeof(FlightK));
+    q += sizeof(FlightK);
+    std::memcpy(q, &E->k, sizeof(EnsK));
+    q += sizeof(EnsK);
+    const int64_t tail[3] = {E->n, block, n_windows};
+    std::memcpy(q, tail, sizeof(tail));
   }
+  hipGraphExec_t exec = nullptr;
+  for (auto &g : h->graphs)
+    if (g.key == key) exec = g.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    HIPX(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int32_t wdx = 0; wdx < n_windows && rc == 0; ++wdx)
+      rc = launch_route(*f, k, E->k, E->n, block, h->stream);
+    hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return launch_error(rc, "ig_fly_graph");
+    }
+    if (ce != hipSuccess)
+      return fail(IG_E_HIP, std::string("ig_fly_graph: capture failed: ") + hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess)
+      return fail(IG_E_HIP, std::string("ig_fly_graph: instantiate failed: ") +
+                                hipGetErrorString(ie));
+    if (h->graphs.size() >= kMaxGraphs) {
+      (void)hipGraphExecDestroy(h->graphs.front().exec);
+      h->graphs.erase(h->graphs.begin());
+    }
+    h->graphs.push_back(GraphEntry{std::move(key), exec});
+  }
+  HIPX(hipEventRecord(h->ev0, h->stream));
+  HIPX(hipGraphLaunch(exec, h->stream));
+  HIPX(hipEventRecord(h->ev1, h->stream));
+  h->timed = true;
   return 0;
 }
 

@@ -175,7 +175,7 @@ class Flight:
 
     def __init__(self, dt, t_max, *, max_steps=0, rec_every=1, rec_rows=0,
                  channels=(), gas=None, bounds=None, planes=(), rng_key=0,
-                 coll_batch=0, occupancy=0):
+                 coll_batch=0, occupancy=0, block=0):
         self.dt = float(dt)
         self.t_max = float(t_max)
         self.max_steps = int(max_steps)
@@ -188,6 +188,18 @@ class Flight:
         self.rng_key = int(rng_key)
         self.coll_batch = int(coll_batch)
         self.occupancy = int(occupancy)
+        self.block = int(block)
+
+    def launch(self, **kw):
+        """A copy with other launch settings (block, occupancy,
+        coll_batch); results do not depend on them."""
+        import copy
+        f = copy.copy(self)
+        for k, v in kw.items():
+            if k not in ("block", "occupancy", "coll_batch"):
+                raise ValueError(f"not a launch setting: {k}")
+            setattr(f, k, int(v))
+        return f
 
     def struct(self, keep):
         p = L.FlightParams()
@@ -227,6 +239,7 @@ class Flight:
         p.rng_key = self.rng_key & 0xFFFFFFFFFFFFFFFF
         p.coll_batch = self.coll_batch
         p.occupancy = self.occupancy
+        p.block = self.block
         return p
 
 
@@ -378,6 +391,15 @@ class Engine:
         L.check(L.lib().ig_fly(self.h, fields.id, C.byref(p), ens.id),
                 "ig_fly")
 
+    def fly_graph(self, fields, flight, ens, n_windows):
+        """Advance ``ens`` by ``n_windows`` windows of ``flight.max_steps``
+        steps as one replay of a captured HIP graph (captured on the first
+        call with these arguments; asynchronous)."""
+        keep = _Keep()
+        p = flight.struct(keep)
+        L.check(L.lib().ig_fly_graph(self.h, fields.id, C.byref(p), ens.id,
+                                     int(n_windows)), "ig_fly_graph")
+
     def last_fly_ms(self):
         out = C.c_double()
         L.check(L.lib().ig_last_fly_ms(self.h, C.byref(out)),
@@ -412,3 +434,25 @@ def device_count():
     n = C.c_int()
     L.check(L.lib().ig_device_count(C.byref(n)), "ig_device_count")
     return n.value
+
+
+def device_info(device=0):
+    """(name, compute units) of a device."""
+    buf = C.create_string_buffer(256)
+    cu = C.c_int32()
+    L.check(L.lib().ig_device_info(int(device), buf, 256, C.byref(cu)),
+            "ig_device_info")
+    return buf.value.decode(errors="replace"), int(cu.value)
+
+
+def read_slice(engine, ens, off, n):
+    """Final state of ions [off, off + n) of an ensemble (no records)."""
+    res = {k: np.zeros(n) for k in ("x", "y", "z", "vx", "vy", "vz", "t_us")}
+    res.update(fate=np.zeros(n, np.int32), face=np.zeros(n, np.int32),
+               label=np.zeros(n, np.int32), ncol=np.zeros(n, np.int64),
+               steps=np.zeros(n, np.int64), nrec=np.zeros(n, np.int32))
+    res["records"] = None
+    o = _results_struct(res)
+    L.check(L.lib().ig_ensemble_read_at(engine.h, ens.id, int(off), int(n),
+                                        o), "ig_ensemble_read_at")
+    return res

@@ -242,18 +242,8 @@ class FlightServer:
         return finished
 
     def engine_read(self, off, n):
-        import ctypes as C  # noqa: F401
-        from .engine import _results_struct
-        res = {k: np.zeros(n) for k in ("x", "y", "z", "vx", "vy", "vz",
-                                        "t_us")}
-        res.update(fate=np.zeros(n, np.int32), face=np.zeros(n, np.int32),
-                   label=np.zeros(n, np.int32), ncol=np.zeros(n, np.int64),
-                   steps=np.zeros(n, np.int64), nrec=np.zeros(n, np.int32))
-        res["records"] = None
-        o = _results_struct(res)
-        L.check(L.lib().ig_ensemble_read_at(self.engine.h, self.slab.id, off,
-                                            n, o), "ig_ensemble_read_at")
-        return res
+        from .engine import read_slice
+        return read_slice(self.engine, self.slab, off, n)
 
     @property
     def busy(self):
