@@ -18,7 +18,9 @@ and the serving leg: flight requests on the 3-D SLIM traveling-wave board
 1e3-5e5 ions per request with their own TW amplitude, speed and m/z)
 arriving as a Poisson stream at a set fraction of the engine's solo
 throughput, served by continuous batching; p50/p99 request latency and
-the sustained ion-steps/s.
+the sustained ion-steps/s.  The precision object states the FP32-basis
+error against FP64 on the full-grid 3-D quadrupole (100,000 ions, 2,000
+RF cycles): transmission of both and per-ion position / TOF differences.
 """
 import argparse
 import json
@@ -163,6 +165,46 @@ def run_serving(eng, n_req, load, flight_us, window, capacity, seed):
     return out
 
 
+def run_precision(eng, n_ions, n_cycles):
+    """The FP32-basis error on the 3-D quadrupole at its full grid
+    (161 x 161 x 401 @ 0.05 mm, RF 1 MHz 200 V0-p with DC at a = 0.02,
+    m/z 400-600, vacuum): the same births flown with FP64 and FP32-basis
+    packs; transmission (exit through the far end plane) of each and the
+    per-ion position and time-of-flight differences of ions that share
+    their fate."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    c = configs.quad_3d(n_ions=n_ions, n_cycles=n_cycles)
+    z_end = (c["pack"]["EAx"].shape[2] - 1) * c["pack"]["h_mm"]
+    out = {}
+    res = {}
+    for prec in ("f64", "f32"):
+        fp = eng.upload_xyz(c["pack"], prec)
+        t0 = time.perf_counter()
+        g = eng.fly_batch(fp, Flight(c["dt"], c["t_max"]), c["ions"])
+        el = time.perf_counter() - t0
+        fp.release()
+        res[prec] = g
+        out[f"{prec}_ion_steps_per_s"] = float(g["steps"].sum()) / el
+        out[f"transmission_{prec}"] = float(np.mean(
+            (g["fate"] == 1) & (g["z"] >= z_end - 1e-9)))
+    a, b = res["f32"], res["f64"]
+    same = a["fate"] == b["fate"]
+    d = np.sqrt((a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 +
+                (a["z"] - b["z"]) ** 2)[same]
+    dt_rel = (np.abs(a["t_us"] - b["t_us"]) / np.abs(b["t_us"]))[same]
+    out.update({
+        "workload": "quad_3d (161x161x401 @ 0.05 mm, 3 sets, vacuum)",
+        "ions": int(n_ions), "rf_cycles": int(n_cycles),
+        "same_fate_fraction": float(same.mean()),
+        "pos_err_mm_median": float(np.median(d)),
+        "pos_err_mm_p99": float(np.percentile(d, 99)),
+        "pos_err_mm_max": float(d.max()),
+        "tof_rel_err_median": float(np.median(dt_rel)),
+        "tof_rel_err_max": float(dt_rel.max())})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +220,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32000)
     ap.add_argument("--cpu-window", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32-vs-FP64 quadrupole error leg")
+    ap.add_argument("--precision-ions", type=int, default=100_000)
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving leg (SLIM board, Poisson arrivals)")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -243,6 +288,12 @@ def main():
                "sample": f"{args.cpu_sample} ions of the same workload x "
                          f"{args.cpu_window} steps (FP64, HS, Philox), "
                          f"{el:.1f} s"}
+    precision = None
+    if not args.no_precision:
+        print(f"[bench] precision: quad_3d full grid, "
+              f"{args.precision_ions} ions, FP32 basis vs FP64",
+              file=sys.stderr, flush=True)
+        precision = run_precision(eng, args.precision_ions, 2000)
     serving = None
     if not args.no_serve:
         print(f"[bench] serving: {args.serve_requests} SLIM requests at "
@@ -281,6 +332,7 @@ def main():
                      "bytes_per_ion_step": per_step},
         "cpu_baseline": cpu,
         "serving": serving,
+        "precision": precision,
     }
     print(json.dumps(line), flush=True)
     eng.close()
