@@ -110,6 +110,7 @@ extern "C" {
 typedef struct ig_engine ig_engine;
 typedef int64_t ig_fields_id;
 typedef int64_t ig_ensemble_id;
+typedef int64_t ig_bases_id;
 
 /* Field pack.  Replaces the arrays the reference passes to every kernel
  * call: r-z EzA/EuA/EzK/EuK/EzG/EuG + mirrored ele
@@ -245,6 +246,26 @@ typedef struct {
   double *records;       /* optional (n, rec_rows, rec_cols) */
 } ig_results;
 
+/* Per-electrode potential bases kept on the device for field composition.
+ * Route RZ: the half grid (n0 = nz, n1 = nr, n2 = 1) of electrode
+ * potentials solved at base_volts (the reference's cache convention, fa =
+ * phi / base_volts, build_rz.py:377-386, 435-534); coord0 = Z_mm (n0) and
+ * coord1 = the mirror-unfolded U_mm (2 n1 - 1), the coordinate arrays
+ * np.gradient receives (ionbench.py:64-72).  Route XYZ: (nx, ny, nz) unit
+ * potentials with uniform spacing h_mm (tracer3d.py:93-117).  metal: the
+ * node metal mask (RZ half grid), for the electrode-aware one-sided
+ * differences. */
+typedef struct {
+  int32_t route;       /* IG_ROUTE_* */
+  int64_t n0, n1, n2;
+  int32_t n_bases;
+  const double *phi;   /* (n_bases, n0, n1, n2) */
+  const uint8_t *metal;/* (n0, n1, n2), nonzero = metal */
+  double base_volts;   /* RZ 1e4, XYZ 1 */
+  const double *coord0, *coord1; /* RZ gradient coordinates (see above) */
+  double h_mm;         /* XYZ node spacing */
+} ig_bases_desc;
+
 /* --- engine ------------------------------------------------------------ */
 const char *ig_last_error(void);
 int ig_abi_version(void);
@@ -262,6 +283,25 @@ int ig_sync(ig_engine *h);
 int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out);
 int ig_fields_release(ig_engine *h, ig_fields_id id);
 int ig_fields_bytes(ig_engine *h, ig_fields_id id, int64_t *out);
+
+/* --- field composition on the device ------------------------------------
+ * Upload per-electrode bases once; then compose any set of a resident
+ * field pack as phi = sum over terms t, in order, of w_t * (phi_{b_t} /
+ * base_volts) and E = -grad(phi) with np.gradient's coefficients and the
+ * electrode-aware one-sided difference into vacuum at metal nodes, V/m on
+ * the mirror-unfolded r-z grid or V/mm in 3-D, stored in the pack's
+ * precision and per-set scale.  Replaces the host composition and
+ * gradient pass of a re-tune: build_rz.py:428-534 with
+ * ionbench.build_field_aware (ionbench.py:44-96), tracer3d.py:54-117
+ * (build_field_aware_3d, _fa3d_correct). */
+int ig_bases_upload(ig_engine *h, const ig_bases_desc *d, ig_bases_id *out);
+int ig_bases_release(ig_engine *h, ig_bases_id id);
+int ig_fields_compose(ig_engine *h, ig_bases_id b, ig_fields_id f, int32_t set,
+                      int32_t n_terms, const int32_t *base_index, const double *weight);
+/* Read one component grid of one set of a resident pack back to the host
+ * as doubles (stored value x the set's storage scale): (n0, n1, n2).
+ * No counterpart in the reference (inspection of device packs). */
+int ig_fields_read_set(ig_engine *h, ig_fields_id f, int32_t set, int32_t comp, double *out);
 
 /* --- device-resident ensembles ------------------------------------------
  * Ion state kept in HBM across calls so that long flights, throughput

@@ -41,6 +41,13 @@ class FieldsDesc(C.Structure):
                 ("n_tab", C.c_int64)]
 
 
+class BasesDesc(C.Structure):
+    _fields_ = [("route", C.c_int32), ("n0", C.c_int64), ("n1", C.c_int64),
+                ("n2", C.c_int64), ("n_bases", C.c_int32), ("phi", _dp),
+                ("metal", _u8p), ("base_volts", C.c_double),
+                ("coord0", _dp), ("coord1", _dp), ("h_mm", C.c_double)]
+
+
 class FlightParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("t_max", C.c_double),
                 ("max_steps", C.c_int64), ("rec_every", C.c_int32),
@@ -123,6 +130,13 @@ SIGNATURES = {
                                C.POINTER(Ions), C.POINTER(Results)]),
     "ig_fly_graph": (C.c_int, [_VP, C.c_int64, C.POINTER(FlightParams),
                                C.c_int64, C.c_int32]),
+    "ig_bases_upload": (C.c_int, [_VP, C.POINTER(BasesDesc),
+                                  C.POINTER(C.c_int64)]),
+    "ig_bases_release": (C.c_int, [_VP, C.c_int64]),
+    "ig_fields_compose": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int32,
+                                    C.c_int32, _i32p, _dp]),
+    "ig_fields_read_set": (C.c_int, [_VP, C.c_int64, C.c_int32, C.c_int32,
+                                     _dp]),
     "ig_last_fly_ms": (C.c_int, [_VP, C.POINTER(C.c_double)]),
 }
 

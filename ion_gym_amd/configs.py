@@ -136,6 +136,32 @@ def funnel_rz(n_ions=1_000_000, seed=3, coarse=1, dt_ns=0.25):
                 dt=dt_ns * 1e-3, t_max=2000.0, gas=gas, charge=q)
 
 
+def funnel_electrodes(coarse=2):
+    """The funnel's electrodes as the reference's per-electrode cache holds
+    them (build_rz.py:377-386): each ring's potential solved at 1e4 V with
+    the others grounded, the DC ladder (100 V down at 1 V/mm), and two RF
+    drive groups (1 MHz, 100 V, 0 and 180 degrees) on alternate rings.
+    Returns (labels, h, bases {idx: phi}, dc, members, groups) for
+    fields_rz.compose_rz and the device composition."""
+    h = 0.1 * coarse
+    nz, nr = 1200 // coarse + 1, 200 // coarse + 1
+    elec = []
+    for i in range(100):
+        z0 = 10.0 + i * 1.0
+        rin = 12.5 + (1.0 - 12.5) * i / 99.0
+        elec.append((i + 1, z0, z0 + 0.5, rin, rin + 3.0))
+    lab = _grid_labels(nz, nr, h, elec)
+    keys = [e[0] for e in elec]
+    sols = solve_rz(lab, [{k: 1e4} for k in keys], h)
+    bases = dict(zip(keys, sols))
+    dc = {i + 1: 100.0 - (elec[i][1] - 10.0) * 1.0 for i in range(100)}
+    members = {i + 1: ("rfA",) if i % 2 == 0 else ("rfB",)
+               for i in range(100)}
+    groups = [DriveGroup("rfA", "sin", 1.0e6, 100.0, 0.0),
+              DriveGroup("rfB", "sin", 1.0e6, 100.0, 180.0)]
+    return lab, h, bases, dc, members, groups
+
+
 def drives_for(model):
     """DriveGroup view of a composed model's channels (for reports)."""
     out = []

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "ig_common.h"
+#include "ig_compose_kernel.h"
 
 using namespace ig;
 
@@ -63,8 +64,14 @@ int fail(int code, const std::string &msg) {
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
-  ~DevBuf() {
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
   }
 };
 
@@ -85,6 +92,16 @@ struct Fields {
   DevBuf grid, metal, label, tab_t, tab_v, waves;
   Waves waves_host{};
   int64_t bytes = 0;
+};
+
+// per-electrode bases for composition on the device
+struct Bases {
+  int32_t route = IG_ROUTE_RZ;
+  int64_t n0 = 0, n1 = 0, n2 = 0;
+  int32_t n_bases = 0;
+  double base_volts = 1.0;
+  DevBuf phi, metal, abc0, abc1, work, terms_i, terms_w;
+  GradAxis g0{}, g1{};
 };
 
 // a captured loop of flight windows and the kernel arguments it replays
@@ -126,6 +143,7 @@ struct ig_engine {
   int64_t next_id = 1;
   std::map<int64_t, std::unique_ptr<Fields>> fields;
   std::map<int64_t, std::unique_ptr<Ensemble>> ens;
+  std::map<int64_t, std::unique_ptr<Bases>> bases;
   int block = 256;
   std::vector<GraphEntry> graphs; // captured window loops (ig_fly_graph)
 };
@@ -204,6 +222,7 @@ int ig_engine_destroy(ig_engine *h) {
   drop_graphs(h);
   h->fields.clear();
   h->ens.clear();
+  h->bases.clear();
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -891,6 +910,185 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
   HIPX(hipGraphLaunch(exec, h->stream));
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
+  return 0;
+}
+
+// np.gradient's spacing for one axis from its coordinate array (numpy's
+// own arithmetic: steps by np.diff, the scalar case when every step is
+// equal, else the three-point coefficients a, b, c)
+static void numpy_axis(const double *x, int64_t n, std::vector<double> &abc, GradAxis &g) {
+  std::vector<double> d((size_t)(n - 1));
+  for (int64_t i = 0; i + 1 < n; ++i) d[i] = x[i + 1] - x[i];
+  bool uni = true;
+  for (double v : d) uni = uni && v == d[0];
+  g.n = n;
+  g.uniform = uni ? 1 : 0;
+  g.dx = d[0];
+  g.dx_lo = d[0];
+  g.dx_hi = d[n - 2];
+  g.h_one = x[1] - x[0];
+  abc.clear();
+  if (!uni) {
+    abc.resize((size_t)(3 * (n - 2)));
+    for (int64_t i = 0; i + 2 < n; ++i) {
+      const double dx1 = d[i], dx2 = d[i + 1];
+      abc[3 * i] = -(dx2) / (dx1 * (dx1 + dx2));
+      abc[3 * i + 1] = (dx2 - dx1) / (dx1 * dx2);
+      abc[3 * i + 2] = dx1 / (dx2 * (dx1 + dx2));
+    }
+  }
+}
+
+int ig_bases_upload(ig_engine *h, const ig_bases_desc *d, ig_bases_id *out) {
+  if (!h || !d || !out || !d->phi || !d->metal)
+    return fail(IG_E_ARG, "ig_bases_upload: null argument");
+  const bool rz = d->route == IG_ROUTE_RZ;
+  if (!rz && d->route != IG_ROUTE_XYZ) return fail(IG_E_ARG, "ig_bases_upload: unknown route");
+  if (d->n0 < 2 || d->n1 < 2 || (rz ? d->n2 != 1 : d->n2 < 2) || d->n_bases < 1)
+    return fail(IG_E_ARG, "ig_bases_upload: bad grid or base count");
+  if (!(d->base_volts > 0.0)) return fail(IG_E_ARG, "ig_bases_upload: base_volts must be > 0");
+  if (rz && (!d->coord0 || !d->coord1))
+    return fail(IG_E_ARG, "ig_bases_upload: r-z bases need coord0 and coord1");
+  if (!rz && !(d->h_mm > 0.0)) return fail(IG_E_ARG, "ig_bases_upload: h_mm must be > 0");
+  if (int rc = set_dev(h)) return rc;
+  auto B = std::make_unique<Bases>();
+  B->route = d->route;
+  B->n0 = d->n0;
+  B->n1 = d->n1;
+  B->n2 = d->n2;
+  B->n_bases = d->n_bases;
+  B->base_volts = d->base_volts;
+  const int64_t nodes = d->n0 * d->n1 * d->n2;
+  int rc = upload(h, B->phi, d->phi, (size_t)(nodes * d->n_bases) * sizeof(double));
+  if (rc) return rc;
+  std::vector<uint8_t> m((size_t)nodes);
+  for (int64_t n = 0; n < nodes; ++n) m[n] = d->metal[n] ? 1 : 0;
+  if ((rc = upload(h, B->metal, m.data(), m.size()))) return rc;
+  if ((rc = dev_alloc(B->work, (size_t)nodes * sizeof(double)))) return rc;
+  if (rz) {
+    std::vector<double> a0, a1;
+    numpy_axis(d->coord0, d->n0, a0, B->g0);
+    numpy_axis(d->coord1, 2 * d->n1 - 1, a1, B->g1);
+    if (!a0.empty() && (rc = upload(h, B->abc0, a0.data(), a0.size() * sizeof(double)))) return rc;
+    if (!a1.empty() && (rc = upload(h, B->abc1, a1.data(), a1.size() * sizeof(double)))) return rc;
+    B->g0.abc = (const double *)B->abc0.p;
+    B->g1.abc = (const double *)B->abc1.p;
+  } else {
+    // np.gradient(phi, h_mm): one scalar spacing on every axis
+    B->g0 = GradAxis{0, 1, d->h_mm, nullptr, d->h_mm, d->h_mm, d->h_mm};
+  }
+  HIPX(hipStreamSynchronize(h->stream));
+  const int64_t id = h->next_id++;
+  h->bases[id] = std::move(B);
+  *out = id;
+  return 0;
+}
+
+int ig_bases_release(ig_engine *h, ig_bases_id id) {
+  if (!h) return fail(IG_E_ARG, "ig_bases_release: null engine");
+  auto it = h->bases.find(id);
+  if (it == h->bases.end()) return fail(IG_E_STATE, "ig_bases_release: unknown bases id");
+  if (int rc = set_dev(h)) return rc;
+  HIPX(hipStreamSynchronize(h->stream));
+  h->bases.erase(it);
+  return 0;
+}
+
+int ig_fields_compose(ig_engine *h, ig_bases_id bid, ig_fields_id fid, int32_t set,
+                      int32_t n_terms, const int32_t *base_index, const double *weight) {
+  if (!h || (n_terms > 0 && (!base_index || !weight)))
+    return fail(IG_E_ARG, "ig_fields_compose: null argument");
+  auto bit = h->bases.find(bid);
+  if (bit == h->bases.end()) return fail(IG_E_STATE, "ig_fields_compose: unknown bases id");
+  auto fit = h->fields.find(fid);
+  if (fit == h->fields.end()) return fail(IG_E_STATE, "ig_fields_compose: unknown field id");
+  Bases &B = *bit->second;
+  Fields &F = *fit->second;
+  const FieldsK &k = F.k;
+  const bool rz = B.route == IG_ROUTE_RZ;
+  if (k.route != B.route) return fail(IG_E_ARG, "ig_fields_compose: route differs from the pack's");
+  if (rz ? (k.n0 != B.n0 || k.n1 != 2 * B.n1 - 1)
+         : (k.n0 != B.n0 || k.n1 != B.n1 || k.n2 != B.n2))
+    return fail(IG_E_ARG, "ig_fields_compose: bases grid does not match the pack");
+  if (set < 0 || set >= k.n_sets) return fail(IG_E_ARG, "ig_fields_compose: set out of range");
+  if (n_terms < 0 || n_terms > (1 << 20)) return fail(IG_E_ARG, "ig_fields_compose: bad term count");
+  for (int32_t t = 0; t < n_terms; ++t)
+    if (base_index[t] < 0 || base_index[t] >= B.n_bases)
+      return fail(IG_E_ARG, "ig_fields_compose: base index out of range");
+  if (int rc = set_dev(h)) return rc;
+  const size_t nt = (size_t)std::max<int32_t>(n_terms, 1);
+  if (B.terms_i.bytes < nt * sizeof(int32_t)) {
+    HIPX(hipStreamSynchronize(h->stream));
+    B.terms_i.reset();
+    B.terms_w.reset();
+    if (int rc = dev_alloc(B.terms_i, nt * sizeof(int32_t))) return rc;
+    if (int rc = dev_alloc(B.terms_w, nt * sizeof(double))) return rc;
+  }
+  if (n_terms > 0) {
+    HIPX(hipMemcpyAsync(B.terms_i.p, base_index, n_terms * sizeof(int32_t),
+                        hipMemcpyHostToDevice, h->stream));
+    HIPX(hipMemcpyAsync(B.terms_w.p, weight, n_terms * sizeof(double), hipMemcpyHostToDevice,
+                        h->stream));
+  }
+  const int64_t nodes = B.n0 * B.n1 * B.n2;
+  hipLaunchKernelGGL(compose_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0,
+                     h->stream, (const double *)B.phi.p, nodes, n_terms,
+                     (const int32_t *)B.terms_i.p, (const double *)B.terms_w.p, B.base_volts,
+                     (double *)B.work.p);
+  const double inv = 1.0 / k.scale[set];
+  const bool f64 = k.prec == IG_PREC_F64;
+  if (rz) {
+    const int64_t out_nodes = B.n0 * (2 * B.n1 - 1);
+    dim3 g((unsigned)((out_nodes + 255) / 256));
+    if (f64)
+      hipLaunchKernelGGL(grad_rz_kernel<double>, g, dim3(256), 0, h->stream,
+                         (const double *)B.work.p, (const uint8_t *)B.metal.p, B.n0, B.n1, B.g0,
+                         B.g1, F.grid.p, (int64_t)k.stride, set, inv);
+    else
+      hipLaunchKernelGGL(grad_rz_kernel<float>, g, dim3(256), 0, h->stream,
+                         (const double *)B.work.p, (const uint8_t *)B.metal.p, B.n0, B.n1, B.g0,
+                         B.g1, F.grid.p, (int64_t)k.stride, set, inv);
+  } else {
+    dim3 g((unsigned)((nodes + 255) / 256));
+    if (f64)
+      hipLaunchKernelGGL(grad_3d_kernel<double>, g, dim3(256), 0, h->stream,
+                         (const double *)B.work.p, (const uint8_t *)B.metal.p, B.n0, B.n1, B.n2,
+                         B.g0, F.grid.p, (int64_t)k.stride, set, inv);
+    else
+      hipLaunchKernelGGL(grad_3d_kernel<float>, g, dim3(256), 0, h->stream,
+                         (const double *)B.work.p, (const uint8_t *)B.metal.p, B.n0, B.n1, B.n2,
+                         B.g0, F.grid.p, (int64_t)k.stride, set, inv);
+  }
+  HIPX(hipGetLastError());
+  // the host term arrays are the caller's: finish the copies before return
+  HIPX(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int ig_fields_read_set(ig_engine *h, ig_fields_id fid, int32_t set, int32_t comp, double *out) {
+  if (!h || !out) return fail(IG_E_ARG, "ig_fields_read_set: null argument");
+  auto fit = h->fields.find(fid);
+  if (fit == h->fields.end()) return fail(IG_E_STATE, "ig_fields_read_set: unknown field id");
+  Fields &F = *fit->second;
+  const FieldsK &k = F.k;
+  if (set < 0 || set >= k.n_sets || comp < 0 || comp >= k.n_comp)
+    return fail(IG_E_ARG, "ig_fields_read_set: set or component out of range");
+  if (int rc = set_dev(h)) return rc;
+  const int64_t nodes = k.n0 * k.n1 * k.n2;
+  DevBuf tmp;
+  if (int rc = dev_alloc(tmp, (size_t)nodes * sizeof(double))) return rc;
+  const int32_t q = set * k.n_comp + comp;
+  dim3 g((unsigned)((nodes + 255) / 256));
+  if (k.prec == IG_PREC_F64)
+    hipLaunchKernelGGL(read_comp_kernel<double>, g, dim3(256), 0, h->stream, F.grid.p, nodes,
+                       (int64_t)k.stride, q, k.scale[set], (double *)tmp.p);
+  else
+    hipLaunchKernelGGL(read_comp_kernel<float>, g, dim3(256), 0, h->stream, F.grid.p, nodes,
+                       (int64_t)k.stride, q, k.scale[set], (double *)tmp.p);
+  HIPX(hipGetLastError());
+  HIPX(hipMemcpyAsync(out, tmp.p, (size_t)nodes * sizeof(double), hipMemcpyDeviceToHost,
+                      h->stream));
+  HIPX(hipStreamSynchronize(h->stream));
   return 0;
 }
 

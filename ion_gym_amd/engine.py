@@ -44,6 +44,20 @@ class _Keep(list):
         return a
 
 
+class Bases:
+    """Per-electrode potential bases resident on the device."""
+
+    def __init__(self, engine, bid):
+        self.engine = engine
+        self.id = bid
+
+    def release(self):
+        if self.id is not None:
+            L.check(L.lib().ig_bases_release(self.engine.h, self.id),
+                    "ig_bases_release")
+            self.id = None
+
+
 class FieldPack:
     """A field pack resident on the device."""
 
@@ -390,6 +404,69 @@ class Engine:
         p = flight.struct(keep)
         L.check(L.lib().ig_fly(self.h, fields.id, C.byref(p), ens.id),
                 "ig_fly")
+
+    # ----------------------------------------------- field composition
+    def upload_bases_rz(self, phi, ele, Z_mm, U_mm, base_volts=1e4):
+        """Per-electrode r-z potentials for composition on the device:
+        ``phi`` (n_bases, nz, nr) at ``base_volts`` each (the reference's
+        cache convention), ``ele`` (nz, nr) labels or mask (nonzero =
+        metal), ``Z_mm``/``U_mm`` the half-grid coordinates
+        (ionbench.build_field_aware's inputs)."""
+        keep = _Keep()
+        phi = keep(np.asarray(phi, np.float64))
+        nb, nz, nr = phi.shape
+        Ue = np.concatenate([-np.asarray(U_mm)[:0:-1], np.asarray(U_mm)])
+        d = L.BasesDesc()
+        d.route = L.ROUTE_RZ
+        d.n0, d.n1, d.n2 = nz, nr, 1
+        d.n_bases = nb
+        d.phi = L.ptr(phi)
+        d.metal = L.ptr(keep(np.asarray(ele) != 0, np.uint8), L._u8p)
+        d.base_volts = float(base_volts)
+        d.coord0 = L.ptr(keep(np.asarray(Z_mm, np.float64)))
+        d.coord1 = L.ptr(keep(np.asarray(Ue, np.float64)))
+        bid = C.c_int64()
+        L.check(L.lib().ig_bases_upload(self.h, C.byref(d), C.byref(bid)),
+                "ig_bases_upload")
+        return Bases(self, bid.value)
+
+    def upload_bases_3d(self, phi, ele, h_mm):
+        """Per-electrode 3-D unit potentials (n_bases, nx, ny, nz) with the
+        node metal mask and spacing (build_field_aware_3d's inputs)."""
+        keep = _Keep()
+        phi = keep(np.asarray(phi, np.float64))
+        nb, nx, ny, nz = phi.shape
+        d = L.BasesDesc()
+        d.route = L.ROUTE_XYZ
+        d.n0, d.n1, d.n2 = nx, ny, nz
+        d.n_bases = nb
+        d.phi = L.ptr(phi)
+        d.metal = L.ptr(keep(np.asarray(ele) != 0, np.uint8), L._u8p)
+        d.base_volts = 1.0
+        d.h_mm = float(h_mm)
+        bid = C.c_int64()
+        L.check(L.lib().ig_bases_upload(self.h, C.byref(d), C.byref(bid)),
+                "ig_bases_upload")
+        return Bases(self, bid.value)
+
+    def compose(self, bases, fields, set_index, terms):
+        """Set ``set_index`` of ``fields`` := -grad(sum of w * phi_b /
+        base_volts over ``terms`` [(b, w), ...] in order), on the device."""
+        keep = _Keep()
+        bi = keep(np.array([t[0] for t in terms], np.int32), np.int32)
+        w = keep(np.array([t[1] for t in terms], np.float64))
+        L.check(L.lib().ig_fields_compose(
+            self.h, bases.id, fields.id, int(set_index), len(terms),
+            L.ptr(bi, L._i32p), L.ptr(w)), "ig_fields_compose")
+
+    def read_set(self, fields, set_index, comp):
+        """One component grid of one set of a resident pack (doubles)."""
+        shape = tuple(int(v) for v in fields.shape)
+        out = np.empty(shape, np.float64)
+        L.check(L.lib().ig_fields_read_set(self.h, fields.id, int(set_index),
+                                           int(comp), L.ptr(out)),
+                "ig_fields_read_set")
+        return out
 
     def fly_graph(self, fields, flight, ens, n_windows):
         """Advance ``ens`` by ``n_windows`` windows of ``flight.max_steps``

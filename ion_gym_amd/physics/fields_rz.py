@@ -110,22 +110,22 @@ class RZModel:
                     ch_duty=self.ch_duty)
 
 
-def compose_rz(bases, ele, h, dc, members, groups):
-    """Compose an r-z model from per-electrode unit potentials.
-
-    bases[idx] is electrode idx's solved potential at 1e4 V (idx 1-based,
-    the reference's cache convention, build_rz.py:377-386); dc[idx] its DC
-    volts; members[idx] the drive-group names it belongs to; groups the
-    DriveGroup list.  Follows build_rz.py:435-534 term by term."""
-    nz, nr = ele.shape
-    A = np.zeros((nz, nr))
+def compose_terms_rz(keys, dc, members, groups):
+    """The composition of an r-z model as ordered term lists, following
+    build_rz.py:435-534 term by term: for each electrode idx in ``keys``
+    order, fa = base_idx / 1e4 enters the static set as dc * fa, a gated
+    drive's (amp * v0 + offset) * fa and a drive offset * fa, the gate set
+    as amp * (v1 - v0) * fa, and each drive group's channel as amp * fa.
+    Returns (terms_A, [terms per channel], terms_G, channel drives in
+    order, tau_gate), each term (position of idx in ``keys``, weight);
+    summing w * fa over a list in order reproduces the reference's
+    accumulation exactly (the host path below and ig_fields_compose)."""
     gmap = {g.name: g for g in groups}
-    chan_B, chan_obj, chan_order = {}, {}, []
-    G = np.zeros((nz, nr))
+    tA, tG = [], []
+    chan_t, chan_obj, chan_order = {}, {}, []
     tau_gate = -1.0
-    for idx in sorted(bases):
-        fa = bases[idx] / 1e4
-        A = A + dc.get(idx, 0.0) * fa
+    for pos, idx in enumerate(keys):
+        tA.append((pos, dc.get(idx, 0.0)))
         for gname in members.get(idx, ()):
             drv = gmap[gname]
             if drv.waveform == "table":
@@ -135,29 +135,56 @@ def compose_rz(bases, ele, h, dc, members, groups):
                 if tau_gate >= 0.0 and abs(tt[1] - tau_gate) > 1e-9:
                     raise ValueError("r-z gate: two distinct gate times")
                 tau_gate = tt[1]
-                A = A + (drv.amplitude_v * tv[0] + drv.offset_v) * fa
-                G = G + drv.amplitude_v * (tv[1] - tv[0]) * fa
+                tA.append((pos, drv.amplitude_v * tv[0] + drv.offset_v))
+                tG.append((pos, drv.amplitude_v * (tv[1] - tv[0])))
                 continue
             if drv.offset_v:
-                A = A + drv.offset_v * fa
+                tA.append((pos, drv.offset_v))
             if drv.amplitude_v == 0.0:
                 continue
-            if drv.name not in chan_B:
-                chan_B[drv.name] = np.zeros((nz, nr))
+            if drv.name not in chan_t:
+                chan_t[drv.name] = []
                 chan_obj[drv.name] = drv
                 chan_order.append(drv.name)
-            chan_B[drv.name] += drv.amplitude_v * fa
+            chan_t[drv.name].append((pos, drv.amplitude_v))
+    return (tA, [chan_t[n] for n in chan_order], tG,
+            [chan_obj[n] for n in chan_order], tau_gate)
+
+
+def apply_terms(fas, terms, shape):
+    """sum of w * fas[pos] over ``terms`` in order, from zeros."""
+    acc = np.zeros(shape)
+    for pos, w in terms:
+        acc = acc + w * fas[pos]
+    return acc
+
+
+def compose_rz(bases, ele, h, dc, members, groups):
+    """Compose an r-z model from per-electrode unit potentials.
+
+    bases[idx] is electrode idx's solved potential at 1e4 V (idx 1-based,
+    the reference's cache convention, build_rz.py:377-386); dc[idx] its DC
+    volts; members[idx] the drive-group names it belongs to; groups the
+    DriveGroup list.  Follows build_rz.py:435-534 term by term
+    (compose_terms_rz)."""
+    nz, nr = ele.shape
+    keys = sorted(bases)
+    fas = [bases[idx] / 1e4 for idx in keys]
+    tA, tK, tG, drives, tau_gate = compose_terms_rz(keys, dc, members,
+                                                    groups)
+    A = apply_terms(fas, tA, (nz, nr))
+    G = apply_terms(fas, tG, (nz, nr))
+    chan_B = [apply_terms(fas, t, (nz, nr)) for t in tK]
     Z_mm = np.arange(nz) * h
     U_mm = np.arange(nr) * h
     _, Ue, EzA, EuA = build_field_aware(Z_mm, U_mm, A, ele)
     EzK_l, EuK_l = [], []
-    for n in chan_order:
-        _, _, a, b = build_field_aware(Z_mm, U_mm, chan_B[n], ele)
+    for B in chan_B:
+        _, _, a, b = build_field_aware(Z_mm, U_mm, B, ele)
         EzK_l.append(a)
         EuK_l.append(b)
     EzK = np.stack(EzK_l) if EzK_l else np.zeros((0,) + EzA.shape)
     EuK = np.stack(EuK_l) if EuK_l else np.zeros((0,) + EuA.shape)
-    drives = [chan_obj[n] for n in chan_order]
     ch_kind = np.array([K_SQUARE if d.waveform == "square" else K_SIN
                         for d in drives], np.int64)
     ch_om = np.array([d.frequency_hz * 1e-6 * 2 * math.pi for d in drives],
