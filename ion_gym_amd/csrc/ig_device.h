@@ -31,10 +31,11 @@ enum : int { kStep = 0, kCollide = 1, kPost = 2, kDone = 3 };
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t lo0 = 0xD2511F53u * c[0];
-    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-    uint32_t lo1 = 0xCD9E8D57u * c[2];
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    // full 64-bit products: one multiply gives both halves
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     uint32_t n0 = hi1 ^ c[1] ^ k0;
     uint32_t n2 = hi0 ^ c[3] ^ k1;
     c[0] = n0;
@@ -368,6 +369,51 @@ __device__ __forceinline__ double wave_eval_fast(const Waves *__restrict__ W, co
   return wave_eval_slow<AFFINE>(W, f, k, t);
 }
 
+// ---- FP32-basis random draws
+// The FP32-basis mode draws whole Philox blocks: its counter n counts
+// 32-bit words and every consumer takes the next block(s), n advancing by
+// 4 per block (the FP64 modes' double draws take a word pair each).
+// Uniforms take the top 24 bits of a word, the FP32 significand; the low 8
+// bits of three words make one more 24-bit uniform (the bits of a Philox
+// word are independent).
+__device__ __forceinline__ void rng_block(Rng &r, uint32_t *w) {
+  const uint64_t blk = (r.n + 3) >> 2;
+  w[0] = (uint32_t)blk;
+  w[1] = (uint32_t)(blk >> 32);
+  w[2] = 0u;
+  w[3] = 0u;
+  philox4x32_10(w, r.k0, r.k1);
+  r.n = (blk + 1) << 2;
+}
+
+// uniform in [0, 1) and in (0, 1] from a word's top 24 bits
+__device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * 5.9604644775390625e-8f; }
+__device__ __forceinline__ float u24p(uint32_t w) {
+  return (float)((w >> 8) + 1u) * 5.9604644775390625e-8f;
+}
+// uniform strictly inside (0, 1): the midpoints (k + 1/2) 2^-24
+__device__ __forceinline__ float u24h(uint32_t w) {
+  return ((float)(w >> 8) + 0.5f) * 5.9604644775390625e-8f;
+}
+// uniform in [0, 1) from the low bytes of three words
+__device__ __forceinline__ float u24_low(uint32_t a, uint32_t b, uint32_t c) {
+  return (float)((a & 0xFFu) | ((b & 0xFFu) << 8) | ((c & 0xFFu) << 16)) *
+         5.9604644775390625e-8f;
+}
+
+// sin and cos of r in [-pi, pi]
+__device__ __forceinline__ void sincos_pi_range(float r, float &sn, float &cs) {
+  sn = sin_pi_range(r);
+  float rc = r + 1.57079632679489662f;
+  if (rc > 3.14159265358979324f) rc = rc - 6.28318530717958648f;
+  cs = sin_pi_range(rc);
+}
+
+// (sin, cos) of the uniformly random angle 2 pi U - pi
+__device__ __forceinline__ void azimuth(float u, float &sn, float &cs) {
+  sincos_pi_range(__builtin_fmaf(6.28318530717958648f, u, -3.14159265358979324f), sn, cs);
+}
+
 // ---- hard sphere, FP32-basis mode, flattened into the lane state machine
 //
 // The FP32 mode samples the reference's collision model (collision3d.py:
@@ -381,6 +427,8 @@ __device__ __forceinline__ double wave_eval_fast(const Waves *__restrict__ W, co
 //   zero or below, after which a fresh E is drawn.  Given survival to step
 //   n, P(due at n) = 1 - exp(-h_n) by the memorylessness of Exp(1): the
 //   same law of collision steps, with no exponential and no draw per step.
+//   c_rel is the exact Maxwell mean relative speed, evaluated in FP32 by a
+//   fitted polynomial (relative error below 5e-7, c_rel_ratio_f).
 // * Partner velocity.  The reference draws g ~ N(0, sig1d^2 I) and accepts
 //   with |w - g| / (sp + 3 sqrt(3) sig1d): the accepted g has density
 //   proportional to |w - g| N(g).  Here the proposal is the mixture
@@ -394,8 +442,9 @@ __device__ __forceinline__ double wave_eval_fast(const Waves *__restrict__ W, co
 //   1.0-1.45 tries on average.
 // * Scattering: the reference's impact angle asin(sqrt(U)), uniform
 //   azimuth and line-of-centres attenuation, in FP32 unit vectors.
-// A lane performs one partner try per pass of its kernel loop while the
-// other lanes keep stepping.
+// Logarithms, reciprocals and square roots use the hardware's FP32
+// instructions; the angles use sin_pi_range.  A lane performs one partner
+// try per pass of its kernel loop while the other lanes keep stepping.
 struct HsTry {
   float wx, wy, wz; // ion velocity in the gas frame
   float sp;         // max(|w|, 1e-7)
@@ -408,16 +457,30 @@ __device__ __forceinline__ float hs_rate_const(const FlightK &p, double dt, doub
   return (float)(dt * dt_to_us * p.P_pa * sigma / (1000.0 * IG_KB * p.T_k));
 }
 
-// an Exp(1) draw from one uniform
-__device__ __forceinline__ float rng_exp1(Rng &rng) {
-  return -logf((float)(1.0 - rng_uniform(rng))); // 1 - U in (0, 1]
-}
+// an Exp(1) variate from a word
+__device__ __forceinline__ float exp1_of(uint32_t w) { return -__logf(u24p(w)); }
 
-// c_rel(sp) / c_bar as a function of s = sp / c* (collision3d.py:265-272)
+// c_rel(sp) / c_bar as a function of s = sp / c* (collision3d.py:265-272):
+// for s < 4 the degree-12 polynomial in x = s^2 / 8 - 1 fitted by
+// ion_gym_amd/csrc/fit_c_rel.py, above it (sqrt(pi)/2)(s + 1/(2 s)), whose
+// relative error there is below 3e-11
 __device__ __forceinline__ float c_rel_ratio_f(float s) {
-  if (s < 1e-12f) return 1.0f;
-  const float sqrt_pi = 1.7724538509055159f;
-  return (s + 0.5f / s) * 0.5f * sqrt_pi * erff(s) + 0.5f * __expf(-s * s);
+  if (s >= 4.0f)
+    return 0.886226925452758f * (s + 0.5f * __builtin_amdgcn_rcpf(s));
+  const float x = __builtin_fmaf(s * s, 0.125f, -1.0f);
+  float q = -7.843976244e-04f;
+  q = __builtin_fmaf(q, x, 1.642305600e-03f);
+  q = __builtin_fmaf(q, x, -1.029022523e-03f);
+  q = __builtin_fmaf(q, x, 1.949520542e-03f);
+  q = __builtin_fmaf(q, x, -6.028446949e-03f);
+  q = __builtin_fmaf(q, x, 1.072184697e-02f);
+  q = __builtin_fmaf(q, x, -1.747645726e-02f);
+  q = __builtin_fmaf(q, x, 3.065115064e-02f);
+  q = __builtin_fmaf(q, x, -5.550525405e-02f);
+  q = __builtin_fmaf(q, x, 1.078872557e-01f);
+  q = __builtin_fmaf(q, x, -2.546248822e-01f);
+  q = __builtin_fmaf(q, x, 1.174990659e+00f);
+  return __builtin_fmaf(q, x, 2.663291561e+00f);
 }
 
 // One step of the hazard clock.  hz is the remaining Exp(1) hazard (NaN:
@@ -426,10 +489,14 @@ __device__ __forceinline__ float c_rel_ratio_f(float s) {
 __device__ __forceinline__ bool hs_hazard_step(Rng &rng, float &hz, double vx, double vy,
                                                double vz, const FlightK &p, float kconst,
                                                float inv_c_star, HsTry &st) {
-  if (!(hz >= 0.0f)) hz = rng_exp1(rng);
+  if (!(hz >= 0.0f)) {
+    uint32_t w[4];
+    rng_block(rng, w);
+    hz = exp1_of(w[0]);
+  }
   const float wx = (float)(vx - p.vg[0]), wy = (float)(vy - p.vg[1]),
               wz = (float)(vz - p.vg[2]);
-  float sp = sqrtf(wx * wx + wy * wy + wz * wz);
+  float sp = __builtin_sqrtf(wx * wx + wy * wy + wz * wz);
   if (sp < 1e-7f) sp = 1e-7f;
   hz -= kconst * (float)p.c_bar * c_rel_ratio_f(sp * inv_c_star);
   if (hz > 0.0f) return false;
@@ -441,60 +508,58 @@ __device__ __forceinline__ bool hs_hazard_step(Rng &rng, float &hz, double vx, d
   return true;
 }
 
-// Box-Muller normal with the second value of each pair cached
-__device__ __forceinline__ float rng_normal_bm(Rng &r) {
-  if (r.has_gauss) {
-    r.has_gauss = 0;
-    return (float)r.gauss;
-  }
-  const float u1 = (float)(1.0 - rng_uniform(r)); // (0, 1]
-  const float u2 = (float)rng_uniform(r);
-  const float rad = sqrtf(-2.0f * logf(u1));
-  float sn, cs;
-  sincospif(2.0f * u2, &sn, &cs);
-  r.gauss = (double)(rad * sn);
-  r.has_gauss = 1;
-  return rad * cs;
-}
-
-// One partner try.  Returns 1 when the collision is complete (velocity
+// One partner try on two Philox blocks (8 words): w0 acceptance, w1-w4
+// the proposal (two Box-Muller pairs for the Maxwell branch; U1, U2,
+// cos theta and azimuth for the speed-weighted one), w5 the next hazard,
+// w6 and w7 the impact angle and azimuth, and the low bytes of w1-w3 the
+// mixture choice.  Returns 1 when the collision is complete (velocity
 // updated, a fresh hazard drawn), 0 to try again, -1 when
 // MAX_PARTNER_TRIES (collision3d.py:72) ran out.
 __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, double &vx,
                                            double &vy, double &vz, const FlightK &p,
                                            double m_ion) {
+  uint32_t w[8];
+  rng_block(rng, w);
+  rng_block(rng, w + 4);
   const float sig = (float)p.sig1d;
   const float gmean = sig * 1.5957691216057308f; // <|g|> = sig1d sqrt(8 / pi)
   float gx, gy, gz, gl;
-  if ((float)rng_uniform(rng) * (st.sp + gmean) < st.sp) {
-    gx = rng_normal_bm(rng) * sig;
-    gy = rng_normal_bm(rng) * sig;
-    gz = rng_normal_bm(rng) * sig;
-    gl = sqrtf(gx * gx + gy * gy + gz * gz);
+  if (u24_low(w[1], w[2], w[3]) * (st.sp + gmean) < st.sp) {
+    // Maxwell: three normals from two Box-Muller pairs
+    const float r1 = sig * __builtin_sqrtf(-2.0f * __logf(u24p(w[1])));
+    const float r2 = sig * __builtin_sqrtf(-2.0f * __logf(u24p(w[3])));
+    float s1, c1, s2, c2;
+    azimuth(u24(w[2]), s1, c1);
+    azimuth(u24(w[4]), s2, c2);
+    gx = r1 * c1;
+    gy = r1 * s1;
+    gz = r2 * c2;
+    gl = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
   } else {
-    const float u1 = (float)(1.0 - rng_uniform(rng)), u2 = (float)(1.0 - rng_uniform(rng));
-    gl = sig * sqrtf(-2.0f * (logf(u1) + logf(u2)));
-    const float c = (float)(2.0 * rng_uniform(rng) - 1.0);
+    // speed-weighted Maxwell: |g| = sig1d sqrt(chi^2_4), isotropic
+    gl = sig * __builtin_sqrtf(-2.0f * __logf(u24p(w[1]) * u24p(w[2])));
+    const float c = __builtin_fmaf(2.0f, u24(w[3]), -1.0f);
     float sn, cs;
-    sincospif((float)(2.0 * rng_uniform(rng)), &sn, &cs);
+    azimuth(u24(w[4]), sn, cs);
     const float one_m = 1.0f - c * c;
-    const float sr = gl * sqrtf(one_m > 0.0f ? one_m : 0.0f);
+    const float sr = gl * __builtin_sqrtf(one_m > 0.0f ? one_m : 0.0f);
     gx = sr * cs;
     gy = sr * sn;
     gz = gl * c;
   }
   const float rx = st.wx - gx, ry = st.wy - gy, rz = st.wz - gz;
-  const float rlen = sqrtf(rx * rx + ry * ry + rz * rz);
-  if (!((float)rng_uniform(rng) * (st.sp + gl) < rlen)) {
+  const float rlen = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
+  if (!(u24(w[0]) * (st.sp + gl) < rlen)) {
     return ++st.tries >= 10000 ? -1 : 0;
   }
-  hz = rng_exp1(rng);
+  hz = exp1_of(w[5]);
   if (rlen < 1e-12f) return 1;
-  const float au = (float)(0.999999999 * rng_uniform(rng));
-  const float sa = sqrtf(au), ca = sqrtf(1.0f - au);
+  // sin(asin(sqrt(U))) = sqrt(U), cos = sqrt(1 - U), U < 1
+  const float au = u24(w[6]);
+  const float sa = __builtin_sqrtf(au), ca = __builtin_sqrtf(1.0f - au);
   float sth, cth;
-  sincospif((float)(2.0 * rng_uniform(rng)), &sth, &cth);
-  const float inv = 1.0f / rlen;
+  azimuth(u24(w[7]), sth, cth);
+  const float inv = __builtin_amdgcn_rcpf(rlen);
   const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
   float px, py, pz;
   if (fabsf(ux) < 0.9f) {
@@ -506,7 +571,7 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
     py = 0.0f;
     pz = ux;
   }
-  const float pinv = 1.0f / sqrtf(px * px + py * py + pz * pz);
+  const float pinv = __builtin_amdgcn_rsqf(px * px + py * py + pz * pz);
   px *= pinv;
   py *= pinv;
   pz *= pinv;
@@ -523,12 +588,15 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
 }
 
 // the SDS jump with the table interpolation, the log-log mass-ratio blend
-// and the direction in FP32
+// and the direction in FP32, on one Philox block per step
 __device__ __forceinline__ void sds_jump_fast(Rng &rng, const double *__restrict__ st,
                                               double log_mr, double V, double mfp, double dt,
                                               double &jx, double &jy, double &jz) {
-  double u = rng_uniform(rng);
-  float x = (float)(u * (1002 - 2));
+  uint32_t wd[4];
+  rng_block(rng, wd);
+  // a quantile strictly inside (0, 1): the table's end quantiles are 0
+  // and its largest step count, and log10 of a 0 quantile is -inf
+  float x = u24h(wd[0]) * (float)(1002 - 2);
   int i = (int)x;
   if (i > 999) i = 999;
   float w = x - (float)i;
@@ -542,12 +610,11 @@ __device__ __forceinline__ void sds_jump_fast(Rng &rng, const double *__restrict
   float dsteps = exp10f(llo + t * (log10f(hi) - llo));
   double nc = V / mfp * dt;
   double rj = sqrt(nc / 100000.0) * (double)dsteps * mfp;
-  float c = (float)(2.0 * rng_uniform(rng) - 1.0);
-  float ph = (float)(2.0 * IG_PI * rng_uniform(rng));
+  float c = __builtin_fmaf(2.0f, u24(wd[1]), -1.0f);
   float one_m = 1.0f - c * c;
   float sn, cs;
-  sincosf(ph, &sn, &cs);
-  double s = rj * (double)sqrtf(one_m > 0.0f ? one_m : 0.0f);
+  azimuth(u24(wd[2]), sn, cs);
+  double s = rj * (double)__builtin_sqrtf(one_m > 0.0f ? one_m : 0.0f);
   jx = s * (double)cs;
   jy = s * (double)sn;
   jz = rj * (double)c;
