@@ -1,0 +1,75 @@
+"""The FP32-basis mode's polynomial helpers, read from the kernel header
+and evaluated in FP32 on the host: the fitted exact-Maxwell mean relative
+speed (collision3d.py:265-272) within 6e-7 relative over s in [0, 30],
+and the reduced-range sine within 2e-7 absolute over [-pi, pi]."""
+import math
+import os
+import re
+
+import numpy as np
+from scipy.special import erf
+
+HDR = os.path.join(os.path.dirname(__file__), "..", "ion_gym_amd", "csrc",
+                   "ig_device.h")
+
+
+def _body(name):
+    src = open(HDR).read()
+    i = src.index(name)
+    j = src.index("\n}\n", i)
+    return src[i:j]
+
+
+def _coeffs(body):
+    """The Horner sequence: the initial q value, then each fmaf addend."""
+    q0 = re.search(r"float q = (-?[0-9.e+-]+)f;", body).group(1)
+    adds = re.findall(r"__builtin_fmaf\(q(?: \* r2)?, (?:x|r2|r), "
+                      r"(-?[0-9.e+-]+)f\)", body)
+    return [float(q0)] + [float(a) for a in adds]
+
+
+def _exact_crel(s):
+    s = np.asarray(s, float)
+    out = np.ones_like(s)
+    m = s > 0
+    out[m] = ((s[m] + 0.5 / s[m]) * 0.5 * np.sqrt(np.pi) * erf(s[m])
+              + 0.5 * np.exp(-s[m] ** 2))
+    return out
+
+
+def test_c_rel_polynomial():
+    c = _coeffs(_body("c_rel_ratio_f(float s)"))
+    assert len(c) == 13                      # degree 12, highest first
+    s = np.linspace(0.0, 30.0, 300001).astype(np.float32)
+    f32 = np.float32
+    out = np.empty_like(s)
+    lo = s < 4.0
+    x = (s[lo] * s[lo] * f32(0.125) - f32(1.0)).astype(np.float32)
+    q = np.full_like(x, f32(c[0]))
+    for a in c[1:]:
+        q = (q * x + f32(a)).astype(np.float32)
+    out[lo] = q
+    hi = ~lo
+    out[hi] = f32(0.886226925452758) * (s[hi] + f32(0.5) / s[hi])
+    rel = np.abs(out.astype(float) - _exact_crel(s.astype(float))) / \
+        _exact_crel(s.astype(float))
+    assert rel.max() < 6e-7, rel.max()
+
+
+def test_sin_polynomial():
+    body = _body("float sin_pi_range(float r)")
+    c = _coeffs(body)
+    assert np.allclose(c, [-1 / math.factorial(11), 1 / math.factorial(9),
+                           -1 / math.factorial(7), 1 / math.factorial(5),
+                           -1 / math.factorial(3)], rtol=1e-7)
+    f32 = np.float32
+    r = np.linspace(-math.pi, math.pi, 200001).astype(np.float32)
+    rr = r.copy()
+    hp, pi = f32(1.57079632679489662), f32(3.14159265358979324)
+    rr = np.where(rr > hp, pi - rr, np.where(rr < -hp, -pi - rr, rr))
+    r2 = (rr * rr).astype(np.float32)
+    q = np.full_like(rr, f32(c[0]))
+    for a in c[1:]:
+        q = (q * r2 + f32(a)).astype(np.float32)
+    v = (q * r2 * rr + rr).astype(np.float32)
+    assert np.max(np.abs(v - np.sin(r.astype(float)))) < 2e-7
