@@ -17,6 +17,7 @@ from .. import _lib as L
 from ..engine import BASE_CHANNELS, RZ_CHANNELS, Engine, Flight
 from .collision3d import gas_kernel_scalars
 from .constants import E_CHG, KG_AMU
+from .drift_extension import rz_edge_fields, rz_extend
 from .ion_envelope import (birth_splat, make_summary, per_ion,
                            rz_birth_refusal)
 
@@ -80,6 +81,9 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
     births = np.asarray(births, np.float64)
     seed_base = spec.source.seed if spec.source.seed is not None else 0
     state = {}
+    # residual |E| on the open faces, the drift extension's guard input
+    # (measured once per model, build_rz.py:650-669)
+    edge_fields = rz_edge_fields(model)
 
     def _fields():
         if "fp" not in state:
@@ -148,9 +152,15 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
                 traj = res["records"][j, :nrec].copy()
                 if sds_on and "n_col" in col_names:
                     traj[:, col_names.index("n_col")] = np.nan
+                kind = int(res["fate"][j])
+                # declared bounds/stations beyond an escape face: one exact
+                # ballistic record (build_rz.py:746-770)
+                traj, kind, _ = rz_extend(traj, col_names, kind, spec, model,
+                                          edge_fields,
+                                          label=f"ion {int(idx[k])}")
                 end = traj[-1]
                 out[k] = (traj, make_summary(
-                    kind=int(res["fate"][j]), tof=float(end[0]),
+                    kind=kind, tof=float(end[0]),
                     mz=env[j].mz, x_end=float(end[1]), y_end=float(end[2]),
                     z_end=float(end[3]),
                     r_end=float(math.hypot(end[2], end[3])),
