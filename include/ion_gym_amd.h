@@ -54,6 +54,7 @@ extern "C" {
 /* routes */
 #define IG_ROUTE_RZ 0
 #define IG_ROUTE_XYZ 1
+#define IG_ROUTE_PLANAR 2 /* ion_gym/physics/build_planar.py:1111-1429 */
 
 /* device storage / arithmetic of the channel grids */
 #define IG_PREC_F64 0       /* FP64 storage, FP64 interpolation */
@@ -103,6 +104,9 @@ extern "C" {
 #define IG_REC_PATH (1u << 10)
 #define IG_REC_E_AXIAL_TINT (1u << 11)
 #define IG_REC_KE_TINT (1u << 12)
+#define IG_REC_E_X (1u << 13) /* planar route only */
+#define IG_REC_E_Y (1u << 14)
+#define IG_REC_E_Z (1u << 15)
 
 #define IG_MAX_CHANNELS 16
 #define IG_MAX_PLANES 16

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libion_gym_amd.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ("ig_engine.hip", "ig_rz_f64.hip", "ig_rz_f32s.hip", "ig_rz_f32.hip",
-           "ig_xyz_f64.hip", "ig_xyz_f32s.hip", "ig_xyz_f32.hip")
+           "ig_xyz_f64.hip", "ig_xyz_f32s.hip", "ig_xyz_f32.hip",
+           "ig_planar_f64.hip", "ig_planar_f32s.hip", "ig_planar_f32.hip")
 ARCH = os.environ.get("ION_GYM_AMD_ARCH", "gfx950")
 
 
@@ -58,7 +59,7 @@ def build(verbose=False, jobs=None, incremental=False):
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with cf.ThreadPoolExecutor(max_workers=jobs or min(7, len(srcs))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(8, len(srcs))) as ex:
         objs = list(ex.map(one, srcs))
     tmp = OUT + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

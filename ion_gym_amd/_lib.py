@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libion_gym_amd.so")
 
 IG_OK = 0
-ROUTE_RZ, ROUTE_XYZ = 0, 1
+ROUTE_RZ, ROUTE_XYZ, ROUTE_PLANAR = 0, 1, 2
 PREC_F64, PREC_F32_STORE, PREC_F32 = 0, 1, 2
 PRECISIONS = {"f64": PREC_F64, "f32_store": PREC_F32_STORE, "f32": PREC_F32}
 WAVE_SIN, WAVE_COS, WAVE_SQUARE, WAVE_TAB_HOLD, WAVE_TAB_LINEAR = range(5)

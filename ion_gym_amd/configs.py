@@ -307,6 +307,63 @@ def _drift_ions(n_ions, seed, mz, e_td, K0, P_torr, T_k, dt_frac, stats,
     return ions, dt, scale, E_vpm, p
 
 
+def planar_quad(n_ions=2000, seed=12, coarse=1, a=0.02, q_ref_mz=500.0,
+                rf_v=200.0, f_hz=1.0e6, n_cycles=50, vz_mm_us=0.5):
+    """Planar route: the transverse section of a linear quadrupole guide
+    (r0 = 4 mm) on a 161 x 161 grid @ 0.05 mm spanning +-4 mm about the
+    axis, rods of radius 4.6 mm as metal.  Bases (V/m, amplitude baked into
+    the channel grids as the planar route stores them): rod pair A and B as
+    the ideal potentials 1/2 +- (X^2 - Y^2) / (2 r0^2); RF 1 MHz at
+    rf_v V0-p on A and B in opposite phase; DC U on A at the Mathieu point
+    (a, q(m/z 500)) in the static set.  m/z uniform in 400-600; births in
+    a 0.5 mm disk about the axis with 0.05 mm/us transverse spread and an
+    axial drift vz; dt = period / 250; n_cycles RF cycles."""
+    h = 0.05 * coarse
+    n = 160 // coarse + 1
+    r0 = 4.0
+    xs = np.arange(n) * h
+    X, Y = np.meshgrid(xs - 4.0, xs - 4.0, indexing="ij")
+    quad = (X * X - Y * Y) / (2.0 * r0 * r0)
+
+    def grad(phi):
+        gx, gy = np.gradient(phi, h)
+        return -gx * 1e3, -gy * 1e3           # V/mm -> V/m
+
+    EAx_u, EAy_u = grad(0.5 + quad)
+    EBx_u, EBy_u = grad(0.5 - quad)
+    R = 4.6
+    c = r0 + R
+    ele = (((X - c) ** 2 + Y ** 2 <= R * R) | ((X + c) ** 2 + Y ** 2 <= R * R)
+           | (X ** 2 + (Y - c) ** 2 <= R * R) | (X ** 2 + (Y + c) ** 2
+                                                <= R * R))
+    om = f_hz * 1e-6 * 2 * math.pi
+    Om = f_hz * 2 * math.pi
+    m_ref = q_ref_mz * KG_AMU
+    q = 4.0 * E_CHG * rf_v / (m_ref * Om ** 2 * (r0 * 1e-3) ** 2)
+    U = a * rf_v / (2.0 * q)
+    pack = dict(ExA=U * EAx_u, EyA=U * EAy_u,
+                ExK=np.stack([rf_v * EAx_u, rf_v * EBx_u]),
+                EyK=np.stack([rf_v * EAy_u, rf_v * EBy_u]),
+                ele=ele.astype(np.float64), mm=h,
+                ch_kind=np.array([0, 0], np.int64),
+                ch_om=np.array([om, om]), ch_ph=np.array([0.0, math.pi]),
+                ch_duty=np.full(2, 0.5))
+    rng = np.random.default_rng(seed)
+    m = rng.uniform(400.0, 600.0, n_ions)
+    rr = 0.5 * np.sqrt(rng.uniform(0, 1, n_ions))
+    th = rng.uniform(0, 2 * math.pi, n_ions)
+    ions = dict(x=4.0 + rr * np.cos(th), y=4.0 + rr * np.sin(th),
+                z=np.zeros(n_ions), vx=rng.normal(0, 0.05, n_ions),
+                vy=rng.normal(0, 0.05, n_ions),
+                vz=np.full(n_ions, vz_mm_us), tob=np.zeros(n_ions),
+                m_amu=m, acc=E_CHG / (m * KG_AMU),
+                sigma_m2=np.full(n_ions, 2e-18),
+                seed=seed + np.arange(n_ions, dtype=np.int64))
+    dt_us = 1e6 / f_hz / 250.0
+    return dict(name="planar_quad", route="planar", pack=pack, ions=ions,
+                dt=dt_us, t_max=n_cycles * 1e6 / f_hz, U=U, q=q, a=a)
+
+
 def drift_rz(n_ions=10000, seed=4000, coarse=1, e_td=10.0, mz=500.0,
              K0=1.5, P_torr=4.0, T_k=300.0, dt_frac=0.1):
     """Drift tube with SDS: the 100-ring stack of _drift_model (uniform

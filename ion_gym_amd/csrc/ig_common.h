@@ -115,6 +115,9 @@ template <int PREC>
 int launch_rz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch, int block,
                    void *stream);
 template <int PREC>
+int launch_planar_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
+                       int block, void *stream);
+template <int PREC>
 int launch_xyz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
                     int block, void *stream);
 

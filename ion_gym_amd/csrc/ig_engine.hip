@@ -21,17 +21,31 @@ namespace {
 int launch_route(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
                         int block, void *stream) {
   if (n_launch <= 0) return 0;
-  const bool rz = f.route == IG_ROUTE_RZ;
-  switch (f.prec) {
-  case IG_PREC_F64:
-    return rz ? launch_rz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream)
-              : launch_xyz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
-  case IG_PREC_F32_STORE:
-    return rz ? launch_rz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream)
-              : launch_xyz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
-  case IG_PREC_F32:
-    return rz ? launch_rz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream)
-              : launch_xyz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
+  switch (f.route) {
+  case IG_ROUTE_RZ:
+    switch (f.prec) {
+    case IG_PREC_F64: return launch_rz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
+    case IG_PREC_F32_STORE:
+      return launch_rz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
+    case IG_PREC_F32: return launch_rz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
+    default: return -2;
+    }
+  case IG_ROUTE_XYZ:
+    switch (f.prec) {
+    case IG_PREC_F64: return launch_xyz_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
+    case IG_PREC_F32_STORE:
+      return launch_xyz_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
+    case IG_PREC_F32: return launch_xyz_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
+    default: return -2;
+    }
+  case IG_ROUTE_PLANAR:
+    switch (f.prec) {
+    case IG_PREC_F64: return launch_planar_prec<IG_PREC_F64>(f, p, e, n_launch, block, stream);
+    case IG_PREC_F32_STORE:
+      return launch_planar_prec<IG_PREC_F32_STORE>(f, p, e, n_launch, block, stream);
+    case IG_PREC_F32: return launch_planar_prec<IG_PREC_F32>(f, p, e, n_launch, block, stream);
+    default: return -2;
+    }
   default: return -2;
   }
 }
@@ -239,14 +253,15 @@ int ig_sync(ig_engine *h) {
 
 int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   if (!h || !d || !out) return fail(IG_E_ARG, "ig_fields_upload: null argument");
-  if (d->route != IG_ROUTE_RZ && d->route != IG_ROUTE_XYZ)
+  if (d->route != IG_ROUTE_RZ && d->route != IG_ROUTE_XYZ && d->route != IG_ROUTE_PLANAR)
     return fail(IG_E_ARG, "ig_fields_upload: unknown route " + std::to_string(d->route));
   if (d->precision < IG_PREC_F64 || d->precision > IG_PREC_F32)
     return fail(IG_E_ARG, "ig_fields_upload: unknown precision");
   const bool rz = d->route == IG_ROUTE_RZ;
-  if (d->n0 < 2 || d->n1 < 2 || (rz ? d->n2 != 1 : d->n2 < 2))
+  const bool two_d = d->route != IG_ROUTE_XYZ; // r-z and planar packs
+  if (d->n0 < 2 || d->n1 < 2 || (two_d ? d->n2 != 1 : d->n2 < 2))
     return fail(IG_E_ARG, "ig_fields_upload: grid shape must be at least 2 nodes per axis "
-                          "(n2 == 1 for the r-z route)");
+                          "(n2 == 1 for the r-z and planar routes)");
   if (d->n_chan < 0 || d->n_chan > IG_MAX_CHANNELS)
     return fail(IG_E_ARG, "ig_fields_upload: n_chan out of [0, " +
                               std::to_string(IG_MAX_CHANNELS) + "]");
@@ -271,7 +286,7 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   k.has_gate = rz && d->has_gate ? 1 : 0;
   k.tau_gate = k.has_gate ? d->tau_gate : -1.0;
   k.n_sets = 1 + k.K + k.has_gate;
-  k.n_comp = rz ? 2 : 3;
+  k.n_comp = two_d ? 2 : 3;
   const int nvals = k.n_sets * k.n_comp;
   k.stride = record_stride(k.prec, nvals);
   for (int s = 0; s < k.n_sets; ++s) {
@@ -1108,12 +1123,20 @@ int ig_fly_batch(ig_engine *h, ig_fields_id f, const ig_flight_params *p, const 
   if (!h || !p || !ions || !out) return fail(IG_E_ARG, "ig_fly_batch: null argument");
   auto fit = h->fields.find(f);
   if (fit == h->fields.end()) return fail(IG_E_STATE, "ig_fly_batch: unknown field id");
-  const bool rz = fit->second->k.route == IG_ROUTE_RZ;
+  const int32_t route = fit->second->k.route;
   int32_t cols = 0;
   if (p->rec_rows > 0) {
-    if (rz) {
+    if (route == IG_ROUTE_RZ) {
       cols = 7;
       for (int b = 0; b < 13; ++b) cols += (p->rec_mask >> b) & 1u;
+    } else if (route == IG_ROUTE_PLANAR) {
+      // build_planar.py:1162-1214: the r-z channels without path and the
+      // integrals, plus e_x, e_y, e_z
+      const uint32_t planar_bits = 0x3FFu | IG_REC_E_X | IG_REC_E_Y | IG_REC_E_Z;
+      if (p->rec_mask & ~planar_bits)
+        return fail(IG_E_ARG, "ig_fly_batch: record channel not produced by the planar route");
+      cols = 7;
+      for (int b = 0; b < 16; ++b) cols += (p->rec_mask >> b) & 1u;
     } else {
       cols = 10;
     }

@@ -21,13 +21,22 @@ BASE_CHANNELS = ["t", "x", "y", "z", "vx", "vy", "vz"]
 XYZ_REC_COLS = ["t", "x", "y", "z", "vx", "vy", "vz", "ex", "ey", "ez"]
 
 
+# the planar route's optional channels in its record order
+# (ion_gym/physics/build_planar.py:1162-1214)
+PLANAR_CHANNELS = ("speed", "ke_ev", "ke_x", "ke_y", "ke_z", "e_field",
+                   "e_axial", "e_radial", "e_x", "e_y", "e_z", "radius",
+                   "n_col")
+_CHANNEL_BIT = {c: b for b, c in enumerate(RZ_CHANNELS)}
+_CHANNEL_BIT.update(e_x=13, e_y=14, e_z=15)
+
+
 def rec_mask(channels):
     m = 0
     for c in channels:
-        if c not in RZ_CHANNELS:
-            raise ValueError(f"r-z record channel {c!r} is not one of "
-                             f"{list(RZ_CHANNELS)}")
-        m |= 1 << RZ_CHANNELS.index(c)
+        if c not in _CHANNEL_BIT:
+            raise ValueError(f"record channel {c!r} is not one of "
+                             f"{sorted(_CHANNEL_BIT)}")
+        m |= 1 << _CHANNEL_BIT[c]
     return m
 
 
@@ -332,6 +341,41 @@ class Engine:
         return FieldPack(self, fid.value, L.ROUTE_RZ, precision, (nx, nu),
                          K)
 
+    def upload_planar(self, pack, precision="f64", chan_scale=None):
+        """Upload a planar pack in the reference's layout: ExA, EyA
+        (nx, ny) V/m; ExK, EyK (K, nx, ny); ele (nx, ny) metal; mm;
+        ch_kind, ch_om, ch_ph, ch_duty; optional tab_t, tab_v, tab_off
+        (ion_gym/physics/build_planar.py:1111-1118)."""
+        keep = _Keep()
+        ExA = np.asarray(pack["ExA"], np.float64)
+        nx, ny = ExA.shape
+        K = int(len(pack["ch_kind"]))
+        grids = np.empty((1 + K, 2, nx, ny), np.float64)
+        grids[0, 0], grids[0, 1] = ExA, pack["EyA"]
+        for k in range(K):
+            grids[1 + k, 0] = pack["ExK"][k]
+            grids[1 + k, 1] = pack["EyK"][k]
+        d = L.FieldsDesc()
+        d.route = L.ROUTE_PLANAR
+        d.precision = L.PRECISIONS[precision]
+        d.n0, d.n1, d.n2 = nx, ny, 1
+        d.h_mm = float(pack["mm"])
+        d.u0 = 0.0
+        d.n_chan = K
+        d.has_gate = 0
+        d.tau_gate = -1.0
+        d.grids = L.ptr(keep(grids))
+        del grids
+        d.chan_scale = L.ptr(keep(chan_scale))
+        d.metal = L.ptr(keep(np.asarray(pack["ele"]) > 0.5, np.uint8),
+                        L._u8p)
+        self._waves(d, keep, pack, K, affine=False)
+        fid = C.c_int64()
+        L.check(L.lib().ig_fields_upload(self.h, C.byref(d), C.byref(fid)),
+                "ig_fields_upload")
+        return FieldPack(self, fid.value, L.ROUTE_PLANAR, precision,
+                         (nx, ny), K)
+
     def upload_xyz(self, pack, precision="f64", chan_scale=None):
         """Upload a 3-D pack in the reference's layout: EAx, EAy, EAz
         (nx, ny, nz) V/mm; ExK, EyK, EzK (K, nx, ny, nz); ele (bool); h_mm;
@@ -489,8 +533,8 @@ class Engine:
         p = flight.struct(keep)
         io = make_ions(keep, ions)
         n = io.n
-        if fields.route == L.ROUTE_RZ:
-            cols = 7 + len(flight.channels)
+        if fields.route in (L.ROUTE_RZ, L.ROUTE_PLANAR):
+            cols = 7 + len(set(flight.channels))
         else:
             cols = 10
         rows = flight.rec_rows

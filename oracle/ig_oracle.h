@@ -9,6 +9,7 @@
  * (-O2 -fno-fast-math -ffp-contract=off), the reference kernels:
  *   ion_gym/physics/tracer_rz.py:113-446   _fly_rec_full (r-z leapfrog)
  *   ion_gym/physics/tracer3d.py:280-575    _fly3d        (3-D RK4)
+ *   ion_gym/physics/build_planar.py:1111-1429 _fly_planar (planar leapfrog)
  *   ion_gym/physics/interp.py:19-36        _bilin
  *   ion_gym/physics/tracer3d.py:121-151    _tri
  *   ion_gym/physics/raster2d.py:597-612    _metal_nn
@@ -63,6 +64,20 @@ typedef struct {
 } or_xyz_fields;
 
 typedef struct {
+  /* planar grids (nx, ny), V/m (build_planar.py:1111-1118) */
+  int64_t nx, ny;
+  const double *ExA, *EyA;
+  const double *ExK, *EyK; /* (K, nx, ny) */
+  const double *ele;       /* (nx, ny) 0/1 metal */
+  double mm;
+  int32_t K;
+  const int64_t *ch_kind;
+  const double *ch_om, *ch_ph, *ch_duty;
+  const double *tab_t, *tab_v;
+  const int64_t *tab_off;
+} or_planar_fields;
+
+typedef struct {
   double dt;        /* r-z: us; 3-D: s */
   double t_max;     /* r-z: us; 3-D: s */
   int64_t max_steps; /* <= 0: fly to fate */
@@ -105,6 +120,8 @@ int or_rz_fly(const or_rz_fields *f, const or_params *p, const or_ions *ions,
               or_out *out, int n_threads);
 int or_xyz_fly(const or_xyz_fields *f, const or_params *p, const or_ions *ions,
                or_out *out, int n_threads);
+int or_planar_fly(const or_planar_fields *f, const or_params *p,
+                  const or_ions *ions, or_out *out, int n_threads);
 /* primitives exposed for the known-answer tests */
 double or_bilin(const double *F, double gx, double gu, int64_t nx, int64_t nu);
 double or_tri(const double *F, double x, double y, double z, int64_t nx,

@@ -46,6 +46,15 @@ class XYZFields(C.Structure):
                 ("tab_v", _dp), ("tab_off", _i64p)]
 
 
+class PlanarFields(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64),
+                ("ExA", _dp), ("EyA", _dp), ("ExK", _dp), ("EyK", _dp),
+                ("ele", _dp), ("mm", C.c_double), ("K", C.c_int32),
+                ("ch_kind", _i64p), ("ch_om", _dp), ("ch_ph", _dp),
+                ("ch_duty", _dp), ("tab_t", _dp), ("tab_v", _dp),
+                ("tab_off", _i64p)]
+
+
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("t_max", C.c_double),
                 ("max_steps", C.c_int64), ("rec_every", C.c_int64),
@@ -91,6 +100,8 @@ def lib():
         L.or_rz_fly.argtypes = [C.POINTER(RZFields), C.POINTER(Params),
                                 C.POINTER(Ions), C.POINTER(Out), C.c_int]
         L.or_xyz_fly.argtypes = [C.POINTER(XYZFields), C.POINTER(Params),
+                                 C.POINTER(Ions), C.POINTER(Out), C.c_int]
+        L.or_planar_fly.argtypes = [C.POINTER(PlanarFields), C.POINTER(Params),
                                  C.POINTER(Ions), C.POINTER(Out), C.c_int]
         L.or_bilin.restype = C.c_double
         L.or_bilin.argtypes = [_dp, C.c_double, C.c_double, C.c_int64,
@@ -326,4 +337,44 @@ def fly_xyz(fields, flight, ions, n_threads=None):
                           n_threads or os.cpu_count())
     if rc != 0:
         raise RuntimeError(f"or_xyz_fly failed ({rc})")
+    return res
+
+
+PLANAR_FLAGS = ("speed", "ke_ev", "ke_x", "ke_y", "ke_z", "e_field",
+                "e_axial", "e_radial", "e_x", "e_y", "e_z", "radius", "n_col")
+
+
+def fly_planar(fields, flight, ions, n_threads=None):
+    """Fly ions through a planar field pack (reference layout dict: ExA,
+    EyA, ExK, EyK (V/m), ele, mm, ch_kind, ch_om, ch_ph, ch_duty, [tab_t,
+    tab_v, tab_off]).  ``flight["flags"]`` are the 13 planar channel flags
+    in PLANAR_FLAGS order; dt/t_max in us."""
+    keep = _Keep()
+    f = PlanarFields()
+    ExA = keep(fields["ExA"])
+    f.nx, f.ny = ExA.shape
+    f.ExA = _p(ExA)
+    f.EyA = _p(keep(fields["EyA"]))
+    K = int(np.asarray(fields["ch_kind"]).shape[0])
+    f.K = K
+    f.ExK = _p(keep(fields["ExK"] if K else np.zeros(1)))
+    f.EyK = _p(keep(fields["EyK"] if K else np.zeros(1)))
+    f.ele = _p(keep(fields["ele"]))
+    f.mm = float(fields["mm"])
+    f.ch_kind = _p(keep(fields["ch_kind"] if K else np.zeros(1), np.int64),
+                   _i64p)
+    for k in ("ch_om", "ch_ph", "ch_duty"):
+        setattr(f, k, _p(keep(fields[k] if K else np.zeros(1))))
+    f.tab_t = _p(keep(fields.get("tab_t", np.zeros(1))))
+    f.tab_v = _p(keep(fields.get("tab_v", np.zeros(1))))
+    f.tab_off = _p(keep(fields.get("tab_off", np.zeros(K + 1)), np.int64),
+                   _i64p)
+    ncols = 7 + sum(bool(v) for v in flight.get("flags", [0] * 13))
+    p = _params(keep, flight, ncols)
+    io = _ions(keep, ions)
+    o, res = _out(io.n, p.rec_rows, ncols)
+    rc = lib().or_planar_fly(C.byref(f), C.byref(p), C.byref(io),
+                             C.byref(o), n_threads or os.cpu_count())
+    if rc != 0:
+        raise RuntimeError(f"or_planar_fly failed ({rc})")
     return res

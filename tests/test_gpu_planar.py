@@ -1,0 +1,153 @@
+"""GPU parity of the planar kernel (build_planar._fly_planar) against the
+CPU oracle on the linear quadrupole guide: FP64 ion by ion to 1e-9
+relative with identical fates in vacuum, with records of every planar
+channel, with hard-sphere gas and with SDS on the shared Philox stream,
+with station planes and bounds; FP32 storage against the oracle on
+float32-rounded grids; the FP32-basis error against FP64 stated."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from test_gpu_rz import _close
+
+pytestmark = pytest.mark.gpu
+
+
+def _parity(g, o, rel=1e-9, frac_ok=1.0):
+    ok = (g["fate"] == o["kind"]) & (g["steps"] == o["steps"])
+    ok &= g["ncol"] == o["ncol"]
+    for k in ("x", "y", "z"):
+        ok &= _close(g[k], o[k], 1.0) <= rel
+    vs = np.sqrt(o["vx"] ** 2 + o["vy"] ** 2 + o["vz"] ** 2) + 1e-30
+    for k in ("vx", "vy", "vz"):
+        ok &= _close(g[k], o[k], vs) <= rel
+    ok &= _close(g["t_us"], o["t"], 1.0) <= rel
+    i = int(np.argmin(ok)) if not ok.all() else 0
+    assert ok.mean() >= frac_ok, (
+        f"{(~ok).sum()} ions differ; ion {i}: gpu fate {g['fate'][i]} steps "
+        f"{g['steps'][i]} x {g['x'][i]!r} | oracle {o['kind'][i]} "
+        f"{o['steps'][i]} {o['x'][i]!r}")
+
+
+@pytest.fixture(scope="module")
+def quad():
+    from ion_gym_amd import configs
+    # rf_v 700 V: q(m/z 500) = 0.86, so the lighter ions are unstable and
+    # reach the rods or the box while the heavier ones are guided
+    return configs.planar_quad(n_ions=400, coarse=2, n_cycles=30, rf_v=700.0)
+
+
+def _flight(c, **kw):
+    from ion_gym_amd.engine import Flight
+    return Flight(c["dt"], c["t_max"], **kw)
+
+
+def test_planar_fp64_matches_oracle(engine, quad):
+    c = quad
+    fp = engine.upload_planar(c["pack"], "f64")
+    g = engine.fly_batch(fp, _flight(c), c["ions"])
+    o = O.fly_planar(c["pack"], dict(dt=c["dt"], t_max=c["t_max"]),
+                     c["ions"])
+    _parity(g, o)
+    assert len(np.unique(o["kind"])) >= 2
+    fp.release()
+
+
+def test_planar_records_match_oracle(engine, quad):
+    from ion_gym_amd.engine import PLANAR_CHANNELS
+    c = quad
+    ions = {k: v[:60] for k, v in c["ions"].items()}
+    rows = 400
+    fp = engine.upload_planar(c["pack"], "f64")
+    g = engine.fly_batch(fp, _flight(c, rec_every=13, rec_rows=rows,
+                                     channels=list(PLANAR_CHANNELS)), ions)
+    o = O.fly_planar(c["pack"], dict(dt=c["dt"], t_max=c["t_max"],
+                                     rec_every=13, rec_rows=rows,
+                                     flags=[1] * 13), ions)
+    _parity(g, o)
+    assert np.array_equal(g["nrec"], o["nrec"])
+    for i in range(60):
+        n = int(o["nrec"][i])
+        a, b = g["records"][i, :n], o["rec"][i, :n]
+        scale = np.maximum(np.abs(b).max(axis=0), 1e-6)
+        assert np.all(np.abs(a - b) <= 1e-9 * scale), i
+    fp.release()
+
+
+def _gas(model):
+    from ion_gym_amd.physics.collision3d import gas_kernel_scalars
+    g0 = gas_kernel_scalars("He", T_k=300.0, P_torr=0.05)
+    return dict(model=model, flow=(0.0, 0.0, 0.02),
+                **{k: g0[k] for k in ("T_k", "P_pa", "c_star", "c_bar",
+                                      "sig1d", "m_gas")})
+
+
+def test_planar_hard_sphere_matches_oracle_philox(engine, quad):
+    c = quad
+    gas = _gas(1)
+    fp = engine.upload_planar(c["pack"], "f64")
+    g = engine.fly_batch(fp, _flight(c, gas=gas), c["ions"])
+    o = O.fly_planar(c["pack"], dict(dt=c["dt"], t_max=c["t_max"], gas=gas,
+                                     rng=O.RNG_PHILOX), c["ions"])
+    assert o["ncol"].sum() > 100
+    _parity(g, o, frac_ok=0.99)
+    fp.release()
+
+
+def test_planar_sds_matches_oracle_philox(engine, quad):
+    from ion_gym_amd.physics import sds as sds_mod
+    c = quad
+    gas = _gas(2)
+    gas["sds_stats"] = sds_mod.default_statistics()
+    ions = {k: v[:200] for k, v in c["ions"].items()}
+    p = sds_mod.transport_params(500.0, 1, gas["m_gas"], 300.0, 0.05, 1.5,
+                                 stats=gas["sds_stats"])
+    ions["sds"] = np.tile([p["damping"], p["mfp_mm"], p["V_mm_us"],
+                           p["log_mr_ratio"]], (200, 1))
+    fp = engine.upload_planar(c["pack"], "f64")
+    g = engine.fly_batch(fp, _flight(c, gas=gas), ions)
+    o = O.fly_planar(c["pack"], dict(dt=c["dt"], t_max=c["t_max"], gas=gas,
+                                     rng=O.RNG_PHILOX), ions)
+    _parity(g, o, frac_ok=0.99)
+    fp.release()
+
+
+def test_planar_planes_and_bounds_match_oracle(engine, quad):
+    c = quad
+    planes = [(2, 3.0, 0.0, (3.9, 4.1, 3.9, 4.1), 6),      # detector patch
+              (2, 6.0, 0.0, (3.5, 4.5, 3.5, 4.5), 5)]      # aperture plate
+    bounds = ((False, False, False, False, False, True),
+              (0, 0, 0, 0, 0, 9.0))
+    fp = engine.upload_planar(c["pack"], "f64")
+    g = engine.fly_batch(fp, _flight(c, planes=planes, bounds=bounds),
+                         c["ions"])
+    o = O.fly_planar(c["pack"], dict(dt=c["dt"], t_max=c["t_max"],
+                                     bnd_on=bounds[0], bnd_val=bounds[1],
+                                     planes=planes), c["ions"])
+    _parity(g, o)
+    assert np.array_equal(g["face"][g["fate"] == 3], o["face"][o["kind"] == 3])
+    kinds = set(np.unique(o["kind"]))
+    assert {3, 5, 6} <= kinds, kinds
+    fp.release()
+
+
+def test_planar_f32_store_and_basis(engine, quad):
+    c = quad
+    p32 = dict(c["pack"])
+    for k in ("ExA", "EyA", "ExK", "EyK"):
+        p32[k] = c["pack"][k].astype(np.float32).astype(np.float64)
+    fp = engine.upload_planar(c["pack"], "f32_store")
+    g = engine.fly_batch(fp, _flight(c), c["ions"])
+    o = O.fly_planar(p32, dict(dt=c["dt"], t_max=c["t_max"]), c["ions"])
+    _parity(g, o)
+    fp.release()
+    # FP32 basis against FP64: positions within 1e-4 mm for shared fates
+    o64 = O.fly_planar(c["pack"], dict(dt=c["dt"], t_max=c["t_max"]),
+                       c["ions"])
+    fp = engine.upload_planar(c["pack"], "f32")
+    g = engine.fly_batch(fp, _flight(c), c["ions"])
+    same = g["fate"] == o64["kind"]
+    assert same.mean() > 0.97
+    for k in ("x", "y", "z"):
+        assert np.max(np.abs(g[k][same] - o64[k][same])) < 1e-4, k
+    fp.release()

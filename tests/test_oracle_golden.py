@@ -347,3 +347,64 @@ def test_rk4_metal_impact_all_eight_corners_and_box():
     assert r["z"][0] == pytest.approx(12.0, abs=1e-9)
     assert r["kind"][1] == 1
     assert r["x"][1] > (n - 1) * h
+
+
+# ---------------------------------------------------------------- planar
+def _planar_uniform(E_vpm=1000.0, K=0):
+    n = 41
+    z = np.zeros((n, n))
+    return dict(ExA=np.full((n, n), E_vpm), EyA=z.copy(),
+                ExK=np.zeros((K, n, n)), EyK=np.zeros((K, n, n)),
+                ele=z.copy(), mm=0.1, ch_kind=np.zeros(K, np.int64),
+                ch_om=np.zeros(K), ch_ph=np.zeros(K), ch_duty=np.full(K, 0.5))
+
+
+def _one_ion(x, y, z, vx, vy, vz, m=100.0, tob=0.0):
+    q = 1.602176634e-19 / (m * 1.66053906660e-27)
+    return dict(x=np.array([x]), y=np.array([y]), z=np.array([z]),
+                vx=np.array([vx]), vy=np.array([vy]), vz=np.array([vz]),
+                tob=np.array([tob]), m_amu=np.array([m]), acc=np.array([q]),
+                sigma_m2=np.zeros(1), seed=np.array([1], np.int64))
+
+
+def test_planar_leapfrog_is_exact_in_a_uniform_field():
+    """build_planar._fly_planar in a uniform transverse field: x follows
+    x0 + v t + a t^2 / 2 exactly (kick-drift-kick is exact for a constant
+    force), y and z drift freely, and the record cadence is
+    int(t / dt) % rec_every."""
+    pk = _planar_uniform()
+    ion = _one_ion(1.0, 2.0, 0.3, 0.01, 0.005, 0.2, tob=0.5)
+    dt, n = 0.01, 200
+    r = O.fly_planar(pk, dict(dt=dt, t_max=1e9, max_steps=n, rec_every=7,
+                              rec_rows=64, flags=[1] + [0] * 12), ion)
+    a = 1000.0 * 1e-9 * ion["acc"][0]
+    t = r["steps"][0] * dt
+    assert r["kind"][0] == -2 and r["steps"][0] == n
+    assert r["x"][0] == pytest.approx(1.0 + 0.01 * t + 0.5 * a * t * t,
+                                      rel=1e-12)
+    assert r["vx"][0] == pytest.approx(0.01 + a * t, rel=1e-12)
+    assert r["y"][0] == pytest.approx(2.0 + 0.005 * t, rel=1e-12)
+    assert r["z"][0] == pytest.approx(0.3 + 0.2 * t, rel=1e-12)
+    # row 0 at birth, then every step whose int(t / dt) is a multiple of 7
+    rec = r["rec"][0, :r["nrec"][0]]
+    assert rec[0, 0] == 0.5
+    ts = np.cumsum(np.full(n, dt))
+    want = [k for k in range(1, n + 1) if int(ts[k - 1] / dt) % 7 == 0]
+    assert len(rec) == 1 + len(want)
+
+
+def test_planar_two_frequency_drive_closed_form():
+    """One sin channel at W on a uniform 1 V/mm basis: from rest the
+    planar flight follows x(t) = x0 + (a / W) [t - sin(W t) / W] (the
+    closed form of ion_gym's test_multidrive gate 2), to 1e-4 relative."""
+    n = 41
+    pk = _planar_uniform(E_vpm=0.0, K=1)
+    pk["ExK"] = np.full((1, n, n), 1000.0)
+    W = 2 * math.pi * 0.5
+    pk["ch_om"] = np.array([W])
+    ion = _one_ion(1.0, 2.0, 0.0, 0.0, 0.0, 0.0)
+    r = O.fly_planar(pk, dict(dt=1e-4, t_max=0.8), ion)
+    a = 1000.0 * 1e-9 * ion["acc"][0]
+    t = r["t"][0]
+    want = 1.0 + a / W * (t - math.sin(W * t) / W)
+    assert r["x"][0] == pytest.approx(want, rel=1e-4)
