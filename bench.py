@@ -51,14 +51,15 @@ def state_bytes_per_ion():
     return 2 * (8 * 16 + 4 * 5 + 8 * 4)
 
 
-def run_gpu(cfg, precision, warmup, steps, window, engine):
+def run_gpu(cfg, precision, warmup, steps, window, engine, compact=True):
     from ion_gym_amd.engine import Flight
     model = cfg["model"]
     fp = engine.upload_rz(model.pack(), precision)
     n = len(cfg["ions"]["x"])
     ens = engine.ensemble(n)
     ens.load(cfg["ions"])
-    fl = Flight(cfg["dt"], cfg["t_max"], gas=cfg["gas"], max_steps=window)
+    fl = Flight(cfg["dt"], cfg["t_max"], gas=cfg["gas"], max_steps=window,
+                compact=compact)
     for _ in range(warmup):
         engine.fly(fp, fl, ens)
     engine.sync()
@@ -95,7 +96,8 @@ def cpu_baseline(cfg, window, n_sample, threads):
     return int(r["steps"].sum()), el
 
 
-def run_serving(eng, n_req, load, flight_us, window, capacity, seed):
+def run_serving(eng, n_req, load, flight_us, window, capacity, seed,
+                compact=True):
     """The serving leg: requests on the SLIM traveling-wave board (11 FP32
     field sets, N2 4 Torr hard sphere) arrive as a Poisson stream and are
     served by one FlightServer (continuous batching over one resident
@@ -131,7 +133,7 @@ def run_serving(eng, n_req, load, flight_us, window, capacity, seed):
     L.check(L.lib().ig_ensemble_set_tunes(eng.h, ens.id, 1, tunes))
     ens.load(cal_ions)
     fl = Flight(cal_tune["dt"], cal_tune["t_max"], gas=c["gas"],
-                max_steps=window)
+                max_steps=window, compact=compact)
     eng.fly(fp, fl, ens)
     s0 = ens.steps()
     t1 = time.perf_counter()
@@ -144,7 +146,7 @@ def run_serving(eng, n_req, load, flight_us, window, capacity, seed):
     gaps = np.random.default_rng(seed + 2).exponential(1.0 / lam, n_req)
     arrivals = list(zip(np.cumsum(gaps) - gaps[0], reqs))
     srv = FlightServer(eng, fp, Flight(cal_tune["dt"], cal_tune["t_max"],
-                                       gas=c["gas"]),
+                                       gas=c["gas"], compact=compact),
                        capacity=capacity, window=window)
     t2 = time.perf_counter()
     done = srv.run_arrivals(arrivals)
@@ -155,7 +157,7 @@ def run_serving(eng, n_req, load, flight_us, window, capacity, seed):
            "requests": len(done), "ions_per_request": "log-U(1e3, 5e5)",
            "flight_us": flight_us, "offered_load": load,
            "arrival_rate_per_s": lam, "window_steps": window,
-           "slab_capacity": capacity,
+           "slab_capacity": capacity, "compact": compact,
            "p50_s": st["p50_s"], "p99_s": st["p99_s"], "mean_s": st["mean_s"],
            "sustained_ion_steps_per_s": srv.ion_steps / wall,
            "solo_ion_steps_per_s": solo, "wall_s": wall,
@@ -220,6 +222,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32000)
     ap.add_argument("--cpu-window", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compact", type=int, default=1, choices=(0, 1),
+                    help="pack the live ions into a dense lane list before "
+                         "each window (results are identical either way)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32-vs-FP64 quadrupole error leg")
     ap.add_argument("--precision-ions", type=int, default=100_000)
@@ -246,7 +251,7 @@ def main():
     print(f"[bench] timing {args.precision}: warmup {args.warmup}, steps "
           f"{args.steps}, window {args.window}", file=sys.stderr, flush=True)
     main_run = run_gpu(cfg, args.precision, args.warmup, args.steps,
-                       args.window, eng)
+                       args.window, eng, bool(args.compact))
     value = main_run["ion_steps"] / main_run["wall_s"]
     ms_per_step = main_run["wall_s"] / args.steps * 1e3
     fp64 = None
@@ -254,7 +259,7 @@ def main():
         print("[bench] timing the plain FP64 kernel", file=sys.stderr,
               flush=True)
         r64 = run_gpu(cfg, "f64", 1, max(2, args.steps // 4), args.window,
-                      eng)
+                      eng, bool(args.compact))
         fp64 = r64["ion_steps"] / r64["wall_s"]
     # roofline of the flight kernel: algorithmic bytes per launch / mean
     # HIP-event launch time on the engine's stream
@@ -300,7 +305,8 @@ def main():
               f"offered load {args.serve_load}", file=sys.stderr, flush=True)
         serving = run_serving(eng, args.serve_requests, args.serve_load,
                               args.serve_flight_us, args.serve_window,
-                              args.serve_capacity, seed=11)
+                              args.serve_capacity, seed=11,
+                              compact=bool(args.compact))
     line = {
         "metric": "ion-steps/s at 1M ions (3-D funnel+HS), FP32 vs FP64 "
                   "plain; p50/p99 latency",
@@ -321,6 +327,7 @@ def main():
                    "field_precision": args.precision,
                    "state_precision": "f64", "dt_ns": cfg["dt"] * 1e3,
                    "live_ions_end": main_run["live_end"],
+                   "compact": bool(args.compact),
                    "field_bytes": main_run["field_bytes"]},
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,

@@ -197,6 +197,10 @@ typedef struct {
   int32_t block;       /* lanes per workgroup: 64, 128, 192 or 256
                           (0 = engine default 256); results do not depend
                           on it */
+  int32_t compact;     /* 1: gather the ensemble's live ions into a dense
+                          index list (one wave ballot each) before the
+                          window, so dead and empty slots take no lanes;
+                          results do not depend on it */
 } ig_flight_params;
 
 /* One tune: the parameters one request or ensemble of a batched launch
@@ -226,7 +230,8 @@ typedef struct {
  * acc = q e/(m amu) for RZ, qm = q e/(m amu) for XYZ (same number). */
 typedef struct {
   int64_t n;
-  const double *x, *y, *z, *vx, *vy, *vz, *tob; /* mm, mm/us, us */
+  const double *x, *y, *z, *vx, *vy, *vz, *tob; /* mm, mm/us, us; all
+                          seven NULL: zero, for ig_ensemble_births_at */
   const double *m_amu;
   const double *acc;     /* C/kg */
   const double *sigma_m2;
@@ -355,7 +360,8 @@ int ig_fly_batch(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
  * later calls replay the cached graph; a changed argument captures anew.
  * Equivalent to n_windows ig_fly calls with no host work between them
  * (the window loop of a long flight), with one launch instead of
- * n_windows.  The graph holds only the flight kernels.  Replaces the
+ * n_windows.  The graph holds only the flight kernels (and, with
+ * compact, the live-list kernels before each window).  Replaces the
  * host loop over windows (ion_gym/physics/ensemble_driver.py:381-529). */
 int ig_fly_graph(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
                  ig_ensemble_id e, int32_t n_windows);
@@ -363,6 +369,71 @@ int ig_fly_graph(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
 /* Kernel timing of the last ig_fly on the engine's stream (HIP events
  * recorded around the launches), in milliseconds. */
 int ig_last_fly_ms(ig_engine *h, double *out);
+
+/* ---- births on the device ------------------------------------------- */
+
+#define IG_BIRTH_POINT 0
+#define IG_BIRTH_DISC 1
+#define IG_BIRTH_LINE 2
+#define IG_BIRTH_GRID 3
+#define IG_BIRTH_BOX 4
+#define IG_BIRTH_GAUSSIAN 5
+
+#define IG_E_BIRTH_TRIES (-6) /* a truncated-Gaussian birth drew no value
+                                 inside its truncation in gauss_tries */
+
+/* One source of the reference's generate_births
+ * (ion_gym/physics/sim_build.py:82-277), for every distribution but
+ * "file" (rows read from a CSV on the host).  The n_per * n_mz ions draw,
+ * in index order, from ONE numpy PCG64 stream (np.random.default_rng(seed))
+ * exactly as the reference's loop does: the position draws of the
+ * distribution, then the kinetic energy, the thermal and the per-axis
+ * velocity spreads, then the birth time.  Host-derived quantities are
+ * passed as the reference computes them in Python. */
+typedef struct {
+  int32_t distribution;   /* IG_BIRTH_* */
+  int32_t axis;           /* disc normal / line direction: 0 x, 1 y, 2 z */
+  int64_t n_per;          /* ions per mass (source.n_ions) */
+  int32_t n_mz;           /* masses in mz_list (>= 1) */
+  const double *mz_list;  /* n_mz masses, Da (host) */
+  double p0[3];           /* x0_mm, y0_mm, z0_mm */
+  double r_mm, len_mm;
+  double box_mm[3];
+  double gauss_sigma[3];  /* gaussian_layout(): sigma per axis, mm (0 = none) */
+  double gauss_trunc[3];  /* truncation half-widths, mm */
+  int32_t gauss_tries;    /* GAUSSIAN_BIRTH_TRIES (sim_build.py:78) */
+  int32_t has_ke;         /* ke_hi > 0 or ke_lo > 0: an energy is drawn */
+  double ke_lo, ke_hi;    /* eV */
+  double dir_unit[3];     /* direction / |direction| (numpy), or zeros */
+  int32_t has_dir;        /* |direction| > 0: the directed term is added */
+  double temperature_k;   /* > 0: thermal spread sqrt(kT/m)/1000 */
+  int32_t has_dv;         /* any dv_fwhm_ms entry nonzero */
+  double dv_sigma[3];     /* dv_fwhm_ms * 1e-3 / FWHM_PER_SIGMA, mm/us */
+  int32_t has_drift;      /* any v_drift_mm_us entry nonzero */
+  double v_drift[3];      /* mm/us */
+  double tob_span_us;
+  uint64_t pcg_state[2];  /* numpy PCG64 state after seeding: (high, low) */
+  uint64_t pcg_inc[2];    /* its increment: (high, low) */
+  const uint64_t *zig_k;  /* numpy's standard-normal ziggurat: 256 fast-  */
+  const double *zig_w;    /* accept thresholds, layer widths and layer    */
+  const double *zig_f;    /* densities (host) */
+} ig_births_desc;
+
+/* The (N, 7) births [x, y, z, vx, vy, vz, tob] of a source into a host
+ * buffer (row-major, caller allocated), N = n_per * n_mz: the replacement
+ * of generate_births (ion_gym/physics/sim_build.py:82-277).  Returns
+ * IG_E_BIRTH_TRIES where the reference raises for a truncated Gaussian
+ * that no draw satisfies (sim_build.py:190-205). */
+int ig_births(ig_engine *h, const ig_births_desc *d, double *out);
+
+/* The same births written on the device into the birth columns (x..vz,
+ * tob) of ensemble slots [offset, offset + N), which must already be
+ * loaded (ig_ensemble_load_at with NULL birth pointers leaves them zero
+ * for this call): a request's ions never pass through host memory.
+ * Replaces generate_births ahead of fly_fn for a served request
+ * (ion_gym/physics/sim_build.py:722 build_run). */
+int ig_ensemble_births_at(ig_engine *h, ig_ensemble_id id, int64_t offset,
+                          const ig_births_desc *d);
 
 #ifdef __cplusplus
 }

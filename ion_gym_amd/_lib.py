@@ -64,7 +64,8 @@ class FlightParams(C.Structure):
                 ("pl_w", (C.c_double * 4) * MAX_PLANES),
                 ("pl_kind", C.c_int32 * MAX_PLANES),
                 ("rng_key", C.c_uint64), ("coll_batch", C.c_int32),
-                ("occupancy", C.c_int32), ("block", C.c_int32)]
+                ("occupancy", C.c_int32), ("block", C.c_int32),
+                ("compact", C.c_int32)]
 
 
 class Tune(C.Structure):
@@ -92,6 +93,30 @@ class Results(C.Structure):
         ("fate", _i32p), ("face", _i32p), ("label", _i32p),
         ("ncol", _i64p), ("steps", _i64p), ("nrec", _i32p),
         ("records", _dp)]
+
+
+BIRTH_POINT, BIRTH_DISC, BIRTH_LINE, BIRTH_GRID, BIRTH_BOX, \
+    BIRTH_GAUSSIAN = range(6)
+E_BIRTH_TRIES = -6
+_u64p = C.POINTER(C.c_uint64)
+_d3 = C.c_double * 3
+_u64x2 = C.c_uint64 * 2
+
+
+class BirthsDesc(C.Structure):
+    _fields_ = [("distribution", C.c_int32), ("axis", C.c_int32),
+                ("n_per", C.c_int64), ("n_mz", C.c_int32),
+                ("mz_list", _dp), ("p0", _d3), ("r_mm", C.c_double),
+                ("len_mm", C.c_double), ("box_mm", _d3),
+                ("gauss_sigma", _d3), ("gauss_trunc", _d3),
+                ("gauss_tries", C.c_int32), ("has_ke", C.c_int32),
+                ("ke_lo", C.c_double), ("ke_hi", C.c_double),
+                ("dir_unit", _d3), ("has_dir", C.c_int32),
+                ("temperature_k", C.c_double), ("has_dv", C.c_int32),
+                ("dv_sigma", _d3), ("has_drift", C.c_int32),
+                ("v_drift", _d3), ("tob_span_us", C.c_double),
+                ("pcg_state", _u64x2), ("pcg_inc", _u64x2),
+                ("zig_k", _u64p), ("zig_w", _dp), ("zig_f", _dp)]
 
 
 # every entry point of include/ion_gym_amd.h: name -> (restype, argtypes)
@@ -138,6 +163,9 @@ SIGNATURES = {
     "ig_fields_read_set": (C.c_int, [_VP, C.c_int64, C.c_int32, C.c_int32,
                                      _dp]),
     "ig_last_fly_ms": (C.c_int, [_VP, C.POINTER(C.c_double)]),
+    "ig_births": (C.c_int, [_VP, C.POINTER(BirthsDesc), _dp]),
+    "ig_ensemble_births_at": (C.c_int, [_VP, C.c_int64, C.c_int64,
+                                        C.POINTER(BirthsDesc)]),
 }
 
 

@@ -94,8 +94,8 @@ struct EnsK {
   double *path, *eat, *ket; // record accumulators
   double *e_last;      // (n, 3) last evaluated field (acceleration units)
   double *rec;         // (n, rec_rows, rec_cols) or null
-  const int32_t *idx;  // optional compacted list of live ions
-  int64_t n_idx;
+  const int32_t *idx;        // optional compacted list of live ions
+  const uint32_t *n_live;    // its length (device), with idx
   const int32_t *tune; // per-ion tune index (with tunes)
   const Tune *tunes;   // per-ensemble tune table, or null: the pack's
                        // waveforms and the flight's dt/t_max/rng_key

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "ig_births.h"
 #include "ig_common.h"
 #include "ig_compose_kernel.h"
 
@@ -133,6 +135,8 @@ struct Ensemble {
   DevBuf tunes;         // Tune table (device)
   DevBuf counts;        // live_by_tune scratch (device)
   DevBuf sums;          // live/steps reduction (device)
+  DevBuf live_idx;      // compacted live-ion list (int32, capacity)
+  DevBuf live_cnt;      // its length (uint32)
   int32_t *tune_idx = nullptr;
   std::vector<std::unique_ptr<DevBuf>> bufs;
   EnsK k{};
@@ -160,6 +164,7 @@ struct ig_engine {
   std::map<int64_t, std::unique_ptr<Bases>> bases;
   int block = 256;
   std::vector<GraphEntry> graphs; // captured window loops (ig_fly_graph)
+  DevBuf bw, bj, bflag, btab, bout; // births: words, jumps, status, tables, rows
 };
 
 namespace {
@@ -428,7 +433,7 @@ int ig_ensemble_create(ig_engine *h, int64_t capacity, int32_t rec_rows, int32_t
   k.rec = nullptr;
   if (rec_rows > 0 && (rc = E->alloc(k.rec, n * (size_t)rec_rows * rec_cols))) return rc;
   k.idx = nullptr;
-  k.n_idx = 0;
+  k.n_live = nullptr;
   k.n = 0;
   int64_t id = h->next_id++;
   h->ens[id] = std::move(E);
@@ -459,9 +464,12 @@ static int load_range(ig_engine *h, Ensemble *E, int64_t off, const ig_ions *io,
     return fail(IG_E_ARG, w + ": ions [" + std::to_string(off) + ", " +
                               std::to_string(off + io->n) + ") exceed the ensemble capacity " +
                               std::to_string(E->cap));
-  if (io->n > 0 && (!io->x || !io->y || !io->z || !io->vx || !io->vy || !io->vz || !io->tob ||
+  const bool no_births =
+      !io->x && !io->y && !io->z && !io->vx && !io->vy && !io->vz && !io->tob;
+  if (io->n > 0 && ((!no_births && (!io->x || !io->y || !io->z || !io->vx || !io->vy ||
+                                    !io->vz || !io->tob)) ||
                     !io->m_amu || !io->acc || !io->seed))
-    return fail(IG_E_ARG, w + ": births, m_amu, acc and seed are required");
+    return fail(IG_E_ARG, w + ": births (all seven or none), m_amu, acc and seed are required");
   if (tune < 0 || tune >= (E->n_tunes > 0 ? E->n_tunes : 1))
     return fail(IG_E_ARG, w + ": tune " + std::to_string(tune) + " is not in the tune table");
   if (int rc = set_dev(h)) return rc;
@@ -477,7 +485,12 @@ static int load_range(ig_engine *h, Ensemble *E, int64_t off, const ig_ions *io,
             {k.vx + off, io->vx},       {k.vy + off, io->vy},   {k.vz + off, io->vz},
             {k.tob + off, io->tob},     {k.m_amu + off, io->m_amu}, {k.acc + off, io->acc},
             {k.seed + off, io->seed}};
-  for (auto &c : cp) HIPX(hipMemcpyAsync(c.dst, c.src, b8, hipMemcpyHostToDevice, st));
+  for (auto &c : cp) {
+    if (c.src)
+      HIPX(hipMemcpyAsync(c.dst, c.src, b8, hipMemcpyHostToDevice, st));
+    else
+      HIPX(hipMemsetAsync(c.dst, 0, b8, st));
+  }
   if (io->sigma_m2)
     HIPX(hipMemcpyAsync(k.sigma + off, io->sigma_m2, b8, hipMemcpyHostToDevice, st));
   else
@@ -812,6 +825,54 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
   k.occupancy = p->occupancy;
   if (p->block != 0 && (p->block < 64 || p->block > 256 || p->block % 64 != 0))
     return fail(IG_E_ARG, "ig_fly: block must be 0 (default), 64, 128, 192 or 256");
+  if (p->compact != 0 && p->compact != 1) return fail(IG_E_ARG, "ig_fly: compact must be 0 or 1");
+  return 0;
+}
+
+// The live ions of an ensemble as a dense index list: one ballot per
+// wave, one atomic per wave for the wave's slice, lanes in order within
+// it (waves in any order: an ion's flight does not depend on its lane).
+__global__ void compact_live_kernel(const int32_t *__restrict__ fate, int64_t n,
+                                    int32_t *__restrict__ idx, uint32_t *__restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n && fate[i] == IG_FATE_FLYING;
+  const uint64_t m = __ballot(live);
+  if (m == 0) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int first = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, first);
+  if (live) idx[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
+}
+
+__global__ void zero_u32_kernel(uint32_t *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+}
+
+// the compaction buffers, sized for the ensemble's capacity (allocated
+// once, before any capture)
+static int live_bufs(Ensemble *E) {
+  if (!E->live_idx.p)
+    if (int rc = dev_alloc(E->live_idx, (size_t)std::max<int64_t>(E->cap, 1) * sizeof(int32_t)))
+      return rc;
+  if (!E->live_cnt.p)
+    if (int rc = dev_alloc(E->live_cnt, sizeof(uint32_t))) return rc;
+  return 0;
+}
+
+// the kernel arguments of one window: with compaction, the live list is
+// rebuilt on the stream first (kernels only, so a graph may capture it)
+static int window_ens(ig_engine *h, Ensemble *E, const ig_flight_params *p, EnsK &ek) {
+  ek = E->k;
+  if (!p->compact || E->n == 0) return 0;
+  if (int rc = live_bufs(E)) return rc;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, (uint32_t *)E->live_cnt.p);
+  hipLaunchKernelGGL(compact_live_kernel, dim3((unsigned)((E->n + 255) / 256)), dim3(256), 0,
+                     h->stream, E->k.fate, E->n, (int32_t *)E->live_idx.p,
+                     (uint32_t *)E->live_cnt.p);
+  ek.idx = (const int32_t *)E->live_idx.p;
+  ek.n_live = (const uint32_t *)E->live_cnt.p;
   return 0;
 }
 
@@ -865,7 +926,9 @@ int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensembl
   Ensemble *E = nullptr;
   if (int rc = prepare_flight(h, fid, p, eid, "ig_fly", f, k, E)) return rc;
   HIPX(hipEventRecord(h->ev0, h->stream));
-  int rc = launch_route(*f, k, E->k, E->n, p->block > 0 ? p->block : h->block, h->stream);
+  EnsK ek;
+  if (int rc = window_ens(h, E, p, ek)) return rc;
+  int rc = launch_route(*f, k, ek, E->n, p->block > 0 ? p->block : h->block, h->stream);
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
   if (rc) return launch_error(rc, "ig_fly");
@@ -881,6 +944,8 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
   Ensemble *E = nullptr;
   if (int rc = prepare_flight(h, fid, p, eid, "ig_fly_graph", f, k, E)) return rc;
   const int block = p->block > 0 ? p->block : h->block;
+  if (p->compact)
+    if (int rc = live_bufs(E)) return rc;
   // the graph is a function of the exact kernel arguments and launch shape
   std::vector<uint8_t> key(sizeof(FieldsK) + sizeof(FlightK) + sizeof(EnsK) + 3 * 8);
   {
@@ -891,7 +956,7 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
     q += sizeof(FlightK);
     std::memcpy(q, &E->k, sizeof(EnsK));
     q += sizeof(EnsK);
-    const int64_t tail[3] = {E->n, block, n_windows};
+    const int64_t tail[3] = {E->n, block, n_windows + (p->compact ? (1ll << 40) : 0)};
     std::memcpy(q, tail, sizeof(tail));
   }
   hipGraphExec_t exec = nullptr;
@@ -901,8 +966,11 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
     hipGraph_t graph = nullptr;
     HIPX(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int32_t wdx = 0; wdx < n_windows && rc == 0; ++wdx)
-      rc = launch_route(*f, k, E->k, E->n, block, h->stream);
+    for (int32_t wdx = 0; wdx < n_windows && rc == 0; ++wdx) {
+      EnsK ek;
+      rc = window_ens(h, E, p, ek);
+      if (rc == 0) rc = launch_route(*f, k, ek, E->n, block, h->stream);
+    }
     hipError_t ce = hipStreamEndCapture(h->stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -1105,6 +1173,160 @@ int ig_fields_read_set(ig_engine *h, ig_fields_id fid, int32_t set, int32_t comp
                       h->stream));
   HIPX(hipStreamSynchronize(h->stream));
   return 0;
+}
+
+// a scratch buffer of at least `bytes` (grown, never shrunk)
+static int scratch(DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return 0;
+  b.reset();
+  return dev_alloc(b, bytes);
+}
+
+// Births of a source (ig_births.h): validate, size the word stream and the
+// jump table, run the passes, and grow the stream when an ion would start
+// past it.  `out` is filled for the destination; `n_out` receives N.
+static int births_run(ig_engine *h, const ig_births_desc *d, const char *who,
+                      const std::function<BirthsOut(int64_t)> &dest, int64_t *n_out) {
+  const std::string w(who);
+  if (!d) return fail(IG_E_ARG, w + ": null source");
+  if (d->distribution < IG_BIRTH_POINT || d->distribution > IG_BIRTH_GAUSSIAN)
+    return fail(IG_E_ARG, w + ": unknown distribution " + std::to_string(d->distribution));
+  if (d->n_per < 1 || d->n_mz < 1 || !d->mz_list)
+    return fail(IG_E_ARG, w + ": n_per >= 1 ions per mass and n_mz >= 1 masses are required");
+  if (d->axis < 0 || d->axis > 2) return fail(IG_E_ARG, w + ": axis must be 0, 1 or 2");
+  if (!d->zig_k || !d->zig_w || !d->zig_f)
+    return fail(IG_E_ARG, w + ": the standard-normal layer tables are required");
+  if (d->distribution == IG_BIRTH_GAUSSIAN && d->gauss_tries < 1)
+    return fail(IG_E_ARG, w + ": gauss_tries must be >= 1");
+  if ((d->pcg_inc[1] & 1u) == 0) return fail(IG_E_ARG, w + ": the PCG64 increment must be odd");
+  const int64_t n = d->n_per * (int64_t)d->n_mz;
+  *n_out = n;
+  if (int rc = set_dev(h)) return rc;
+  // expected words per ion: one per uniform, ~1.03 per normal, 1/p per
+  // truncated-Gaussian axis with p the probability of landing inside
+  double est = 1.0; // the birth time
+  if (d->distribution == IG_BIRTH_DISC) est += 2.0;
+  if (d->distribution == IG_BIRTH_BOX) est += 3.0;
+  if (d->distribution == IG_BIRTH_GAUSSIAN)
+    for (int k = 0; k < 3; ++k)
+      if (d->gauss_sigma[k] > 0.0) {
+        const double acc = std::erf(d->gauss_trunc[k] / (d->gauss_sigma[k] * std::sqrt(2.0)));
+        est += 1.03 / std::max(acc, 1e-3);
+      }
+  if (d->has_ke) est += 1.0;
+  if (d->temperature_k > 0.0) est += 3.09;
+  if (d->has_dv) est += 3.09;
+  BirthsK b{};
+  b.dist = d->distribution;
+  b.axis = d->axis;
+  b.n_per = d->n_per;
+  b.n = n;
+  b.n_mz = d->n_mz;
+  b.grid_g = (int64_t)std::ceil(std::sqrt((double)d->n_per));
+  for (int k = 0; k < 3; ++k) {
+    b.p0[k] = d->p0[k];
+    b.box[k] = d->box_mm[k];
+    b.gsig[k] = d->gauss_sigma[k];
+    b.gtrunc[k] = d->gauss_trunc[k];
+    b.dir[k] = d->dir_unit[k];
+    b.dvs[k] = d->dv_sigma[k];
+    b.vd[k] = d->v_drift[k];
+  }
+  b.r_mm = d->r_mm;
+  b.len_mm = d->len_mm;
+  b.gtries = d->gauss_tries;
+  b.has_ke = d->has_ke;
+  b.has_dir = d->has_dir;
+  b.has_dv = d->has_dv;
+  b.has_drift = d->has_drift;
+  b.ke_lo = d->ke_lo;
+  b.ke_hi = d->ke_hi;
+  b.temp_k = d->temperature_k;
+  b.tob_span = d->tob_span_us;
+  // tables: 256 thresholds, 256 widths, 256 densities, the masses
+  const size_t tab_bytes = 256 * 8 * 3 + (size_t)d->n_mz * 8;
+  if (int rc = scratch(h->btab, tab_bytes)) return rc;
+  uint8_t *tb = (uint8_t *)h->btab.p;
+  HIPX(hipMemcpyAsync(tb, d->zig_k, 256 * 8, hipMemcpyHostToDevice, h->stream));
+  HIPX(hipMemcpyAsync(tb + 2048, d->zig_w, 256 * 8, hipMemcpyHostToDevice, h->stream));
+  HIPX(hipMemcpyAsync(tb + 4096, d->zig_f, 256 * 8, hipMemcpyHostToDevice, h->stream));
+  HIPX(hipMemcpyAsync(tb + 6144, d->mz_list, (size_t)d->n_mz * 8, hipMemcpyHostToDevice,
+                      h->stream));
+  b.zk = (const uint64_t *)tb;
+  b.zw = (const double *)(tb + 2048);
+  b.zf = (const double *)(tb + 4096);
+  b.mz = (const double *)(tb + 6144);
+  int32_t levels = 1;
+  while (levels < 62 && ((n - 1) >> levels) > 0) ++levels;
+  if (int rc = scratch(h->bflag, 16)) return rc;
+  const BirthsOut out = dest(n);
+  for (int c = 0; c < 7; ++c)
+    if (!out.col[c]) return fail(IG_E_NOMEM, w + ": no device memory for the births");
+  double grow = 1.15;
+  for (int attempt = 0; attempt < 8; ++attempt, grow *= 1.5) {
+    const int64_t P = (int64_t)std::ceil((double)n * est * grow) + 4096;
+    const int64_t nW = P + 65536;
+    if (P + 2 > (int64_t)INT32_MAX / 2)
+      return fail(IG_E_ARG, w + ": " + std::to_string(n) + " births need more stream positions "
+                                "than one call indexes; split the source");
+    if (int rc = scratch(h->bw, (size_t)nW * 8)) return rc;
+    if (int rc = scratch(h->bj, (size_t)levels * (size_t)(P + 2) * 4)) return rc;
+    if (births_launch(b, d->pcg_state[0], d->pcg_state[1], d->pcg_inc[0], d->pcg_inc[1],
+                      (uint64_t *)h->bw.p, nW, (int32_t *)h->bj.p, P, levels, out,
+                      (int64_t *)h->bflag.p, h->stream))
+      return fail(IG_E_HIP, w + ": births launch failed: " + hipGetErrorString(hipGetLastError()));
+    int64_t flag[2] = {0, 0};
+    HIPX(hipMemcpyAsync(flag, h->bflag.p, 16, hipMemcpyDeviceToHost, h->stream));
+    HIPX(hipStreamSynchronize(h->stream));
+    if (flag[0] & 1) continue; // an ion starts past the stream: grow it
+    if (flag[0] & 2)
+      return fail(IG_E_BIRTH_TRIES,
+                  w + ": ion " + std::to_string(flag[1]) +
+                      ": no draw of the truncated Gaussian fell inside the truncation in " +
+                      std::to_string(d->gauss_tries) + " tries");
+    return 0;
+  }
+  return fail(IG_E_STATE, w + ": the word stream could not be sized for the births");
+}
+
+int ig_births(ig_engine *h, const ig_births_desc *d, double *out) {
+  if (!h || !out) return fail(IG_E_ARG, "ig_births: null argument");
+  int64_t n = 0;
+  int rc = births_run(h, d, "ig_births",
+                      [&](int64_t nn) {
+                        BirthsOut o{};
+                        if (scratch(h->bout, (size_t)std::max<int64_t>(nn, 1) * 7 * 8) == 0)
+                          for (int c = 0; c < 7; ++c) o.col[c] = (double *)h->bout.p + c;
+                        o.stride = 7;
+                        return o;
+                      },
+                      &n);
+  if (rc) return rc;
+  HIPX(hipMemcpyAsync(out, h->bout.p, (size_t)n * 7 * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPX(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int ig_ensemble_births_at(ig_engine *h, ig_ensemble_id id, int64_t offset,
+                          const ig_births_desc *d) {
+  if (!h || !d) return fail(IG_E_ARG, "ig_ensemble_births_at: null argument");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_births_at: unknown ensemble id");
+  const int64_t n = d->n_per * (int64_t)d->n_mz;
+  if (offset < 0 || n < 0 || offset + n > E->n)
+    return fail(IG_E_ARG, "ig_ensemble_births_at: slots [" + std::to_string(offset) + ", " +
+                              std::to_string(offset + n) + ") are not all loaded");
+  int64_t nn = 0;
+  const EnsK &k = E->k;
+  return births_run(h, d, "ig_ensemble_births_at",
+                    [&](int64_t) {
+                      BirthsOut o{};
+                      double *cols[7] = {k.x, k.y, k.z, k.vx, k.vy, k.vz, k.tob};
+                      for (int c = 0; c < 7; ++c) o.col[c] = cols[c] + offset;
+                      o.stride = 1;
+                      return o;
+                    },
+                    &nn);
 }
 
 int ig_last_fly_ms(ig_engine *h, double *out) {

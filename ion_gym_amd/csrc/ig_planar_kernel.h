@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i;
   if (e.idx) {
-    if (tid >= e.n_idx) return;
+    if (tid >= (int64_t)*e.n_live) return;
     i = e.idx[tid];
   } else {
     if (tid >= e.n) return;

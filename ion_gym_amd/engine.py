@@ -133,13 +133,22 @@ class Ensemble:
         self.id = None
 
 
+_BIRTH_KEYS = ("x", "y", "z", "vx", "vy", "vz", "tob")
+
+
 def make_ions(keep, ions):
-    n = int(len(ions["x"]))
+    """The C ABI's ig_ions for a dict of per-ion arrays.  Without any of
+    the birth keys (x..vz, tob) the slots' births stay zero, to be drawn
+    on the device (ion_gym_amd.births.births_into)."""
+    no_births = all(ions.get(k) is None for k in _BIRTH_KEYS)
+    n = int(len(ions["m_amu"] if no_births else ions["x"]))
     io = L.Ions()
     io.n = n
     for k in ("x", "y", "z", "vx", "vy", "vz", "tob", "m_amu", "acc"):
         a = ions.get(k)
         if a is None:
+            if no_births and k in _BIRTH_KEYS:
+                continue
             if k == "tob":
                 a = np.zeros(n)
             else:
@@ -198,7 +207,7 @@ class Flight:
 
     def __init__(self, dt, t_max, *, max_steps=0, rec_every=1, rec_rows=0,
                  channels=(), gas=None, bounds=None, planes=(), rng_key=0,
-                 coll_batch=0, occupancy=0, block=0):
+                 coll_batch=0, occupancy=0, block=0, compact=True):
         self.dt = float(dt)
         self.t_max = float(t_max)
         self.max_steps = int(max_steps)
@@ -212,6 +221,7 @@ class Flight:
         self.coll_batch = int(coll_batch)
         self.occupancy = int(occupancy)
         self.block = int(block)
+        self.compact = bool(compact)
 
     def launch(self, **kw):
         """A copy with other launch settings (block, occupancy,
@@ -219,9 +229,9 @@ class Flight:
         import copy
         f = copy.copy(self)
         for k, v in kw.items():
-            if k not in ("block", "occupancy", "coll_batch"):
+            if k not in ("block", "occupancy", "coll_batch", "compact"):
                 raise ValueError(f"not a launch setting: {k}")
-            setattr(f, k, int(v))
+            setattr(f, k, bool(v) if k == "compact" else int(v))
         return f
 
     def struct(self, keep):
@@ -263,6 +273,7 @@ class Flight:
         p.coll_batch = self.coll_batch
         p.occupancy = self.occupancy
         p.block = self.block
+        p.compact = int(self.compact)
         return p
 
 

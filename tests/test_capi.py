@@ -47,7 +47,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     structs = {"ig_fields_desc": _lib.FieldsDesc,
                "ig_flight_params": _lib.FlightParams,
                "ig_ions": _lib.Ions, "ig_results": _lib.Results,
-               "ig_tune": _lib.Tune}
+               "ig_tune": _lib.Tune, "ig_bases_desc": _lib.BasesDesc,
+               "ig_births_desc": _lib.BirthsDesc}
     lines = ["#include <stdio.h>", "#include <stddef.h>",
              '#include "ion_gym_amd.h"', "int main(void) {"]
     for cname, py in structs.items():
