@@ -267,6 +267,25 @@ def generate_births(source, engine, seed=None, deck_path=None):
     return out
 
 
+def ion_attributes(source, sigma_m2=None, seed=None):
+    """The per-ion arrays of a source other than its births: mass (Da) by
+    contiguous block, acc = q e / (m amu) (ion_gym/physics/build_rz.py:
+    703-713), the per-ion seed base + i (ion_envelope.py:82-92) and the
+    hard-sphere cross-section (a number, or a function of the mass)."""
+    from .physics.constants import E_CHG, KG_AMU
+    n_per = int(source.n_ions)
+    mz = np.asarray(source.mz_list, float)
+    m = np.repeat(mz, n_per)
+    base = resolve_run_seed(source) if seed is None else int(seed)
+    out = dict(m_amu=m, acc=int(source.charge) * E_CHG / (m * KG_AMU),
+               seed=base + np.arange(len(m), dtype=np.int64))
+    if sigma_m2 is not None:
+        out["sigma_m2"] = (np.asarray([sigma_m2(v) for v in m], float)
+                           if callable(sigma_m2)
+                           else np.full(len(m), float(sigma_m2)))
+    return out
+
+
 def births_into(engine, ensemble, offset, source, seed=None):
     """Draw the source's births on the device straight into ensemble
     slots [offset, offset + N), which must be loaded already (per-ion

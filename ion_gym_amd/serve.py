@@ -42,8 +42,14 @@ BYTES_PER_ION = 8 * 19 + 8 * 3 + 8 * 4 + 4 * 6
 
 @dataclass
 class FlightRequest:
+    """One request: its ions' per-ion arrays and its tune.  With a
+    ``source`` (a SourceSpec), ``ions`` may omit the births (x..vz, tob):
+    they are drawn on the device into the request's slots at admission
+    (ion_gym_amd.births), so the request's births never exist on the
+    host."""
     ions: dict
     tune: dict
+    source: Optional[object] = None
     rid: int = -1
     t_submit: float = 0.0
     t_admit: float = 0.0
@@ -53,7 +59,8 @@ class FlightRequest:
 
     @property
     def n(self):
-        return len(self.ions["x"])
+        return len(self.ions["x"] if "x" in self.ions else
+                   self.ions["m_amu"])
 
     @property
     def latency(self):
@@ -198,6 +205,9 @@ class FlightServer:
             L.check(L.lib().ig_ensemble_load_at(self.engine.h, self.slab.id,
                                                 off, io, row),
                     "ig_ensemble_load_at")
+            if req.source is not None and "x" not in req.ions:
+                from .births import births_into
+                births_into(self.engine, self.slab, off, req.source)
         if dirty:
             self._set_tunes()
 

@@ -127,3 +127,59 @@ def test_slim_requests_are_the_same_alone_and_in_the_mix(engine):
         assert r.result["ncol"].sum() > 0
     srv.release()
     fp.release()
+
+
+def test_requests_by_source_draw_births_on_the_device(engine):
+    """A request that brings a SourceSpec instead of births rows has its
+    births drawn on the device into its slots at admission; its results
+    equal those of the same request carrying the rows generate_births
+    returns, alone and in a mix with row-carrying requests."""
+    from ion_gym_amd import births as B
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    from ion_gym_amd.physics.flight_spec import SourceSpec
+    from ion_gym_amd.serve import FlightRequest, FlightServer, make_tune
+    c = configs.funnel_rz(n_ions=3000, seed=3, coarse=2)
+    fp = engine.upload_rz(c["model"].pack(), "f32")
+    fl = Flight(c["dt"], c["t_max"], gas=c["gas"])
+
+    def src(k):
+        return SourceSpec(distribution="disc", n_ions=700, r_mm=3.0,
+                          x0_mm=0.5, ke_lo=0.0, ke_hi=0.0,
+                          temperature_k=300.0, mz_list=[300.0, 900.0],
+                          tob_span_us=0.2, seed=40 + k)
+
+    def tune(k):
+        return make_tune(c["model"].pack(), dt=c["dt"], t_max=0.3,
+                         rng_key=9 + k, set_scale=[1.0, 0.8, 0.8])
+
+    def by_source(k):
+        ions = B.ion_attributes(src(k), sigma_m2=2e-18)
+        return FlightRequest(ions=ions, tune=tune(k), source=src(k))
+
+    def by_rows(k):
+        ions = B.ion_attributes(src(k), sigma_m2=2e-18)
+        rows = B.generate_births(src(k), engine)
+        for j, key in enumerate(("x", "y", "z", "vx", "vy", "vz", "tob")):
+            ions[key] = rows[:, j].copy()
+        return FlightRequest(ions=ions, tune=tune(k))
+
+    ref = []
+    for k in range(3):
+        srv = FlightServer(engine, fp, fl, capacity=1400, window=250)
+        r = by_rows(k)
+        srv.submit(r)
+        srv.drain()
+        ref.append(r.result)
+        srv.release()
+    srv = FlightServer(engine, fp, fl, capacity=2800, window=190)
+    mix = [by_source(0), by_rows(1), by_source(2)]
+    for r in mix:
+        srv.submit(r)
+    srv.drain()
+    for r, a in zip(mix, ref):
+        for key in ("x", "y", "z", "vx", "vy", "vz", "t_us", "fate", "ncol",
+                    "steps"):
+            assert np.array_equal(r.result[key], a[key]), (r.rid, key)
+    srv.release()
+    fp.release()
