@@ -89,6 +89,10 @@ extern "C" {
 #define IG_FATE_PLANE 5
 #define IG_FATE_DETECT 6
 
+/* ig_results.face of an ion refused at birth (ig_ensemble_refuse_births):
+   fate IG_FATE_IMPACT inside metal, IG_FATE_BOX outside the domain */
+#define IG_FACE_BIRTH (-2)
+
 /* optional record channels, bit i = the i-th r-z channel of
    ion_gym/physics/build_rz.py:569-571 in that order */
 #define IG_REC_SPEED (1u << 0)
@@ -434,6 +438,22 @@ int ig_births(ig_engine *h, const ig_births_desc *d, double *out);
  * (ion_gym/physics/sim_build.py:722 build_run). */
 int ig_ensemble_births_at(ig_engine *h, ig_ensemble_id id, int64_t offset,
                           const ig_births_desc *d);
+
+/* The per-ion birth refusals of the fly functions, on the device, for the
+ * not yet flown ions of slots [offset, offset + n): a birth outside the
+ * solved domain (ion_gym/physics/birth_domain.py:27-38, nodes 0..n-1
+ * with a 1e-9 node tolerance) ends with fate IG_FATE_BOX, one inside
+ * metal with fate IG_FATE_IMPACT and its electrode label -- by the
+ * nearest-node rule on the half grid for RZ and PLANAR
+ * (ion_gym/physics/raster2d.py:624-646, r = hypot(y, z) for RZ,
+ * build_rz.py:679-690) and the all-eight-corners rule for XYZ
+ * (ion_gym/physics/tracer3d.py:192-230).  Refused ions keep their birth
+ * state, get face IG_FACE_BIRTH and are never flown, as the reference's
+ * birth_splat (ion_gym/physics/ion_envelope.py:184-223).  *n_refused
+ * (optional) receives how many were refused. */
+int ig_ensemble_refuse_births(ig_engine *h, ig_ensemble_id id,
+                              ig_fields_id f, int64_t offset, int64_t n,
+                              int64_t *n_refused);
 
 #ifdef __cplusplus
 }

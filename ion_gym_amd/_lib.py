@@ -18,6 +18,7 @@ PRECISIONS = {"f64": PREC_F64, "f32_store": PREC_F32_STORE, "f32": PREC_F32}
 WAVE_SIN, WAVE_COS, WAVE_SQUARE, WAVE_TAB_HOLD, WAVE_TAB_LINEAR = range(5)
 GAS_VACUUM, GAS_HS, GAS_SDS = 0, 1, 2
 FATE_FLYING = -2
+FACE_BIRTH = -2
 FATE_EMPTY = -4
 MAX_CHANNELS = 16
 MAX_PLANES = 16
@@ -166,6 +167,9 @@ SIGNATURES = {
     "ig_births": (C.c_int, [_VP, C.POINTER(BirthsDesc), _dp]),
     "ig_ensemble_births_at": (C.c_int, [_VP, C.c_int64, C.c_int64,
                                         C.POINTER(BirthsDesc)]),
+    "ig_ensemble_refuse_births": (C.c_int, [_VP, C.c_int64, C.c_int64,
+                                            C.c_int64, C.c_int64,
+                                            C.POINTER(C.c_int64)]),
 }
 
 

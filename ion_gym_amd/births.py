@@ -301,3 +301,18 @@ def births_into(engine, ensemble, offset, source, seed=None):
         raise RuntimeError(f"generate_births: {msg}")
     L.check(rc, "ig_ensemble_births_at")
     del keep
+
+
+def refuse_births(engine, fields, ensemble, offset=0, n=None):
+    """The fly functions' birth refusals on the device for the unflown
+    ions of slots [offset, offset + n) (ig_ensemble_refuse_births): outside
+    the domain -> fate 1, inside metal -> fate 0 with the electrode label;
+    such ions get face -2 (refused at birth) and are never flown.  Returns
+    how many were refused."""
+    n = ensemble.n - int(offset) if n is None else int(n)
+    out = C.c_int64()
+    L.check(L.lib().ig_ensemble_refuse_births(engine.h, ensemble.id,
+                                              fields.id, int(offset), n,
+                                              C.byref(out)),
+            "ig_ensemble_refuse_births")
+    return out.value

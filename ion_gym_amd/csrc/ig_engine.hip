@@ -16,6 +16,7 @@
 #include "ig_births.h"
 #include "ig_common.h"
 #include "ig_compose_kernel.h"
+#include "ig_metal.h"
 
 using namespace ig;
 
@@ -1327,6 +1328,77 @@ int ig_ensemble_births_at(ig_engine *h, ig_ensemble_id id, int64_t offset,
                       return o;
                     },
                     &nn);
+}
+
+// the birth refusals of the three routes (ig_ensemble_refuse_births)
+__global__ void refuse_births_kernel(const FieldsK f, const EnsK e, int64_t off, int64_t n,
+                                     unsigned long long *count) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = off + t;
+  if (e.fate[i] != IG_FATE_FLYING || e.steps[i] != 0) return;
+  const double tol = 1e-9; // birth_domain.DOMAIN_EDGE_TOL
+  const double x = e.x[i], y = e.y[i], z = e.z[i];
+  int fate = -1, label = 0;
+  auto inside = [&](double g, int64_t nn) { return -tol <= g && g <= (double)(nn - 1) + tol; };
+  if (f.route == IG_ROUTE_XYZ) {
+    const double inv_h = 1.0 / f.h;
+    const double gx = x * inv_h, gy = y * inv_h, gz = z * inv_h;
+    if (!inside(gx, f.n0) || !inside(gy, f.n1) || !inside(gz, f.n2))
+      fate = IG_FATE_BOX;
+    else if (in_metal_3d(f.metal, gx, gy, gz, f.n0, f.n1, f.n2))
+      fate = IG_FATE_IMPACT;
+  } else {
+    // r-z: the half grid (n0, nr) of the unfolded pack (n0, 2 nr - 1);
+    // planar: the pack's own (n0, n1)
+    const bool rz = f.route == IG_ROUTE_RZ;
+    const int64_t ncol = f.n1, nr = rz ? (f.n1 + 1) / 2 : f.n1, c0 = rz ? nr - 1 : 0;
+    const double g0 = x / f.h, g1 = (rz ? hypot(y, z) : y) / f.h;
+    if (!inside(g0, f.n0) || !inside(g1, nr)) {
+      fate = IG_FATE_BOX;
+    } else {
+      const double fi = floor(g0 + 0.5), fj = floor(g1 + 0.5);
+      if (fi >= 0.0 && fj >= 0.0 && fi <= (double)(f.n0 - 1) && fj <= (double)(nr - 1)) {
+        const int64_t node = (int64_t)fi * ncol + c0 + (int64_t)fj;
+        if (f.metal[node]) {
+          fate = IG_FATE_IMPACT;
+          label = f.label ? (int)f.label[node] : 0;
+        }
+      }
+    }
+  }
+  if (fate < 0) return;
+  e.fate[i] = fate;
+  e.face[i] = IG_FACE_BIRTH;
+  e.label[i] = label;
+  atomicAdd(count, 1ull);
+}
+
+int ig_ensemble_refuse_births(ig_engine *h, ig_ensemble_id id, ig_fields_id fid, int64_t offset,
+                              int64_t n, int64_t *n_refused) {
+  if (!h) return fail(IG_E_ARG, "ig_ensemble_refuse_births: null engine");
+  Ensemble *E = find_ens(h, id);
+  if (!E) return fail(IG_E_STATE, "ig_ensemble_refuse_births: unknown ensemble id");
+  auto fit = h->fields.find(fid);
+  if (fit == h->fields.end())
+    return fail(IG_E_STATE, "ig_ensemble_refuse_births: unknown field id");
+  const FieldsK &fk = fit->second->k;
+  if (offset < 0 || n < 0 || offset + n > E->n)
+    return fail(IG_E_ARG, "ig_ensemble_refuse_births: slots [" + std::to_string(offset) + ", " +
+                              std::to_string(offset + n) + ") are not all loaded");
+  if (int rc = set_dev(h)) return rc;
+  if (int rc = scratch(h->bflag, 16)) return rc;
+  unsigned long long *cnt = (unsigned long long *)h->bflag.p;
+  HIPX(hipMemsetAsync(cnt, 0, 8, h->stream));
+  if (n > 0)
+    hipLaunchKernelGGL(refuse_births_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       h->stream, fk, E->k, offset, n, cnt);
+  HIPX(hipGetLastError());
+  unsigned long long c = 0;
+  HIPX(hipMemcpyAsync(&c, cnt, 8, hipMemcpyDeviceToHost, h->stream));
+  HIPX(hipStreamSynchronize(h->stream));
+  if (n_refused) *n_refused = (int64_t)c;
+  return 0;
 }
 
 int ig_last_fly_ms(ig_engine *h, double *out) {

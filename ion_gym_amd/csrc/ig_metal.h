@@ -1,0 +1,30 @@
+// The metal tests of the flight kernels, shared with the birth refusals.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ig {
+
+// raster2d._metal_nn (raster2d.py:597-612)
+__device__ __forceinline__ bool metal_nn(const uint8_t *__restrict__ m, double gx, double gy,
+                                         int64_t nx, int64_t ny) {
+  double fi = floor(gx + 0.5), fj = floor(gy + 0.5);
+  if (!(fi >= 0.0) || !(fj >= 0.0) || fi > (double)(nx - 1) || fj > (double)(ny - 1)) return false;
+  return m[(int64_t)fi * ny + (int64_t)fj] != 0;
+}
+
+// tracer3d._in_metal_3d (tracer3d.py:174-189)
+__device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, double x, double y,
+                                            double z, int64_t nx, int64_t ny, int64_t nz) {
+  if (!(x >= 0.0) || !(y >= 0.0) || !(z >= 0.0) || x > nx - 1 || y > ny - 1 || z > nz - 1)
+    return false;
+  int64_t i = (int64_t)x, j = (int64_t)y, k = (int64_t)z;
+  if (i > nx - 2) i = nx - 2;
+  if (j > ny - 2) j = ny - 2;
+  if (k > nz - 2) k = nz - 2;
+  const uint8_t *a = m + (i * ny + j) * nz + k;
+  const int64_t di = ny * nz, dj = nz;
+  return a[0] && a[di] && a[dj] && a[di + dj] && a[1] && a[di + 1] && a[dj + 1] && a[di + dj + 1];
+}
+
+} // namespace ig

@@ -20,6 +20,7 @@
 #pragma once
 #pragma clang fp contract(off)
 #include "ig_device.h"
+#include "ig_metal.h"
 #include <type_traits>
 
 namespace ig {
@@ -131,14 +132,6 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
       }
     }
   }
-}
-
-// raster2d._metal_nn (raster2d.py:597-612)
-__device__ __forceinline__ bool metal_nn(const uint8_t *__restrict__ m, double gx, double gy,
-                                         int64_t nx, int64_t ny) {
-  double fi = floor(gx + 0.5), fj = floor(gy + 0.5);
-  if (!(fi >= 0.0) || !(fj >= 0.0) || fi > (double)(nx - 1) || fj > (double)(ny - 1)) return false;
-  return m[(int64_t)fi * ny + (int64_t)fj] != 0;
 }
 
 // tracer_rz._write_row (tracer_rz.py:40-111)

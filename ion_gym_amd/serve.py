@@ -56,6 +56,7 @@ class FlightRequest:
     t_done: float = 0.0
     result: Optional[dict] = None
     steps: int = 0
+    refused: int = 0
 
     @property
     def n(self):
@@ -109,8 +110,12 @@ class FlightServer:
     """Serve flight requests on one engine and one resident field pack."""
 
     def __init__(self, engine, fields, flight, *, capacity,
-                 max_requests=256, window=2000, hbm_budget=None):
+                 max_requests=256, window=2000, hbm_budget=None,
+                 refuse=True):
         self.engine = engine
+        # the fly functions' birth refusals at admission (an ion born in
+        # metal or outside the domain is never flown; face -2)
+        self.refuse = bool(refuse)
         self.fields = fields
         self.flight = flight
         self.flight.max_steps = int(window)
@@ -208,6 +213,10 @@ class FlightServer:
             if req.source is not None and "x" not in req.ions:
                 from .births import births_into
                 births_into(self.engine, self.slab, off, req.source)
+            if self.refuse:
+                from .births import refuse_births
+                req.refused = refuse_births(self.engine, self.fields,
+                                            self.slab, off, req.n)
         if dirty:
             self._set_tunes()
 

@@ -149,3 +149,52 @@ def zig_normals(words, n, k, w, f):
                 math.exp(-0.5 * x * x):
             out.append(x)
     return np.array(out), p
+
+
+DOMAIN_EDGE_TOL = 1e-9
+
+
+def refuse_2d(ele, h, a_mm, b_mm):
+    """(fate, label) per birth of a 2-D route (r-z: a = x, b = hypot(y,
+    z) on the half grid; planar: a = x, b = y): outside nodes 0..n-1 on
+    either axis, 1e-9 node tolerance (birth_domain.py:27-38) -> 1;
+    else nearest-node metal (raster2d.py:624-646) -> 0 with the label;
+    else -1 (flies)."""
+    ele = np.asarray(ele)
+    n0, n1 = ele.shape
+    fate = np.full(len(a_mm), -1, np.int32)
+    label = np.zeros(len(a_mm), np.int32)
+    for q in range(len(a_mm)):
+        g0, g1 = a_mm[q] / h, b_mm[q] / h
+        if not (-DOMAIN_EDGE_TOL <= g0 <= n0 - 1 + DOMAIN_EDGE_TOL
+                and -DOMAIN_EDGE_TOL <= g1 <= n1 - 1 + DOMAIN_EDGE_TOL):
+            fate[q] = 1
+            continue
+        gi, gj = int(math.floor(g0 + 0.5)), int(math.floor(g1 + 0.5))
+        if 0 <= gi < n0 and 0 <= gj < n1 and ele[gi, gj] > 0:
+            fate[q] = 0
+            label[q] = int(ele[gi, gj])
+    return fate, label
+
+
+def refuse_3d(metal, h, x, y, z):
+    """fate per 3-D birth (tracer3d.py:192-230): outside the domain in
+    node units x / h with the 1e-9 tolerance -> 1; inside metal by the
+    all-eight-corners rule of _in_metal_3d (tracer3d.py:174-189: outside
+    the box is not metal, the cell index clamped to n - 2) -> 0; else -1."""
+    m = np.asarray(metal).astype(bool)
+    nx, ny, nz = m.shape
+    inv_h = 1.0 / float(h)
+    fate = np.full(len(x), -1, np.int32)
+    for q in range(len(x)):
+        g = (x[q] * inv_h, y[q] * inv_h, z[q] * inv_h)
+        if not all(-DOMAIN_EDGE_TOL <= gv <= n - 1 + DOMAIN_EDGE_TOL
+                   for gv, n in zip(g, (nx, ny, nz))):
+            fate[q] = 1
+            continue
+        if any(gv < 0.0 or gv > n - 1 for gv, n in zip(g, (nx, ny, nz))):
+            continue
+        i, j, k = (min(int(gv), n - 2) for gv, n in zip(g, (nx, ny, nz)))
+        if m[i:i + 2, j:j + 2, k:k + 2].all():
+            fate[q] = 0
+    return fate

@@ -180,3 +180,27 @@ def test_births_file_rows(tmp_path):
     with pytest.raises(ValueError, match="missing column"):
         B.read_births_file(SourceSpec(distribution="file",
                                       births_file=str(q), n_ions=1))
+
+
+def test_oracle_refusals_on_hand_made_grids():
+    ele = np.zeros((5, 4), np.int32)
+    ele[2, 1] = 7
+    h = 0.5
+    # node 2 spans [1.75, 2.25) in mm on axis 0 by the nearest-node rule
+    a = np.array([1.0, 1.0, 1.126, 0.0, 2.0 + 5e-10, 2.0 + 1e-8, -1e-10,
+                  -1e-8, 0.3])
+    b = np.array([0.5, 0.74, 0.5, 0.0, 0.0, 0.0, 0.0, 0.0, 0.5])
+    fate, label = OB.refuse_2d(ele, h, a, b)
+    # nodes: (2, 1) metal, (2, 1.48) -> (2, 1) metal, (2.252, 1) -> (2, 1),
+    # origin, the last node + 1e-9 tolerance in and out, below zero in
+    # and out, (0.6, 1) vacuum
+    assert list(fate) == [0, 0, 0, -1, -1, 1, -1, 1, -1]
+    assert list(label[:3]) == [7, 7, 7]
+    m = np.zeros((3, 3, 3), bool)
+    m[1:, 1:, 1:] = True
+    x = np.array([1.5, 1.5, 0.5, 2.0, 2.0 + 1e-8, 1.0])
+    y = np.array([1.5, 1.5, 1.5, 2.0, 2.0, 1.0])
+    z = np.array([1.5, 0.5, 1.5, 2.0, 2.0, 1.0])
+    # a fully metal cell; two cells with metal on one face only; the far
+    # corner clamps into the metal cell; past the box; the shared corner
+    assert list(OB.refuse_3d(m, 1.0, x, y, z)) == [0, -1, -1, 0, 1, 0]

@@ -144,3 +144,102 @@ def test_births_into_ensemble_slots(engine):
     ens.release()
     ref.release()
     fp.release()
+
+
+def _load_rows(engine, rows, ions_like):
+    ens = engine.ensemble(len(rows))
+    ions = {k: np.asarray(v)[:len(rows)] for k, v in ions_like.items()
+            if k not in ("x", "y", "z", "vx", "vy", "vz", "tob")}
+    for j, k in enumerate(("x", "y", "z", "vx", "vy", "vz", "tob")):
+        ions[k] = rows[:, j].copy()
+    ens.load(ions)
+    return ens
+
+
+def test_birth_refusals_on_the_device(engine):
+    """Births scattered over and beyond the funnel (r-z), the planar
+    quadrupole and the 3-D quadrupole: the device refuses exactly the ions
+    the oracle's restatement of the fly functions' checks refuses (outside
+    -> fate 1, metal -> fate 0 with the r-z electrode label, face -2), and
+    a refused ion is never flown."""
+    from ion_gym_amd import births as B
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    rng = np.random.default_rng(21)
+    n = 4000
+    # r-z funnel: x over [-2, L + 2] mm, r over [0, R + 2] mm
+    c = configs.funnel_rz(n_ions=n, seed=3, coarse=2)
+    m = c["model"]
+    h = m.mm_per_gu
+    nz, nr = m.ele.shape
+    x = rng.uniform(-2.0, (nz - 1) * h + 2.0, n)
+    r = rng.uniform(0.0, (nr - 1) * h + 2.0, n)
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    rows = np.zeros((n, 7))
+    rows[:, 0], rows[:, 1], rows[:, 2] = x, r * np.cos(th), r * np.sin(th)
+    rows[:, 3] = 1.0
+    fp = engine.upload_rz(m.pack(), "f64")
+    ens = _load_rows(engine, rows, c["ions"])
+    k = B.refuse_births(engine, fp, ens)
+    got = ens.read(records=False)
+    fate, label = OB.refuse_2d(m.ele, h, x, np.hypot(rows[:, 1], rows[:, 2]))
+    ref = fate >= 0
+    assert k == ref.sum() and (fate == 0).sum() > 50 and (fate == 1).sum() > 50
+    assert np.array_equal(got["fate"][ref], fate[ref])
+    assert np.array_equal(got["label"][ref], label[ref])
+    assert np.all(got["face"][ref] == -2)
+    assert np.all(got["fate"][~ref] == -2)
+    fl = Flight(c["dt"], c["t_max"], gas=c["gas"], max_steps=50)
+    engine.fly(fp, fl, ens)
+    after = ens.read(records=False)
+    assert np.all(after["steps"][ref] == 0) and np.all(after["steps"][~ref] > 0)
+    assert np.array_equal(after["x"][ref], x[ref])
+    ens.release()
+    fp.release()
+    # planar quadrupole: (x, y) over and beyond the box
+    q = configs.planar_quad(n_ions=10, coarse=2)
+    q["pack"] = dict(q["pack"])
+    ele = np.array(q["pack"]["ele"], float)
+    ele[20:40, 30:50] = 1.0       # a metal block beside the rods
+    q["pack"]["ele"] = ele
+    hq = q["pack"]["mm"]
+    nx, ny = ele.shape
+    rows = np.zeros((n, 7))
+    rows[:, 0] = rng.uniform(-1.0, (nx - 1) * hq + 1.0, n)
+    rows[:, 1] = rng.uniform(-1.0, (ny - 1) * hq + 1.0, n)
+    fp = engine.upload_planar(q["pack"], "f64")
+    ens = _load_rows(engine, rows, {k: np.resize(v, n) for k, v in
+                                    q["ions"].items()})
+    B.refuse_births(engine, fp, ens)
+    fate, _ = OB.refuse_2d(ele, hq, rows[:, 0], rows[:, 1])
+    got = ens.read(records=False)
+    want = np.where(fate >= 0, fate, -2)
+    assert np.array_equal(got["fate"], want) and (fate == 0).sum() > 50
+    ens.release()
+    fp.release()
+    # 3-D quadrupole
+    c3 = configs.quad_3d(n_ions=10, coarse=4, n_cycles=2)
+    pk = dict(c3["pack"])
+    mt = np.array(pk["ele"], bool)
+    mt[5:15, 8:20, 30:60] = True    # a metal block: whole cells of metal
+    mt[25, 25, 25] = True           # and a lone node (no whole cell)
+    pk["ele"] = mt
+    h3 = pk["h_mm"]
+    ext = [(s - 1) * h3 for s in mt.shape]
+    rows = np.zeros((n, 7))
+    for a in range(3):
+        rows[:, a] = rng.uniform(-0.5, ext[a] + 0.5, n)
+    # half of them within a node of a metal node, where the all-eight rule
+    # decides
+    near = np.argwhere(mt)[rng.integers(0, int(mt.sum()), n // 2)]
+    rows[:n // 2, :3] = (near + rng.uniform(-1.0, 1.0, (n // 2, 3))) * h3
+    fp = engine.upload_xyz(pk, "f64")
+    ens = _load_rows(engine, rows, {k: np.resize(v, n) for k, v in
+                                    c3["ions"].items()})
+    B.refuse_births(engine, fp, ens)
+    fate = OB.refuse_3d(mt, h3, rows[:, 0], rows[:, 1], rows[:, 2])
+    got = ens.read(records=False)
+    assert np.array_equal(got["fate"], np.where(fate >= 0, fate, -2))
+    assert (fate == 0).sum() > 50 and (fate == 1).sum() > 50
+    ens.release()
+    fp.release()
