@@ -9,6 +9,7 @@
  *               kick-drift-kick leapfrog, bilinear field on the
  *               mirror-unfolded (axial, radial) half plane, 3-D ion state.
  *   route XYZ : ion_gym/physics/tracer3d.py:280-575   (_fly3d)
+ *               and, with SDS gas, tracer3d.py:851-1025 (_fly3d_sds)
  *               classic RK4, trilinear field, 3-D state.
  *
  * The reference has no FFI: its boundary is the Python call
@@ -167,8 +168,10 @@ typedef struct {
  * scalar/array arguments of _fly_rec_full (ion_gym/physics/tracer_rz.py:
  * 114-123) and _fly3d (ion_gym/physics/tracer3d.py:281-290). */
 typedef struct {
-  double dt;           /* step in the route's clock: RZ us, XYZ s (the
-                          reference's dt_ns*1e-3 and dt_ns*1e-9) */
+  double dt;           /* step in the route's clock: RZ and PLANAR us,
+                          XYZ s (the reference's dt_ns*1e-3 and
+                          dt_ns*1e-9); XYZ with IG_GAS_SDS us, the clock
+                          of _fly3d_sds (tracer3d.py:1117: dt_ns*1e-3) */
   double t_max;        /* flight time limit, same unit, while t < t_max */
   int64_t max_steps;   /* steps this call may advance each ion; <= 0 =
                           until fate.  Ions still flying keep fate

@@ -424,6 +424,67 @@ def drift_batch_rz(n_ens=64, seed=4000, coarse=1, size_lo=1e3, size_hi=2e5,
                 ensembles=ens)
 
 
+def drift_3d(n_ions=4000, seed=6000, coarse=1, e_vpmm=5.0, mz=500.0, K0=1.5,
+             P_torr=4.0, T_k=300.0, dt_frac=0.1, rf_vpmm=0.0, rf_mhz=1.0,
+             t_max_us=2000.0):
+    """A 3-D SDS drift cell (the _fly3d_sds route): 41 x 41 x 301 nodes @
+    0.1 mm (4 x 4 x 30 mm); a uniform static field e_vpmm along z (V/mm);
+    optionally one sine drive channel of rf_vpmm along x at rf_mhz; a
+    metal post (x, y in 2.6-3.4 mm, z in 10-12 mm) and a detector station
+    plane at z = 25 mm (x, y in 0.5-3.5 mm); N2 at 4 Torr, 300 K; the ion's
+    declared K0 = 1.5 cm^2/(V s); births near (2, 2, 1) mm with 300 K
+    velocities and the ion's SDS numbers; dt = dt_frac / gamma in us.
+    ``coarse`` > 1 divides the grid resolution (tests)."""
+    from .physics import sds as sds_mod
+    h = 0.1 * coarse
+    nx = ny = 40 // coarse + 1
+    nz = 300 // coarse + 1
+    shape = (nx, ny, nz)
+    EAx, EAy = np.zeros(shape), np.zeros(shape)
+    EAz = np.full(shape, float(e_vpmm))
+    X, Y, Z = np.meshgrid(np.arange(nx) * h, np.arange(ny) * h,
+                          np.arange(nz) * h, indexing="ij")
+    ele = ((np.abs(X - 3.0) <= 0.4) & (np.abs(Y - 3.0) <= 0.4)
+           & (Z >= 10.0) & (Z <= 12.0))
+    K = 1 if rf_vpmm else 0
+    pack = dict(EAx=EAx, EAy=EAy, EAz=EAz, ele=ele, h_mm=h,
+                ch_kind=np.zeros(K, np.int64),
+                ch_om=np.full(K, 2 * math.pi * rf_mhz),
+                ch_ph=np.zeros(K), ch_amp=np.ones(K), ch_off=np.zeros(K),
+                ch_duty=np.full(K, 0.5))
+    if K:
+        pack.update(ExK=np.full((1,) + shape, float(rf_vpmm)),
+                    EyK=np.zeros((1,) + shape), EzK=np.zeros((1,) + shape))
+    else:
+        pack.update(ExK=np.zeros((0,) + shape), EyK=np.zeros((0,) + shape),
+                    EzK=np.zeros((0,) + shape))
+    g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
+    stats = sds_mod.default_statistics()
+    p = sds_mod.transport_params(mz, 1, g["m_gas"], T_k, P_torr, K0,
+                                 stats=stats)
+    dt = dt_frac / p["damping"]
+    rng = np.random.default_rng(seed)
+    m = np.full(n_ions, mz)
+    s1 = np.sqrt(K_B * T_k / (m * KG_AMU)) * 1e-3
+    v = rng.normal(0.0, 1.0, (3, n_ions)) * s1
+    ions = dict(x=2.0 + rng.uniform(-0.3, 0.3, n_ions),
+                y=2.0 + rng.uniform(-0.3, 0.3, n_ions),
+                z=np.full(n_ions, 1.0), vx=v[0], vy=v[1], vz=v[2],
+                tob=rng.uniform(0.0, 0.5, n_ions), m_amu=m,
+                acc=E_CHG / (m * KG_AMU), sigma_m2=np.zeros(n_ions),
+                seed=seed + np.arange(n_ions, dtype=np.int64),
+                sds=np.tile([p["damping"], p["mfp_mm"], p["V_mm_us"],
+                             p["log_mr_ratio"]], (n_ions, 1)))
+    gas = dict(model=2, T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
+               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
+               flow=(0.0, 0.0, 0.0), sds_stats=stats)
+    planes = [(2, 25.0, 1.0, (0.5, 3.5, 0.5, 3.5), 6)]
+    return dict(name="drift_3d", route="xyz", pack=pack, ions=ions, dt=dt,
+                t_max=float(t_max_us), gas=gas, planes=planes,
+                K_mm2_per_V_us=p["K_mm2_per_V_us"], D=p["D_mm2_per_us"],
+                transport=p)
+
+
 def hs_drift_rz(n_ions=2000, seed=5000, coarse=2, e_vpmm=1.0, mz=500.0,
                 gas="He", P_torr=1.0, T_k=300.0, sigma_m2=2.0e-18,
                 x_birth=76.0, dt_ns=None):

@@ -130,7 +130,10 @@ constexpr size_t kMaxGraphs = 16;
 
 struct Ensemble {
   int64_t cap = 0, n = 0;
-  int32_t route = IG_ROUTE_RZ; // clock of k.t: r-z us, 3-D s
+  int32_t route = IG_ROUTE_RZ;
+  // units of the state: -1 not flown yet; 0 t in us and v in mm/us (r-z,
+  // planar, 3-D SDS); 1 t in s and v in m/s once flown (3-D RK4)
+  int32_t clock = -1;
   int32_t rec_rows = 0, rec_cols = 0;
   int32_t n_tunes = 0;
   DevBuf tunes;         // Tune table (device)
@@ -533,6 +536,7 @@ int ig_ensemble_load(ig_engine *h, ig_ensemble_id id, const ig_ions *io) {
   if (!E) return fail(IG_E_STATE, "ig_ensemble_load: unknown ensemble id");
   int rc = load_range(h, E, 0, io, 0, "ig_ensemble_load");
   if (rc) return rc;
+  E->clock = -1; // every ion replaced: not flown yet
   E->n = io->n;
   E->k.n = io->n;
   return 0;
@@ -571,7 +575,8 @@ static int read_range(ig_engine *h, Ensemble *E, int64_t off, int64_t n, ig_resu
             {o->steps, k.steps + off, b8}, {o->nrec, k.nrec + off, b4}};
   for (auto &c : cp)
     if (c.dst) HIPX(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, st));
-  if (E->route == IG_ROUTE_XYZ && (o->vx || o->vy || o->vz)) {
+  const bool si = E->route == IG_ROUTE_XYZ && E->clock == 1;
+  if (si && (o->vx || o->vy || o->vz)) {
     // the 3-D kernel carries m/s from its first entry on (nrec >= 1);
     // report mm/us as the reference does (tracer3d.py:779: v * 1e-3)
     std::vector<int32_t> nr((size_t)n);
@@ -590,7 +595,7 @@ static int read_range(ig_engine *h, Ensemble *E, int64_t off, int64_t n, ig_resu
     HIPX(hipMemcpyAsync(t.data(), k.t + off, b8, hipMemcpyDeviceToHost, st));
     HIPX(hipMemcpyAsync(tob.data(), k.tob + off, b8, hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
-    if (E->route == IG_ROUTE_XYZ)
+    if (si)
       for (int64_t i = 0; i < n; ++i) o->t_us[i] = tob[i] + t[i] * 1e6;
     else
       for (int64_t i = 0; i < n; ++i) o->t_us[i] = tob[i] + t[i];
@@ -910,7 +915,13 @@ static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params
     k.sds_stats = (const double *)h->sds.p;
   }
   fk = &fit->second->k;
+  const int32_t clk = (fk->route == IG_ROUTE_XYZ && k.gas != IG_GAS_SDS) ? 1 : 0;
+  if (E->clock >= 0 && (E->clock != clk || E->route != fk->route))
+    return fail(IG_E_ARG, w + ": the ensemble was flown on another route or clock (3-D RK4 "
+                              "carries s and m/s, every other flight us and mm/us); load it "
+                              "again");
   E->route = fk->route;
+  E->clock = clk;
   if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec);
   return 0;
 }

@@ -9,7 +9,8 @@
 // The two mid-step RK4 evaluations share one lab time
 // (tracer3d.py:403, 408), so their waveform weights are computed once.
 // The periodic transporter (tracer3d.py:434-458) is not part of this
-// kernel: the host refuses it.
+// kernel: the host refuses it.  The 3-D SDS flight (_fly3d_sds) is
+// xyz_sds_kernel below.
 #pragma once
 #pragma clang fp contract(off)
 #include "ig_device.h"
@@ -520,6 +521,232 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   if constexpr (kFlat) e.hz[i] = hz;
 }
 
+// tracer3d._fly3d_sds (tracer3d.py:851-1025) with _sds_field
+// (tracer3d.py:792-812): the 3-D SDS flight in mm/us.  Per step: E at
+// (x, tob + t); a_f = qm 1e-6 E; the single damped kick v += factor (a_f -
+// (v - v_gas) gamma) dt with factor = (1 - e^{-gamma dt}) / (gamma dt)
+// (plain a_f without damping); x += v dt; the ICDF diffusion jump when the
+// ion's mean free path is positive; t += dt; then metal (all eight corners,
+// no bisection), the box, and the station/bound planes crossed over the
+// whole step (advection plus jump), each ending the flight with a record;
+// else the record every rec_every steps.  Record rows are (tob + t, x, y, z,
+// vx, vy, vz, Ex, Ey, Ez) with the field at the recorded point and time.
+// The flight clock is microseconds (the reference's dt_ns * 1e-3) and the
+// velocities stay in mm/us.  The FP32-basis mode reads its field and draws
+// its jump as the other kernels do in that mode.
+template <int PREC, int NS>
+__global__ __launch_bounds__(256) void xyz_sds_kernel(const FieldsK f, const FlightK p,
+                                                      const EnsK e) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i;
+  if (e.idx) {
+    if (tid >= (int64_t)*e.n_live) return;
+    i = e.idx[tid];
+  } else {
+    if (tid >= e.n) return;
+    i = tid;
+  }
+  if (e.fate[i] != IG_FATE_FLYING) return;
+  const int64_t steps0 = e.steps[i];
+  double x = e.x[i], y = e.y[i], z = e.z[i];
+  double vx = e.vx[i], vy = e.vy[i], vz = e.vz[i];
+  double t = e.t[i];
+  const double tob = e.tob[i], qm = e.acc[i];
+  const double h = f.h, inv_h = f.inv_h;
+  const Waves *W = f.waves;
+  double dt = p.dt, t_max = p.t_max;
+  uint64_t rkey = p.rng_key;
+  const double *tscale = nullptr;
+  if (e.tunes) {
+    const Tune *T = e.tunes + e.tune[i];
+    W = &T->w;
+    dt = T->dt;
+    t_max = T->t_max;
+    rkey = T->rng_key;
+    tscale = T->scale;
+  }
+  const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
+  const int K = f.K;
+  const bool rec = p.rec_rows > 0 && e.rec != nullptr;
+  int32_t nrec = e.nrec[i];
+  Rng rng;
+  {
+    uint64_t key = (uint64_t)e.seed[i] ^ rkey;
+    rng.k0 = (uint32_t)key;
+    rng.k1 = (uint32_t)(key >> 32);
+    rng.n = e.rng_n[i];
+    rng.has_gauss = e.has_gauss[i];
+    rng.gauss = e.gauss[i];
+    rng.cached = false;
+  }
+  const double damp = e.sds_damp[i], mfp = e.sds_mfp[i], V = e.sds_V[i], logmr = e.sds_logmr[i];
+  double factor = 1.0;
+  if (damp > 0.0) {
+    const double tterm = damp * dt;
+    factor = (1.0 - exp(-tterm)) / tterm;
+  }
+  constexpr int NW = NS > 1 ? NS - 1 : 1;
+  double w[NW];
+  float wt[NS];
+  double wtt = __builtin_nan("");
+  auto efield = [&](double px, double py, double pz, double tt, double &a, double &b,
+                    double &c) {
+    if (!(tt == wtt)) {
+      if constexpr (PREC == IG_PREC_F32) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          double v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+          v = v * f.scale[q];
+          if (tscale) v = v * tscale[q];
+          wt[q] = (float)v;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+          if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
+      }
+      wtt = tt;
+    }
+    if constexpr (PREC == IG_PREC_F32)
+      xyz_field_f32<NS>(f, px * inv_h, py * inv_h, pz * inv_h, wt, a, b, c);
+    else
+      xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
+  };
+  const int R = rec ? p.rec_rows : 0;
+  auto write = [&](int row) {
+    double *r = e.rec + ((size_t)i * p.rec_rows + row) * p.rec_cols;
+    const double tus = tob + t;
+    r[0] = tus;
+    r[1] = x;
+    r[2] = y;
+    r[3] = z;
+    r[4] = vx;
+    r[5] = vy;
+    r[6] = vz;
+    double a, b, c;
+    efield(x, y, z, tus, a, b, c);
+    r[7] = a;
+    r[8] = b;
+    r[9] = c;
+  };
+  if (nrec == 0) {
+    if (rec) write(0);
+    nrec = 1;
+  }
+  int fate = IG_FATE_FLYING, hit_ip = -1;
+  int64_t s = 0;
+  const int64_t lim = p.max_steps;
+  for (;;) {
+    if (!(t < t_max)) {
+      fate = IG_FATE_TIMEOUT;
+      break;
+    }
+    if (lim > 0 && s >= lim) break;
+    double ex, ey, ez;
+    efield(x, y, z, tob + t, ex, ey, ez);
+    const double afx = qm * 1e-6 * ex, afy = qm * 1e-6 * ey, afz = qm * 1e-6 * ez;
+    double aex = afx, aey = afy, aez = afz;
+    if (damp > 0.0) {
+      aex = factor * (afx - (vx - p.vg[0]) * damp);
+      aey = factor * (afy - (vy - p.vg[1]) * damp);
+      aez = factor * (afz - (vz - p.vg[2]) * damp);
+    }
+    vx = vx + aex * dt;
+    vy = vy + aey * dt;
+    vz = vz + aez * dt;
+    const double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
+    x = x + vx * dt;
+    y = y + vy * dt;
+    z = z + vz * dt;
+    if (mfp > 0.0) {
+      double jx, jy, jz;
+      if constexpr (PREC == IG_PREC_F32)
+        sds_jump_fast(rng, p.sds_stats, logmr, V, mfp, dt, jx, jy, jz);
+      else
+        sds_jump(rng, p.sds_stats, logmr, V, mfp, dt, jx, jy, jz);
+      x = x + jx;
+      y = y + jy;
+      z = z + jz;
+    }
+    t += dt;
+    s += 1;
+    if (in_metal_3d(f.metal, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
+      fate = IG_FATE_IMPACT;
+      if (nrec < R) write(nrec);
+      if (rec && nrec < R) nrec++;
+      break;
+    }
+    if (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h || y > (ny - 1) * h ||
+        z > (nz - 1) * h) {
+      fate = IG_FATE_BOX;
+      if (nrec < R) write(nrec);
+      if (rec && nrec < R) nrec++;
+      break;
+    }
+    bool hit = false;
+    for (int ip = 0; ip < p.n_pl; ++ip) {
+      const int col = p.pl_col[ip];
+      const double cn = col == 0 ? x : (col == 1 ? y : z);
+      const double co = col == 0 ? xo : (col == 1 ? yo : zo);
+      const double sgn = p.pl_sgn[ip], val = p.pl_val[ip];
+      bool crossed;
+      if (sgn == 0.0)
+        crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
+      else
+        crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
+      if (!crossed) continue;
+      const double den = cn - co;
+      double fr = den == 0.0 ? 0.0 : (val - co) / den;
+      if (fr < 0.0) fr = 0.0;
+      if (fr > 1.0) fr = 1.0;
+      const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
+      const double t1 = col == 0 ? yc : xc;
+      const double t2 = col == 2 ? yc : zc;
+      const bool ins = p.pl_w[ip][0] <= t1 && t1 <= p.pl_w[ip][1] && p.pl_w[ip][2] <= t2 &&
+                       t2 <= p.pl_w[ip][3];
+      if (p.pl_kind[ip] == IG_FATE_DETECT) {
+        if (!ins) continue;
+      } else if (p.pl_kind[ip] == IG_FATE_PLANE && ins) {
+        continue;
+      }
+      x = xc;
+      y = yc;
+      z = zc;
+      vx = vxo + fr * (vx - vxo);
+      vy = vyo + fr * (vy - vyo);
+      vz = vzo + fr * (vz - vzo);
+      t = to + fr * (t - to);
+      fate = p.pl_kind[ip];
+      hit_ip = ip;
+      hit = true;
+      break;
+    }
+    if (hit) {
+      if (nrec < R) write(nrec);
+      if (rec && nrec < R) nrec++;
+      break;
+    }
+    if ((steps0 + s) % p.rec_every == 0 && nrec < R) {
+      write(nrec);
+      nrec++;
+    }
+  }
+  e.x[i] = x;
+  e.y[i] = y;
+  e.z[i] = z;
+  e.vx[i] = vx;
+  e.vy[i] = vy;
+  e.vz[i] = vz;
+  e.t[i] = t;
+  e.fate[i] = fate;
+  e.face[i] = hit_ip;
+  e.steps[i] = steps0 + s;
+  e.nrec[i] = nrec;
+  e.rng_n[i] = rng.n;
+  e.has_gauss[i] = rng.has_gauss;
+  e.gauss[i] = rng.gauss;
+}
+
 template <int PREC, int NS>
 static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
                          int block, hipStream_t st) {
@@ -538,7 +765,7 @@ static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int6
     hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 1>), grid, dim3(block), 0, st, f,
                        p, e);
   } else {
-    return -2; // 3-D SDS is its own kernel (tracer3d._fly3d_sds), not this one
+    hipLaunchKernelGGL((xyz_sds_kernel<PREC, NS>), grid, dim3(block), 0, st, f, p, e);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -317,11 +317,170 @@ static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *
   out->nrec[ii] = nrec;
 }
 
+/* tracer3d._fly3d_sds (tracer3d.py:851-1025): mm, us, mm/us throughout;
+ * dt and t_max in us; one field evaluation per step; the damped kick,
+ * the drift, the diffusion jump when mfp > 0, then metal (no bisection),
+ * box and planes crossed over the whole step; records (t_us, x, y, z,
+ * vx, vy, vz, Ex, Ey, Ez) with the field at the record point and time
+ * (_sds_field, tracer3d.py:792-812). */
+static void rec_row_sds(double *row, double t_us, double x, double y, double z, double vx,
+                        double vy, double vz, const or_xyz_fields *f) {
+  row[0] = t_us;
+  row[1] = x;
+  row[2] = y;
+  row[3] = z;
+  row[4] = vx;
+  row[5] = vy;
+  row[6] = vz;
+  efield3(f, x, y, z, t_us, row + 7, row + 8, row + 9);
+}
+
+static void fly_one3_sds(const or_xyz_fields *f, const or_params *p, const or_ions *io,
+                         or_out *out, int64_t ii) {
+  const double qm = io->acc[ii];
+  double x = io->x[ii], y = io->y[ii], z = io->z[ii];
+  double vx = io->vx[ii], vy = io->vy[ii], vz = io->vz[ii];
+  const double tob = io->tob[ii], dt = p->dt, t_max = p->t_max;
+  const double damping = io->sds ? io->sds[4 * ii] : 0.0;
+  const double mfp = io->sds ? io->sds[4 * ii + 1] : 0.0;
+  const double V = io->sds ? io->sds[4 * ii + 2] : 0.0;
+  const double logmr = io->sds ? io->sds[4 * ii + 3] : 0.0;
+  const int64_t nx = f->nx, ny = f->ny, nz = f->nz;
+  const double h = f->h_mm, inv_h = 1.0 / h;
+  or_rng rng;
+  if (p->rng == OR_RNG_MT)
+    or_mt_seed(&rng, (uint32_t)io->seed[ii]);
+  else
+    or_philox_seed(&rng, (uint64_t)io->seed[ii], p->rng_key);
+  double *rec = p->rec_rows > 0 && out->rec ? out->rec + ii * p->rec_rows * p->rec_cols : 0;
+  const int64_t R = p->rec_rows, C = p->rec_cols;
+  double t = 0.0;
+  if (rec) rec_row_sds(rec, tob, x, y, z, vx, vy, vz, f);
+  int64_t nrec = 1, step = 0;
+  int kind = 2, hit_ip = -1;
+  const int64_t limit = p->max_steps;
+  while (t < t_max) {
+    if (limit > 0 && step >= limit) {
+      kind = -2;
+      break;
+    }
+    double ex, ey, ez;
+    efield3(f, x, y, z, tob + t, &ex, &ey, &ez);
+    const double afx = qm * 1e-6 * ex, afy = qm * 1e-6 * ey, afz = qm * 1e-6 * ez;
+    double aex, aey, aez;
+    if (damping > 0.0) {
+      const double tterm = damping * dt;
+      const double factor = (1.0 - exp(-tterm)) / tterm;
+      aex = factor * (afx - (vx - p->vg[0]) * damping);
+      aey = factor * (afy - (vy - p->vg[1]) * damping);
+      aez = factor * (afz - (vz - p->vg[2]) * damping);
+    } else {
+      aex = afx;
+      aey = afy;
+      aez = afz;
+    }
+    vx = vx + aex * dt;
+    vy = vy + aey * dt;
+    vz = vz + aez * dt;
+    const double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
+    x = x + vx * dt;
+    y = y + vy * dt;
+    z = z + vz * dt;
+    if (mfp > 0.0) {
+      double jx, jy, jz;
+      or_sds_jump(&rng, p->sds_stats, logmr, V, mfp, dt, &jx, &jy, &jz);
+      x = x + jx;
+      y = y + jy;
+      z = z + jz;
+    }
+    t += dt;
+    step += 1;
+    if (in_metal_3d(f->ele, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
+      kind = 0;
+      if (rec && nrec < R) rec_row_sds(rec + nrec * C, tob + t, x, y, z, vx, vy, vz, f);
+      if (nrec < R) nrec++;
+      break;
+    }
+    if (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h || y > (ny - 1) * h ||
+        z > (nz - 1) * h) {
+      kind = 1;
+      if (rec && nrec < R) rec_row_sds(rec + nrec * C, tob + t, x, y, z, vx, vy, vz, f);
+      if (nrec < R) nrec++;
+      break;
+    }
+    int hit_pl = 0;
+    for (int ip = 0; ip < p->n_pl; ++ip) {
+      const int64_t col = p->pl_col[ip];
+      const double cn = col == 0 ? x : (col == 1 ? y : z);
+      const double co = col == 0 ? xo : (col == 1 ? yo : zo);
+      const double sgn = p->pl_sgn[ip], val = p->pl_val[ip];
+      int crossed;
+      if (sgn == 0.0)
+        crossed = (cn - val) * (co - val) <= 0.0 && cn != co;
+      else
+        crossed = (cn - val) * sgn >= 0.0 && (co - val) * sgn < 0.0;
+      if (!crossed) continue;
+      const double den = cn - co;
+      double fr = den == 0.0 ? 0.0 : (val - co) / den;
+      if (fr < 0.0) fr = 0.0;
+      if (fr > 1.0) fr = 1.0;
+      const double xc = xo + fr * (x - xo), yc = yo + fr * (y - yo), zc = zo + fr * (z - zo);
+      const double w1 = col == 0 ? yc : xc, w2 = col == 2 ? yc : zc;
+      const double *pw = p->pl_w + 4 * ip;
+      const int ins = pw[0] <= w1 && w1 <= pw[1] && pw[2] <= w2 && w2 <= pw[3];
+      if (p->pl_kind[ip] == 6) {
+        if (!ins) continue;
+      } else if (p->pl_kind[ip] == 5 && ins) {
+        continue;
+      }
+      x = xc;
+      y = yc;
+      z = zc;
+      vx = vxo + fr * (vx - vxo);
+      vy = vyo + fr * (vy - vyo);
+      vz = vzo + fr * (vz - vzo);
+      t = to + fr * (t - to);
+      kind = (int)p->pl_kind[ip];
+      hit_ip = ip;
+      hit_pl = 1;
+      break;
+    }
+    if (hit_pl) {
+      if (rec && nrec < R) rec_row_sds(rec + nrec * C, tob + t, x, y, z, vx, vy, vz, f);
+      if (nrec < R) nrec++;
+      break;
+    }
+    if (step % p->rec_every == 0 && nrec < R) {
+      if (rec) rec_row_sds(rec + nrec * C, tob + t, x, y, z, vx, vy, vz, f);
+      nrec++;
+    }
+  }
+  out->x[ii] = x;
+  out->y[ii] = y;
+  out->z[ii] = z;
+  out->vx[ii] = vx;
+  out->vy[ii] = vy;
+  out->vz[ii] = vz;
+  out->t[ii] = tob + t;
+  out->kind[ii] = kind;
+  out->face[ii] = hit_ip;
+  out->label[ii] = 0;
+  out->ncol[ii] = 0;
+  out->steps[ii] = step;
+  out->nrec[ii] = nrec;
+}
+
 int or_xyz_fly(const or_xyz_fields *f, const or_params *p, const or_ions *ions,
                or_out *out, int n_threads) {
   if (!f || !p || !ions || !out || f->nx < 2 || f->ny < 2 || f->nz < 2 || p->rec_every < 1)
     return -1;
+  if (p->gas == 2 && !p->sds_stats) return -1;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
-  for (int64_t i = 0; i < ions->n; ++i) fly_one3(f, p, ions, out, i);
+  for (int64_t i = 0; i < ions->n; ++i) {
+    if (p->gas == 2)
+      fly_one3_sds(f, p, ions, out, i);
+    else
+      fly_one3(f, p, ions, out, i);
+  }
   return 0;
 }

@@ -308,7 +308,8 @@ def fly_xyz(fields, flight, ions, n_threads=None):
     """Fly ions through a 3-D field pack (reference layout dict: EAx, EAy,
     EAz, ExK, EyK, EzK, ele (bool), h_mm, ch_kind, ch_om, ch_ph, ch_amp,
     ch_off, ch_duty, [tab_t, tab_v, tab_off]).  ``flight`` dt/t_max are in
-    seconds; ion velocities in mm/us; ``acc`` carries qm."""
+    seconds (RK4) or, with SDS gas (model 2, the _fly3d_sds flight), in
+    microseconds; ion velocities in mm/us; ``acc`` carries qm."""
     keep = _Keep()
     f = XYZFields()
     EAx = keep(fields["EAx"])
