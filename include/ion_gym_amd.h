@@ -208,6 +208,16 @@ typedef struct {
                           index list (one wave ballot each) before the
                           window, so dead and empty slots take no lanes;
                           results do not depend on it */
+  /* the periodic transporter of the 3-D RK4 flight (tracer3d.py:433-458,
+     fly3d's transporter argument, :722-735): tp_axis 0 = none, else
+     1 + the axis (1 x, 2 y, 3 z); an ion crossing the accept plane in tp_direction within a
+     step is shifted, with the step's start point, by emit - accept along
+     the axis; the tp_max_passes-th crossing ends it with fate
+     IG_FATE_TRANSPORTER.  Records then carry an 11th column, the pass
+     count (the reference's wrap_passes). */
+  int32_t tp_axis;
+  double tp_accept_mm, tp_emit_mm, tp_direction;
+  int64_t tp_max_passes;
 } ig_flight_params;
 
 /* One tune: the parameters one request or ensemble of a batched launch

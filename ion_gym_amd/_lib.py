@@ -66,7 +66,9 @@ class FlightParams(C.Structure):
                 ("pl_kind", C.c_int32 * MAX_PLANES),
                 ("rng_key", C.c_uint64), ("coll_batch", C.c_int32),
                 ("occupancy", C.c_int32), ("block", C.c_int32),
-                ("compact", C.c_int32)]
+                ("compact", C.c_int32), ("tp_axis", C.c_int32),
+                ("tp_accept_mm", C.c_double), ("tp_emit_mm", C.c_double),
+                ("tp_direction", C.c_double), ("tp_max_passes", C.c_int64)]
 
 
 class Tune(C.Structure):

@@ -72,6 +72,9 @@ struct FlightK {
   uint64_t rng_key;
   int32_t coll_batch; // parked hard-sphere lanes a wave scatters together
   int32_t occupancy;  // register-budget variant of the kernel (1, 2, 3)
+  int32_t tp_col;     // transporter axis, -1 none
+  double tp_accept, tp_d, tp_sgn; // accept plane, emit - accept, direction
+  int64_t tp_maxp;
 };
 
 // Device-resident ion state, structure of arrays.  t is the elapsed flight
@@ -94,6 +97,7 @@ struct EnsK {
   double *path, *eat, *ket; // record accumulators
   double *e_last;      // (n, 3) last evaluated field (acceleration units)
   double *rec;         // (n, rec_rows, rec_cols) or null
+  int32_t *tp_pass;    // transporter crossings so far
   const int32_t *idx;        // optional compacted list of live ions
   const uint32_t *n_live;    // its length (device), with idx
   const int32_t *tune; // per-ion tune index (with tunes)

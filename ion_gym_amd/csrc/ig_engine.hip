@@ -423,7 +423,8 @@ int ig_ensemble_create(ig_engine *h, int64_t capacity, int32_t rec_rows, int32_t
   if ((rc = E->alloc(k.ncol, n))) return rc;
   if ((rc = E->alloc(k.steps, n))) return rc;
   if ((rc = E->alloc(k.hz, n))) return rc;
-  int32_t **i32[] = {&k.has_gauss, &k.fate, &k.face, &k.label, &k.nrec, &E->tune_idx};
+  int32_t **i32[] = {&k.has_gauss, &k.fate, &k.face, &k.label, &k.nrec, &k.tp_pass,
+                     &E->tune_idx};
   for (int32_t **a : i32)
     if ((rc = E->alloc(*a, n))) return rc;
   // slots hold no ion until loaded: a fate the kernels skip
@@ -521,6 +522,7 @@ static int load_range(ig_engine *h, Ensemble *E, int64_t off, const ig_ions *io,
   HIPX(hipMemsetAsync(k.face + off, 0xff, b4, st)); // -1
   HIPX(hipMemsetAsync(k.label + off, 0, b4, st));
   HIPX(hipMemsetAsync(k.nrec + off, 0, b4, st));
+  HIPX(hipMemsetAsync(k.tp_pass + off, 0, b4, st));
   std::vector<int32_t> fz((size_t)n, IG_FATE_FLYING), tz((size_t)n, tune);
   if (io->fate_in)
     for (int64_t i = 0; i < n; ++i) fz[i] = io->fate_in[i];
@@ -832,6 +834,22 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
   if (p->block != 0 && (p->block < 64 || p->block > 256 || p->block % 64 != 0))
     return fail(IG_E_ARG, "ig_fly: block must be 0 (default), 64, 128, 192 or 256");
   if (p->compact != 0 && p->compact != 1) return fail(IG_E_ARG, "ig_fly: compact must be 0 or 1");
+  k.tp_col = -1;
+  if (p->tp_axis != 0) {
+    if (p->tp_axis < 1 || p->tp_axis > 3)
+      return fail(IG_E_ARG, "ig_fly: transporter axis must be 0 (none), 1 (x), 2 (y) or 3 (z)");
+    if (p->tp_accept_mm == p->tp_emit_mm)
+      return fail(IG_E_ARG, "transporter: accept_mm == emit_mm -- a zero-length transporter "
+                            "is a no-op in disguise");
+    if (p->tp_max_passes < 1) return fail(IG_E_ARG, "ig_fly: transporter max_passes must be >= 1");
+    if (p->gas_model == IG_GAS_SDS)
+      return fail(IG_E_ARG, "ig_fly: the transporter belongs to the 3-D RK4 flight, not SDS");
+    k.tp_col = p->tp_axis - 1;
+    k.tp_accept = p->tp_accept_mm;
+    k.tp_d = p->tp_emit_mm - p->tp_accept_mm;
+    k.tp_sgn = p->tp_direction;
+    k.tp_maxp = p->tp_max_passes;
+  }
   return 0;
 }
 
@@ -920,6 +938,10 @@ static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params
     return fail(IG_E_ARG, w + ": the ensemble was flown on another route or clock (3-D RK4 "
                               "carries s and m/s, every other flight us and mm/us); load it "
                               "again");
+  if (k.tp_col >= 0 && fk->route != IG_ROUTE_XYZ)
+    return fail(IG_E_ARG, w + ": the transporter belongs to the 3-D route");
+  if (k.tp_col >= 0 && p->rec_rows > 0 && E->rec_cols < 11)
+    return fail(IG_E_ARG, w + ": records with the transporter need 11 columns (the pass count)");
   E->route = fk->route;
   E->clock = clk;
   if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec);
@@ -1443,7 +1465,7 @@ int ig_fly_batch(ig_engine *h, ig_fields_id f, const ig_flight_params *p, const 
       cols = 7;
       for (int b = 0; b < 16; ++b) cols += (p->rec_mask >> b) & 1u;
     } else {
-      cols = 10;
+      cols = p->tp_axis > 0 ? 11 : 10;
     }
   }
   ig_ensemble_id e = 0;

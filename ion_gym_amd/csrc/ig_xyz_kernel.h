@@ -8,9 +8,9 @@
 // and box.  FP64 arithmetic in the reference's order (contraction off).
 // The two mid-step RK4 evaluations share one lab time
 // (tracer3d.py:403, 408), so their waveform weights are computed once.
-// The periodic transporter (tracer3d.py:434-458) is not part of this
-// kernel: the host refuses it.  The 3-D SDS flight (_fly3d_sds) is
-// xyz_sds_kernel below.
+// The periodic transporter (tracer3d.py:433-458) shifts an ion crossing
+// its accept plane.  The 3-D SDS flight (_fly3d_sds) is xyz_sds_kernel
+// below.
 #pragma once
 #pragma clang fp contract(off)
 #include "ig_device.h"
@@ -266,7 +266,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     else
       xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
   };
-  auto write = [&](int row, double tus) {
+  int32_t tpp = p.tp_col >= 0 ? e.tp_pass[i] : 0;
+  // a record row; with the transporter the 11th column is the pass count
+  // (0 on a station-plane row, which the reference leaves unset)
+  auto write = [&](int row, double tus, int32_t wrap) {
     double *r = e.rec + ((size_t)i * p.rec_rows + row) * p.rec_cols;
     r[0] = tus;
     r[1] = x;
@@ -280,9 +283,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     r[7] = a;
     r[8] = b;
     r[9] = c;
+    if (p.rec_cols > 10) r[10] = (double)wrap;
   };
   if (nrec == 0) {
-    if (rec) write(0, tob);
+    if (rec) write(0, tob, 0);
     nrec = 1;
   }
   const int R = rec ? p.rec_rows : 0;
@@ -425,7 +429,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     }
     if (stage == kPost) {
       stage = kStep;
-      if (in_metal_3d(f.metal, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
+      // the transporter plane, before any terminal test: a crossing of
+      // the accept plane in its direction counts a pass and shifts the
+      // ion and the step's start point by emit - accept; the last
+      // allowed pass ends the flight unshifted
+      if (p.tp_col >= 0) {
+        const double cn = p.tp_col == 0 ? x : (p.tp_col == 1 ? y : z);
+        const double co = p.tp_col == 0 ? xo : (p.tp_col == 1 ? yo : zo);
+        if ((cn - p.tp_accept) * p.tp_sgn >= 0.0 && (co - p.tp_accept) * p.tp_sgn < 0.0) {
+          tpp += 1;
+          if (tpp >= p.tp_maxp) {
+            fate = IG_FATE_TRANSPORTER;
+            if (nrec < R) write(nrec, tob + t * 1e6, tpp);
+            if (rec && nrec < R) nrec++;
+            stage = kDone;
+          } else if (p.tp_col == 0) {
+            x += p.tp_d;
+            xo += p.tp_d;
+          } else if (p.tp_col == 1) {
+            y += p.tp_d;
+            yo += p.tp_d;
+          } else {
+            z += p.tp_d;
+            zo += p.tp_d;
+          }
+        }
+      }
+      if (stage != kDone && in_metal_3d(f.metal, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
         double f0 = 0.0, f1 = 1.0;
         for (int b = 0; b < 40; ++b) {
           double fm = 0.5 * (f0 + f1);
@@ -444,7 +474,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         vz = vzo + fr * (vz - vzo);
         t = to + fr * dt;
         fate = IG_FATE_IMPACT;
-        if (nrec < R) write(nrec, tob + t * 1e6);
+        if (nrec < R) write(nrec, tob + t * 1e6, tpp);
         if (rec && nrec < R) nrec++;
         stage = kDone;
       }
@@ -483,7 +513,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
           t = to + fr * dt;
           fate = p.pl_kind[ip];
           hit_ip = ip;
-          if (nrec < R) write(nrec, tob + t * 1e6);
+          if (nrec < R) write(nrec, tob + t * 1e6, 0);
           if (rec && nrec < R) nrec++;
           stage = kDone;
           break;
@@ -492,12 +522,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       if (stage != kDone && (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h ||
                              y > (ny - 1) * h || z > (nz - 1) * h)) {
         fate = IG_FATE_BOX;
-        if (nrec < R) write(nrec, tob + t * 1e6);
+        if (nrec < R) write(nrec, tob + t * 1e6, tpp);
         if (rec && nrec < R) nrec++;
         stage = kDone;
       }
       if (stage != kDone && (steps0 + s) % p.rec_every == 0 && nrec < R) {
-        write(nrec, tob + t * 1e6);
+        write(nrec, tob + t * 1e6, tpp);
         nrec++;
       }
     }
@@ -519,6 +549,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   e.has_gauss[i] = rng.has_gauss;
   e.gauss[i] = rng.gauss;
   if constexpr (kFlat) e.hz[i] = hz;
+  if (p.tp_col >= 0) e.tp_pass[i] = tpp;
 }
 
 // tracer3d._fly3d_sds (tracer3d.py:851-1025) with _sds_field

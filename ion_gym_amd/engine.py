@@ -207,7 +207,8 @@ class Flight:
 
     def __init__(self, dt, t_max, *, max_steps=0, rec_every=1, rec_rows=0,
                  channels=(), gas=None, bounds=None, planes=(), rng_key=0,
-                 coll_batch=0, occupancy=0, block=0, compact=True):
+                 coll_batch=0, occupancy=0, block=0, compact=True,
+                 transporter=None):
         self.dt = float(dt)
         self.t_max = float(t_max)
         self.max_steps = int(max_steps)
@@ -222,6 +223,10 @@ class Flight:
         self.occupancy = int(occupancy)
         self.block = int(block)
         self.compact = bool(compact)
+        # the 3-D periodic transporter (tracer3d.py:722-735): a dict with
+        # axis ('x'|'y'|'z'), accept_mm, emit_mm, direction (default -1)
+        # and max_passes (default 10**9), or None
+        self.transporter = None if transporter is None else dict(transporter)
 
     def launch(self, **kw):
         """A copy with other launch settings (block, occupancy,
@@ -274,6 +279,13 @@ class Flight:
         p.occupancy = self.occupancy
         p.block = self.block
         p.compact = int(self.compact)
+        tp = self.transporter
+        if tp is not None:
+            p.tp_axis = 1 + {"x": 0, "y": 1, "z": 2}[str(tp["axis"])]
+            p.tp_accept_mm = float(tp["accept_mm"])
+            p.tp_emit_mm = float(tp["emit_mm"])
+            p.tp_direction = float(tp.get("direction", -1))
+            p.tp_max_passes = int(tp.get("max_passes", 10 ** 9))
         return p
 
 
@@ -547,7 +559,7 @@ class Engine:
         if fields.route in (L.ROUTE_RZ, L.ROUTE_PLANAR):
             cols = 7 + len(set(flight.channels))
         else:
-            cols = 10
+            cols = 11 if flight.transporter is not None else 10
         rows = flight.rec_rows
         res = {k: np.zeros(n) for k in ("x", "y", "z", "vx", "vy", "vz",
                                         "t_us")}

@@ -97,6 +97,10 @@ typedef struct {
   const int64_t *pl_kind;
   int32_t rng;
   uint64_t rng_key;
+  /* 3-D transporter (tracer3d.py:433-458): tp_col -1 none, else the axis */
+  int32_t tp_col;
+  double tp_accept, tp_emit, tp_sgn;
+  int64_t tp_maxp;
 } or_params;
 
 typedef struct {

@@ -113,8 +113,11 @@ static void efield3(const or_xyz_fields *f, double px, double py, double pz, dou
   *ez = c;
 }
 
+/* a record row (tracer3d.py:441-451 and the other record sites); with
+ * an 11th column, the transporter pass count (wrs), which the station-
+ * plane row leaves at 0 */
 static void rec_row(double *row, double t_us, double x, double y, double z, double vx,
-                    double vy, double vz, const or_xyz_fields *f) {
+                    double vy, double vz, const or_xyz_fields *f, int64_t C, int64_t wrap) {
   row[0] = t_us;
   row[1] = x;
   row[2] = y;
@@ -123,6 +126,7 @@ static void rec_row(double *row, double t_us, double x, double y, double z, doub
   row[5] = vy * 1e-3;
   row[6] = vz * 1e-3;
   efield3(f, x, y, z, t_us, row + 7, row + 8, row + 9);
+  if (C > 10) row[10] = (double)wrap;
 }
 
 static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *io,
@@ -143,8 +147,9 @@ static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *
   double *rec = p->rec_rows > 0 && out->rec ? out->rec + ii * p->rec_rows * p->rec_cols : 0;
   int64_t R = p->rec_rows, C = p->rec_cols;
   const double c = 1e3;
-  if (rec) rec_row(rec, tob, x, y, z, vx, vy, vz, f);
-  int64_t nrec = 1, step = 0, ncol = 0;
+  if (rec) rec_row(rec, tob, x, y, z, vx, vy, vz, f, C, 0);
+  int64_t nrec = 1, step = 0, ncol = 0, tp_pass = 0;
+  const double tp_d = p->tp_emit - p->tp_accept;
   int kind = 2, hit_ip = -1;
   double t = 0.0;
   int64_t limit = p->max_steps;
@@ -203,6 +208,39 @@ static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *
         ncol += 1;
       }
     }
+    if (p->tp_col >= 0) {
+      double cn, co;
+      if (p->tp_col == 0) {
+        cn = x;
+        co = xo;
+      } else if (p->tp_col == 1) {
+        cn = y;
+        co = yo;
+      } else {
+        cn = z;
+        co = zo;
+      }
+      if ((cn - p->tp_accept) * p->tp_sgn >= 0.0 && (co - p->tp_accept) * p->tp_sgn < 0.0) {
+        tp_pass += 1;
+        if (tp_pass >= p->tp_maxp) {
+          kind = 4;
+          if (rec && nrec < R) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f, C,
+                                       tp_pass);
+          if (nrec < R) nrec++;
+          break;
+        }
+        if (p->tp_col == 0) {
+          x += tp_d;
+          xo += tp_d;
+        } else if (p->tp_col == 1) {
+          y += tp_d;
+          yo += tp_d;
+        } else {
+          z += tp_d;
+          zo += tp_d;
+        }
+      }
+    }
     if (in_metal_3d(f->ele, x * inv_h, y * inv_h, z * inv_h, nx, ny, nz)) {
       double f0 = 0.0, f1 = 1.0;
       for (int b = 0; b < 40; ++b) {
@@ -222,7 +260,8 @@ static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *
       vz = vzo + fr * (vz - vzo);
       t = to + fr * dt;
       kind = 0;
-      if (rec && nrec < R) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f);
+      if (rec && nrec < R)
+        rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f, C, tp_pass);
       if (nrec < R) nrec++;
       break;
     }
@@ -284,7 +323,7 @@ static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *
         break;
       }
       if (hit_pl) {
-        if (rec && nrec < R) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f);
+        if (rec && nrec < R) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f, C, 0);
         if (nrec < R) nrec++;
         break;
       }
@@ -292,12 +331,13 @@ static void fly_one3(const or_xyz_fields *f, const or_params *p, const or_ions *
     if (x < 0.0 || y < 0.0 || z < 0.0 || x > (nx - 1) * h || y > (ny - 1) * h ||
         z > (nz - 1) * h) {
       kind = 1;
-      if (rec && nrec < R) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f);
+      if (rec && nrec < R)
+        rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f, C, tp_pass);
       if (nrec < R) nrec++;
       break;
     }
     if (step % p->rec_every == 0 && nrec < R) {
-      if (rec) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f);
+      if (rec) rec_row(rec + nrec * C, tob + t * 1e6, x, y, z, vx, vy, vz, f, C, tp_pass);
       nrec++;
     }
   }

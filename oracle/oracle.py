@@ -68,7 +68,9 @@ class Params(C.Structure):
                 ("bnd_val", C.c_double * 6), ("n_pl", C.c_int32),
                 ("pl_col", _i64p), ("pl_val", _dp), ("pl_sgn", _dp),
                 ("pl_w", _dp), ("pl_kind", _i64p), ("rng", C.c_int32),
-                ("rng_key", C.c_uint64)]
+                ("rng_key", C.c_uint64), ("tp_col", C.c_int32),
+                ("tp_accept", C.c_double), ("tp_emit", C.c_double),
+                ("tp_sgn", C.c_double), ("tp_maxp", C.c_int64)]
 
 
 class Ions(C.Structure):
@@ -213,6 +215,14 @@ def _params(keep, fl, route_cols):
     for i in range(6):
         p.bnd_on[i] = int(bool(fl.get("bnd_on", [0] * 6)[i]))
         p.bnd_val[i] = float(fl.get("bnd_val", [0.0] * 6)[i])
+    tp = fl.get("transporter")
+    p.tp_col = -1
+    if tp is not None:
+        p.tp_col = {"x": 0, "y": 1, "z": 2}[str(tp["axis"])]
+        p.tp_accept = float(tp["accept_mm"])
+        p.tp_emit = float(tp["emit_mm"])
+        p.tp_sgn = float(tp.get("direction", -1))
+        p.tp_maxp = int(tp.get("max_passes", 10 ** 9))
     planes = fl.get("planes", [])
     p.n_pl = len(planes)
     if planes:
@@ -331,9 +341,10 @@ def fly_xyz(fields, flight, ions, n_threads=None):
     f.tab_v = _p(keep(fields.get("tab_v", np.zeros(1))))
     f.tab_off = _p(keep(fields.get("tab_off", np.zeros(K + 1)), np.int64),
                    _i64p)
-    p = _params(keep, flight, 10)
+    cols = 11 if flight.get("transporter") is not None else 10
+    p = _params(keep, flight, cols)
     io = _ions(keep, ions)
-    o, res = _out(io.n, p.rec_rows, 10)
+    o, res = _out(io.n, p.rec_rows, cols)
     rc = lib().or_xyz_fly(C.byref(f), C.byref(p), C.byref(io), C.byref(o),
                           n_threads or os.cpu_count())
     if rc != 0:

@@ -242,3 +242,35 @@ def test_quad_full_grid_fp32_basis_error(engine):
     d = np.sqrt((a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 +
                 (a["z"] - b["z"]) ** 2)[same]
     assert np.median(d) < 1e-5 and d.max() < 1e-3, (np.median(d), d.max())
+
+
+def test_quad_transporter_matches_oracle(engine):
+    """The periodic transporter along the rods (accept 15 mm, emit 5 mm,
+    direction +1, three passes) with hard-sphere gas: ion by ion against
+    the oracle's Philox flight, fates 4 on the third pass, and the pass
+    count column of every record row."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.physics.collision3d import gas_kernel_scalars
+    c = configs.quad_3d(n_ions=300, coarse=4, n_cycles=40)
+    g0 = gas_kernel_scalars("N2", T_k=300.0, P_torr=0.01)
+    c["gas"] = dict(model=1, **{k: g0[k] for k in ("T_k", "P_pa", "c_star",
+                                                   "c_bar", "sig1d",
+                                                   "m_gas")})
+    ions = dict(c["ions"])
+    ions["sigma_m2"] = np.full(len(ions["x"]), 2e-18)
+    tp = dict(axis="z", accept_mm=15.0, emit_mm=5.0, direction=1.0,
+              max_passes=3)
+    fp = engine.upload_xyz(c["pack"], "f64")
+    g = engine.fly_batch(fp, _f(c, transporter=tp, rec_every=100,
+                                rec_rows=600), ions)
+    o = O.fly_xyz(c["pack"], dict(dt=c["dt"], t_max=c["t_max"], gas=c["gas"],
+                                  rng=O.RNG_PHILOX, transporter=tp,
+                                  rec_every=100, rec_rows=600), ions)
+    _parity(g, o, frac_ok=0.99)
+    assert (o["kind"] == 4).sum() > 10
+    same = g["fate"] == o["kind"]
+    assert np.array_equal(g["nrec"][same], o["nrec"][same])
+    for i in np.nonzero(same)[0][:100]:
+        n = int(o["nrec"][i])
+        assert np.array_equal(g["records"][i, :n, 10], o["rec"][i, :n, 10])
+    fp.release()

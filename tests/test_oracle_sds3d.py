@@ -67,3 +67,46 @@ def test_diffusion_and_drift():
     alive = o["kind"] == -2
     vz = (o["z"] - ions["z"])[alive] / t
     assert abs(vz.mean() / (c["K_mm2_per_V_us"] * 5.0) - 1.0) < 0.02
+
+
+def test_transporter_unwraps_to_the_straight_line():
+    """Field-free 3-D RK4 flight along z through a transporter (accept
+    8.005 mm, emit 2 mm, direction +1, three passes): the pass count in
+    each record row is the number of accept crossings of the unwrapped
+    path z_u = z0 + v t, the flight ends with fate 4 on the third pass,
+    and z + 2 (accept - emit) there is the straight line (the last pass
+    is not shifted)."""
+    nx, ny, nz, h = 5, 5, 101, 0.1
+    pack = dict(EAx=np.zeros((nx, ny, nz)), EAy=np.zeros((nx, ny, nz)),
+                EAz=np.zeros((nx, ny, nz)), ExK=np.zeros((0, nx, ny, nz)),
+                EyK=np.zeros((0, nx, ny, nz)), EzK=np.zeros((0, nx, ny, nz)),
+                ele=np.zeros((nx, ny, nz), bool), h_mm=h,
+                ch_kind=np.zeros(0, np.int64), ch_om=np.zeros(0),
+                ch_ph=np.zeros(0), ch_amp=np.zeros(0), ch_off=np.zeros(0),
+                ch_duty=np.zeros(0))
+    n = 3
+    ions = dict(x=np.full(n, 0.2), y=np.full(n, 0.2), z=np.array([1.0, 1.3, 4.0]),
+                vx=np.zeros(n), vy=np.zeros(n), vz=np.array([1.0, 0.7, 1.3]),
+                tob=np.zeros(n), m_amu=np.full(n, 100.0),
+                acc=np.full(n, 9.6e5), sigma_m2=np.zeros(n),
+                seed=np.arange(n, dtype=np.int64))
+    tp = dict(axis="z", accept_mm=8.005, emit_mm=2.0, direction=1.0,
+              max_passes=3)
+    dt = 1e-8
+    o = O.fly_xyz(pack, dict(dt=dt, t_max=1e-3, transporter=tp, rec_every=37,
+                             rec_rows=2000), ions)
+    d = tp["accept_mm"] - tp["emit_mm"]
+    assert np.all(o["kind"] == 4)
+    for i in range(n):
+        rows = o["rec"][i, :o["nrec"][i]]
+        t_us = rows[:, 0]
+        zu = ions["z"][i] + ions["vz"][i] * t_us
+        passes = np.floor((zu - tp["accept_mm"]) / d) + 1
+        passes = np.maximum(passes, 0)
+        # the last row is the third pass, recorded unshifted
+        assert np.array_equal(rows[:-1, 10], passes[:-1])
+        assert rows[-1, 10] == 3
+        assert np.allclose(rows[:-1, 3] + rows[:-1, 10] * d, zu[:-1],
+                           rtol=0, atol=1e-9)
+        assert abs(o["z"][i] + 2 * d - (ions["z"][i] + ions["vz"][i]
+                                         * (o["t"][i]))) < 1e-9
