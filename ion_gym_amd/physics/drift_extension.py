@@ -31,18 +31,20 @@ _BASE = ("t", "x", "y", "z", "vx", "vy", "vz")
 _CHANNEL_RULES = {
     "speed": "const", "ke_ev": "const",
     "ke_x": "const", "ke_y": "const", "ke_z": "const",
-    "n_col": "const",
+    "n_col": "const", "wrap_passes": "const",
     "e_field": "zero", "e_axial": "zero", "e_radial": "zero",
+    "e_x": "zero", "e_y": "zero", "e_z": "zero",
     "e_axial_tint": "const",
     "path_mm": lambda prev, s, dt: prev + s["sp"] * dt,
     "ke_tint": lambda prev, s, dt: prev + s["ke_ev"] * dt,
 }
 
 
-def edge_field_max(components, axis, side):
+def edge_field_max(components, axis, side, inset=0):
     """Worst-case |E| over one boundary face: each component a list of
     (array, weight) pairs bounded by sum |w a|, the components combined
-    as a vector norm, the max over the face's nodes."""
+    as a vector norm, the max over the face's nodes (``inset`` node rows
+    inside the face)."""
     acc = None
     for comp in components:
         mag = None
@@ -51,7 +53,7 @@ def edge_field_max(components, axis, side):
                 continue
             a = np.asarray(arr, float)
             sl = [slice(None)] * a.ndim
-            sl[axis] = 0 if side == "lo" else -1
+            sl[axis] = int(inset) if side == "lo" else -1 - int(inset)
             face = np.abs(float(w) * a[tuple(sl)])
             mag = face if mag is None else mag + face
         if mag is None:
@@ -234,3 +236,45 @@ def rz_extend(traj, col_names, kind, spec, model, edge_fields, label=""):
         derived={"radius": lambda prev, row, idx, dt:
                  math.hypot(row[idx["y"]], row[idx["z"]])},
         label=label)
+
+
+def _w3d_max(pack, k):
+    """Worst-case |w| of 3-D drive channel k (drift_extension.py:298-311):
+    |amp| max|base| + |off|, max|base| 1 for sin/cos/square and the
+    table slice's max |value| for a table."""
+    kind = int(pack["ch_kind"][k])
+    if kind in (3, 4):
+        o0, o1 = int(pack["tab_off"][k]), int(pack["tab_off"][k + 1])
+        tv = np.asarray(pack["tab_v"], float)[o0:o1]
+        base = float(np.max(np.abs(tv))) if tv.size else 0.0
+    else:
+        base = 1.0
+    return (abs(float(pack["ch_amp"][k])) * base
+            + abs(float(pack["ch_off"][k])))
+
+
+def xyz_edge_fields(pack):
+    """{face: worst-case residual |E| (V/mm)} on the six faces of a 3-D
+    pack (drift_extension.py:313-330, route 3d)."""
+    comps = []
+    for a, ka in (("EAx", "ExK"), ("EAy", "EyK"), ("EAz", "EzK")):
+        K = np.asarray(pack[ka])
+        comps.append([(pack[a], 1.0)]
+                     + [(K[k], _w3d_max(pack, k)) for k in range(K.shape[0])])
+    return {f"{ax}_{s}": edge_field_max(comps, i, s)
+            for i, ax in enumerate("xyz") for s in ("lo", "hi")}
+
+
+def xyz_escape_face(shape, h_mm, p):
+    """(face key, axis, side, words) of a 3-D escape at kernel point p:
+    the face the point is furthest outside (drift_extension.py:356-400)."""
+    best, out = 0.0, None
+    for i, ax in enumerate("xyz"):
+        hi = (shape[i] - 1) * h_mm
+        for side, d in (("lo", -p[i]), ("hi", p[i] - hi)):
+            if d > best:
+                best, out = d, (f"{ax}_{side}", i, side, f"{ax} {side}")
+    if out is None:
+        raise ValueError(f"xyz_escape_face: point {tuple(p)} is inside the "
+                         f"domain; no escape face")
+    return out

@@ -135,6 +135,7 @@ class IntegrationSpec:
     t_max_us: float = 50.0
     rec_every: int = 10
     record_channels: List[str] = field(default_factory=list)
+    max_records: int = 200000    # 3-D record buffer rows per ion
 
 
 @dataclass
@@ -169,3 +170,6 @@ class FlightSpec:
     # kind 5 impact plate / 6 detector)
     planes: List[tuple] = field(default_factory=list)
     precision: str = "f64"       # engine field storage: f64 | f32_store | f32
+    # the 3-D periodic transporter (sim_spec transporter; tracer3d.py:
+    # 722-735): dict(axis, accept_mm, emit_mm, direction, max_passes)
+    transporter: Optional[dict] = None

@@ -93,3 +93,28 @@ def rz_birth_refusal(ele, h, x, r, i):
             f"effective metal of electrode {int(ele[gi, gj])} (nearest-node "
             f"rule, node pitch {h} mm)")
     return None
+
+
+def xyz_birth_refusal(metal, h, x, y, z, i):
+    """The 3-D birth checks (tracer3d.py:192-230): outside nodes 0..n-1 of
+    any axis in node units x / h (birth_domain.py:27-38), then the
+    all-eight-corners metal rule of the kernels' impact test
+    (tracer3d.py:174-189).  Returns None or the exception to record."""
+    m = np.asarray(metal)
+    nx, ny, nz = m.shape
+    inv_h = 1.0 / float(h)
+    g = (x * inv_h, y * inv_h, z * inv_h)
+    for gv, n, ax in zip(g, (nx, ny, nz), "xyz"):
+        if not (-DOMAIN_EDGE_TOL <= gv <= n - 1 + DOMAIN_EDGE_TOL):
+            return BirthOutsideDomain(
+                f"ion {i} birth at (x={x:.3f}, y={y:.3f}, z={z:.3f}) mm lies "
+                f"outside the solved domain: {ax} is at node {gv:.3f}, and "
+                f"the domain spans nodes 0 to {n - 1} (pitch {h} mm)")
+    if all(0.0 <= gv <= n - 1 for gv, n in zip(g, (nx, ny, nz))):
+        a, b, c = (min(int(gv), n - 2) for gv, n in zip(g, (nx, ny, nz)))
+        if np.all(m[a:a + 2, b:b + 2, c:c + 2]):
+            return BirthInMetal(
+                f"ion {i} birth at (x={x:.3f}, y={y:.3f}, z={z:.3f}) mm lies "
+                f"inside the effective metal (all-eight-corner voxel rule, "
+                f"node pitch {h} mm)")
+    return None

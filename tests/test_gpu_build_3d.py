@@ -1,0 +1,128 @@
+"""The engine-backed 3-D fly function (build_stl3d fly_fn over fly3d and
+fly3d_sds) against the oracle: trajectory columns and summaries on the
+quadrupole in vacuum and with the transporter, the SDS drift cell through
+collisions.model 'sds', birth refusals, and the exact drift extension to a
+bound beyond the solved box."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ion_gym_amd.physics.flight_spec import (BoundsSpec, CollisionSpec,
+                                             FlightSpec, IntegrationSpec,
+                                             SourceSpec)
+
+pytestmark = pytest.mark.gpu
+
+
+def _births(ions):
+    return np.stack([ions[k] for k in ("x", "y", "z", "vx", "vy", "vz",
+                                       "tob")], axis=1)
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0))
+
+
+def test_quad_fly_fn_matches_oracle():
+    from ion_gym_amd import configs
+    from ion_gym_amd.physics.build_3d import make_3d_fly_fn
+    c = configs.quad_3d(n_ions=200, coarse=4, n_cycles=40)
+    ions = c["ions"]
+    n = len(ions["x"])
+    tp = dict(axis="z", accept_mm=15.0, emit_mm=5.0, direction=1.0,
+              max_passes=2)
+    spec = FlightSpec(
+        source=SourceSpec(n_ions=1, mz_list=list(ions["m_amu"]), seed=0),
+        integration=IntegrationSpec(
+            dt_ns=c["dt"] * 1e9, t_max_us=c["t_max"] * 1e6, rec_every=50,
+            record_channels=["speed", "ke_ev", "e_field", "radius", "n_col",
+                             "wrap_passes"]),
+        transporter=tp)
+    fly_fn, cols = make_3d_fly_fn(c["pack"], _births(ions), spec)
+    got = fly_fn.batch(range(n))
+    oi = dict(ions, seed=np.arange(n, dtype=np.int64))
+    fl = dict(dt=spec.integration.dt_ns * 1e-9,
+              t_max=spec.integration.t_max_us * 1e-6, rec_every=50,
+              rec_rows=2000, transporter=tp)
+    o = O.fly_xyz(c["pack"], fl, oi)
+    kinds = set()
+    for i, (tr, s) in enumerate(got):
+        nr = int(o["nrec"][i])
+        assert tr.shape == (nr, len(cols))
+        assert _rel(tr[:, :7], o["rec"][i, :nr, :7]) <= 1e-9
+        sp = np.sqrt((o["rec"][i, :nr, 4:7] ** 2).sum(axis=1))
+        assert _rel(tr[:, cols.index("speed")], sp) <= 1e-9
+        assert np.array_equal(tr[:, cols.index("wrap_passes")],
+                              o["rec"][i, :nr, 10])
+        assert s["kind"] == o["kind"][i]
+        assert abs(s["tof"] - o["t"][i]) <= 1e-9 * max(1.0, o["t"][i])
+        assert s["n_col"] == 0 and not s["truncated"]
+        kinds.add(s["kind"])
+    assert 4 in kinds
+
+
+def test_sds_cell_fly_fn_matches_oracle():
+    from ion_gym_amd import configs
+    from ion_gym_amd.physics.build_3d import make_3d_fly_fn
+    c = configs.drift_3d(n_ions=300, coarse=2, t_max_us=300.0)
+    ions = c["ions"]
+    n = len(ions["x"])
+    spec = FlightSpec(
+        source=SourceSpec(n_ions=n, mz_list=[500.0], seed=6000),
+        collisions=CollisionSpec(enabled=True, model="sds", gas="N2",
+                                 T_k=300.0, P_torr=4.0, K0_cm2_per_Vs=1.5),
+        integration=IntegrationSpec(dt_ns=c["dt"] * 1e3, t_max_us=300.0,
+                                    rec_every=200,
+                                    record_channels=["e_z", "n_col"]),
+        planes=c["planes"])
+    fly_fn, cols = make_3d_fly_fn(c["pack"], _births(ions), spec)
+    got = fly_fn.batch(range(n))
+    fl = dict(dt=spec.integration.dt_ns * 1e-3, t_max=300.0, gas=c["gas"],
+              planes=c["planes"], rng=O.RNG_PHILOX, rec_every=200,
+              rec_rows=2000)
+    o = O.fly_xyz(c["pack"], fl, ions)
+    for i, (tr, s) in enumerate(got):
+        nr = int(o["nrec"][i])
+        assert _rel(tr[:, :7], o["rec"][i, :nr, :7]) <= 1e-9
+        assert _rel(tr[:, cols.index("e_z")], o["rec"][i, :nr, 9]) <= 1e-9
+        assert np.all(np.isnan(tr[:, cols.index("n_col")]))
+        assert s["kind"] == o["kind"][i] and s["n_col"] is None
+    assert {0, 6}.issubset({s["kind"] for _, s in got})
+
+
+def test_birth_refusal_and_drift_extension():
+    """A field-free box (11 x 11 x 41 nodes @ 0.5 mm) with a metal block:
+    an ion born in the block splats at birth; ions leaving through z_hi
+    with a bound declared 5 mm beyond end on it, exactly."""
+    from ion_gym_amd.physics.build_3d import make_3d_fly_fn
+    shape = (11, 11, 41)
+    z = np.zeros(shape)
+    ele = np.zeros(shape, bool)
+    ele[0:3, 0:3, 10:14] = True
+    pack = dict(EAx=z, EAy=z, EAz=z, ExK=np.zeros((0,) + shape),
+                EyK=np.zeros((0,) + shape), EzK=np.zeros((0,) + shape),
+                ele=ele, h_mm=0.5, ch_kind=np.zeros(0, np.int64),
+                ch_om=np.zeros(0), ch_ph=np.zeros(0), ch_amp=np.zeros(0),
+                ch_off=np.zeros(0), ch_duty=np.zeros(0))
+    births = np.array([[2.5, 2.5, 1.0, 0.1, -0.05, 1.2, 0.0],
+                       [3.0, 2.0, 2.0, 0.0, 0.0, 0.8, 0.3],
+                       [0.5, 0.5, 6.0, 0.0, 0.0, 1.0, 0.0]])
+    spec = FlightSpec(
+        source=SourceSpec(n_ions=3, mz_list=[200.0], seed=1),
+        integration=IntegrationSpec(dt_ns=2.0, t_max_us=60.0, rec_every=100),
+        bounds=BoundsSpec(z_max_on=True, z_max=25.0))
+    fly_fn, cols = make_3d_fly_fn(pack, births, spec)
+    out = fly_fn.batch(range(3))
+    for i in (0, 1):
+        tr, s = out[i]
+        assert s["kind"] == 3
+        b = births[i]
+        t_hit = (25.0 - b[2]) / b[5]
+        assert abs(tr[-1, 3] - 25.0) < 1e-9
+        assert abs(tr[-1, 0] - (b[6] + t_hit)) < 1e-6
+        assert abs(tr[-1, 1] - (b[0] + b[3] * t_hit)) < 1e-6
+        assert tr[-2, 3] > 20.0 and tr[-2, 3] < 25.0
+    tr, s = out[2]
+    assert s["kind"] == 0 and s.get("born_in_metal") and tr.shape[0] == 1
