@@ -118,3 +118,32 @@ def xyz_birth_refusal(metal, h, x, y, z, i):
                 f"inside the effective metal (all-eight-corner voxel rule, "
                 f"node pitch {h} mm)")
     return None
+
+
+def planar_birth_refusal(ele, h, x, y, i):
+    """The planar birth checks (build_planar.py:1588-1605): outside nodes
+    0..n-1 of x or y, then the nearest-node metal rule on the planar
+    labels.  Returns None or the exception to record."""
+    gx, gy = x / h, y / h
+    nx, ny = ele.shape
+    for g, n, ax in ((gx, nx, "x"), (gy, ny, "y")):
+        if not (-DOMAIN_EDGE_TOL <= g <= n - 1 + DOMAIN_EDGE_TOL):
+            return BirthOutsideDomain(
+                f"ion {i} birth at ({x:.3f}, {y:.3f}) mm lies outside the "
+                f"solved domain: {ax} is at node {g:.3f}, and the domain "
+                f"spans nodes 0 to {n - 1} (pitch {h} mm)")
+    gi, gj = int(np.floor(gx + 0.5)), int(np.floor(gy + 0.5))
+    if 0 <= gi < nx and 0 <= gj < ny and ele[gi, gj] > 0:
+        return BirthInMetal(
+            f"ion {i} birth at ({x:.3f}, {y:.3f}) mm lies inside the "
+            f"effective metal of electrode {int(ele[gi, gj])} (nearest-node "
+            f"rule, node pitch {h} mm)")
+    return None
+
+
+def call_rng_preserved(fn, *args, **kw):
+    """ion_gym/physics/numba_rng.py:20-28 saves and restores numba's
+    per-thread generator around a kernel call.  The engine keeps no
+    global generator (every ion's draws come from its own counter-based
+    stream), so there is nothing to save: the call is made as is."""
+    return fn(*args, **kw)
