@@ -25,6 +25,7 @@ from .collision3d import gas_kernel_scalars
 from .constants import E_CHG, KG_AMU
 from .drift_extension import extend_ballistic, xyz_edge_fields, xyz_escape_face
 from .ion_envelope import birth_splat, make_summary, per_ion, xyz_birth_refusal
+from .stations import all_planes
 
 # the optional channels the 3-D route fills, in the reference's
 # OPTIONAL_CHANNELS order (io/sim_spec.py:119-138)
@@ -37,7 +38,7 @@ _IMPOSSIBLE = (math.inf, -math.inf, math.inf, -math.inf)
 def kernel_planes(spec):
     """Stations, then the enabled bounds as whole-plane kills with fate 3
     (stations.py:199-226, tracer3d.pack_planes)."""
-    out = [tuple(p) for p in spec.planes]
+    out = all_planes(spec)
     flags, vals = spec.bounds.as_tuple()
     for k in range(6):
         if flags[k]:
@@ -108,7 +109,7 @@ def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
     h = float(pack["h_mm"])
     shape = metal.shape
     flags, _ = spec.bounds.as_tuple()
-    declares_downstream = bool(spec.planes) or any(flags)
+    declares_downstream = bool(all_planes(spec)) or any(flags)
     state = {}
 
     def _fields():

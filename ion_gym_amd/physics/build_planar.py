@@ -22,6 +22,7 @@ from .constants import E_CHG, KG_AMU
 from .drift_extension import edge_field_max, extend_ballistic
 from .ion_envelope import (birth_splat, make_summary, per_ion,
                            planar_birth_refusal)
+from .stations import station_planes
 
 
 def planar_edge_fields(pack):
@@ -91,8 +92,8 @@ def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
     # axis and the window over its two transverse axes shifted by the
     # anchor (stations.py:94-130), z carrying none
     off = (anx, any_, 0.0)
-    planes = []
-    for (a, v, sgn, win, kind) in spec.planes:
+    planes = station_planes(spec, off)
+    for (a, v, sgn, win, kind) in (spec.planes or []):
         a = int(a)
         t1, t2 = [q for q in range(3) if q != a]
         w = (win[0] - off[t1], win[1] - off[t1], win[2] - off[t2],

@@ -20,6 +20,7 @@ from .constants import E_CHG, KG_AMU
 from .drift_extension import rz_edge_fields, rz_extend
 from .ion_envelope import (birth_splat, make_summary, per_ion,
                            rz_birth_refusal)
+from .stations import all_planes
 
 _ENGINES = {}
 _ENGINES_LOCK = threading.Lock()
@@ -77,7 +78,7 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
     n_rows = min(max(n_rows, 16), 4_000_000)
     flight = Flight(dt, t_max, rec_every=rec_every, rec_rows=n_rows,
                     channels=chans, gas=gas,
-                    bounds=spec.bounds.as_tuple(), planes=spec.planes)
+                    bounds=spec.bounds.as_tuple(), planes=all_planes(spec))
     births = np.asarray(births, np.float64)
     seed_base = spec.source.seed if spec.source.seed is not None else 0
     state = {}

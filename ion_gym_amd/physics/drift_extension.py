@@ -21,6 +21,8 @@ import math
 
 import numpy as np
 
+from .stations import all_planes
+
 EDGE_FIELD_MAX_V_PER_MM = 1e-3
 
 _BASE = ("t", "x", "y", "z", "vx", "vy", "vz")
@@ -137,7 +139,7 @@ def extend_ballistic(traj, col_names, kind, spec, *, edge_e_vpermm,
         int(p[0]) == face_axis
         and ((face_side == "hi" and float(p[1]) >= pos[face_axis])
              or (face_side == "lo" and float(p[1]) <= pos[face_axis]))
-        for p in (spec.planes or []))
+        for p in all_planes(spec))
     if not own_bound and not own_station:
         return traj, kind, None
     tb, bface = math.inf, None
@@ -156,7 +158,7 @@ def extend_ballistic(traj, col_names, kind, spec, *, edge_e_vpermm,
             tb, bface = dt, k
     st_dts = []
     ts, ts_kind = math.inf, None
-    for pl in (spec.planes or []):
+    for pl in all_planes(spec):
         a = int(pl[0])
         if vel[a] == 0.0:
             continue

@@ -173,3 +173,6 @@ class FlightSpec:
     # the 3-D periodic transporter (sim_spec transporter; tracer3d.py:
     # 722-735): dict(axis, accept_mm, emit_mm, direction, max_passes)
     transporter: Optional[dict] = None
+    # declared stations (ion_gym_amd.physics.stations.StationSpec): the
+    # kernel-reaching ones become planes before ``planes``
+    stations: List = field(default_factory=list)

@@ -126,3 +126,37 @@ def test_birth_refusal_and_drift_extension():
         assert tr[-2, 3] > 20.0 and tr[-2, 3] < 25.0
     tr, s = out[2]
     assert s["kind"] == 0 and s.get("born_in_metal") and tr.shape[0] == 1
+
+
+def test_declared_stations_on_a_field_free_box():
+    """A detect-splat station at z = 10 mm (window x in 1-4 mm) absorbs
+    the ions crossing inside its window, with fate 6 at the exact
+    crossing; a pass station at z = 5 mm is found on the records by
+    first_detection at the straight-line crossing."""
+    from ion_gym_amd.physics.build_3d import make_3d_fly_fn
+    from ion_gym_amd.physics.stations import StationSpec, first_detection
+    shape = (11, 11, 41)
+    z = np.zeros(shape)
+    pack = dict(EAx=z, EAy=z, EAz=z, ExK=np.zeros((0,) + shape),
+                EyK=np.zeros((0,) + shape), EzK=np.zeros((0,) + shape),
+                ele=np.zeros(shape, bool), h_mm=0.5,
+                ch_kind=np.zeros(0, np.int64), ch_om=np.zeros(0),
+                ch_ph=np.zeros(0), ch_amp=np.zeros(0), ch_off=np.zeros(0),
+                ch_duty=np.zeros(0))
+    births = np.array([[2.0, 2.5, 1.0, 0.0, 0.0, 1.0, 0.0],
+                       [4.5, 2.5, 1.0, 0.0, 0.0, 1.1, 0.2]])
+    spec = FlightSpec(
+        source=SourceSpec(n_ions=2, mz_list=[150.0], seed=4),
+        integration=IntegrationSpec(dt_ns=3.0, t_max_us=30.0, rec_every=7),
+        stations=[StationSpec(name="det", kind="detect", on_hit="splat",
+                              axis="z", pos_mm=10.0, window={"x": [1, 4]}),
+                  StationSpec(name="log", kind="detect", on_hit="pass",
+                              axis="z", pos_mm=5.0)])
+    fly_fn, cols = make_3d_fly_fn(pack, births, spec)
+    (t0, s0), (t1, s1) = fly_fn.batch(range(2))
+    assert s0["kind"] == 6 and abs(s0["z_end"] - 10.0) < 1e-9
+    assert abs(s0["tof"] - 9.0) < 1e-6
+    assert s1["kind"] == 1        # outside the window: on to the box end
+    for tr, b in ((t0, births[0]), (t1, births[1])):
+        h = first_detection(tr, cols, spec, name="log")
+        assert abs(h["t_us"] - (b[6] + (5.0 - b[2]) / b[5])) < 1e-9
