@@ -39,11 +39,13 @@ HBM_PEAK_GBS = 8000.0
 
 
 def gather_bytes_per_ion_step(n_sets, value_bytes):
-    """Algorithmic gather bytes of one r-z ion-step of this engine: one
-    bilinear field evaluation per step (the second evaluation of a step is
-    reused as the first of the next, see ig_rz_kernel.h) of 2 components
-    x n_sets sets x 4 corners, plus one metal-mask byte."""
-    return 1 * (2 * n_sets) * 4 * value_bytes + 1
+    """Algorithmic gather bytes of one r-z ion-step of the reference's
+    kick-drift-kick step (tracer_rz.py:207-440): two bilinear field
+    evaluations of 2 components x n_sets sets x 4 corners, plus one
+    metal-mask byte (E C c s + m).  The engine's kernel gathers half of
+    it: the second evaluation of a step is the first of the next (same
+    point, same lab time) and is kept in registers."""
+    return 2 * (2 * n_sets) * 4 * value_bytes + 1
 
 
 def state_bytes_per_ion():
