@@ -52,7 +52,7 @@ def candidates(precision, gas_model):
     """The launch settings worth measuring for a precision and gas."""
     hs = gas_model == L.GAS_HS
     occs = OCCUPANCIES if hs else (1,)
-    cbs = COLL_BATCHES if (hs and precision != "f32") else (0,)
+    cbs = COLL_BATCHES if hs else (0,)
     return [dict(block=b, occupancy=o, coll_batch=c)
             for b, o, c in itertools.product(BLOCKS, occs, cbs)]
 

@@ -457,15 +457,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       }
     }
     if constexpr (kFlat) {
-      // one partner try per pass for every lane whose collision is due
-      if (stage == kCollide) {
-        const int rc = hs_try_flat(rng, hst, hz, vx, vy, vz, p, m_ion);
-        if (rc > 0) {
-          ncol += 1;
-          stage = kPost;
-        } else if (rc < 0) {
-          fate = -3; // no partner accepted: refused, as the reference raises
-          stage = kDone;
+      // one partner try per pass for every lane whose collision is due,
+      // once coll_batch of them are parked (or no lane can step)
+      const uint64_t parked = __ballot(stage == kCollide);
+      if (parked) {
+        const uint64_t moving = __ballot(stage == kStep || stage == kPost);
+        if (__popcll(parked) >= coll_batch || moving == 0) {
+          if (stage == kCollide) {
+            const int rc = hs_try_flat(rng, hst, hz, vx, vy, vz, p, m_ion);
+            if (rc > 0) {
+              ncol += 1;
+              stage = kPost;
+            } else if (rc < 0) {
+              fate = -3; // no partner accepted: refused, as the reference raises
+              stage = kDone;
+            }
+          }
         }
       }
     } else if constexpr (GAS == IG_GAS_HS) {

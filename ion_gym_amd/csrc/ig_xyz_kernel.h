@@ -392,18 +392,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       }
     }
     if constexpr (kFlat) {
-      if (stage == kCollide) {
-        double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
-        const int rc = hs_try_flat(rng, hst, hz, sx, sy, sz, p, m_amu);
-        if (rc > 0) {
-          vx = sx * 1e3;
-          vy = sy * 1e3;
-          vz = sz * 1e3;
-          ncol += 1;
-          stage = kPost;
-        } else if (rc < 0) {
-          fate = -3;
-          stage = kDone;
+      const uint64_t parked = __ballot(stage == kCollide);
+      if (parked) {
+        const uint64_t moving = __ballot(stage == kStep || stage == kPost);
+        if (__popcll(parked) >= coll_batch || moving == 0) {
+          if (stage == kCollide) {
+            double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
+            const int rc = hs_try_flat(rng, hst, hz, sx, sy, sz, p, m_amu);
+            if (rc > 0) {
+              vx = sx * 1e3;
+              vy = sy * 1e3;
+              vz = sz * 1e3;
+              ncol += 1;
+              stage = kPost;
+            } else if (rc < 0) {
+              fate = -3;
+              stage = kDone;
+            }
+          }
         }
       }
     } else if constexpr (GAS == IG_GAS_HS) {
