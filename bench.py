@@ -53,7 +53,7 @@ def state_bytes_per_ion():
     return 2 * (8 * 16 + 4 * 5 + 8 * 4)
 
 
-def run_gpu(cfg, precision, warmup, steps, window, engine, compact=True):
+def run_gpu(cfg, precision, warmup, steps, window, engine, compact=3):
     from ion_gym_amd.engine import Flight
     model = cfg["model"]
     fp = engine.upload_rz(model.pack(), precision)
@@ -99,7 +99,7 @@ def cpu_baseline(cfg, window, n_sample, threads):
 
 
 def run_serving(eng, n_req, load, flight_us, window, capacity, seed,
-                compact=True):
+                compact=3):
     """The serving leg: requests on the SLIM traveling-wave board (11 FP32
     field sets, N2 4 Torr hard sphere) arrive as a Poisson stream and are
     served by one FlightServer (continuous batching over one resident
@@ -224,9 +224,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=32000)
     ap.add_argument("--cpu-window", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compact", type=int, default=1, choices=(0, 1),
-                    help="pack the live ions into a dense lane list before "
-                         "each window (results are identical either way)")
+    ap.add_argument("--compact", type=int, default=3, choices=(0, 1, 2, 3),
+                    help="live-ion lane list before each window: 0 none, 1 "
+                         "slot order, 2 cell order, 3 the engine's choice "
+                         "per route (results are identical either way)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32-vs-FP64 quadrupole error leg")
     ap.add_argument("--precision-ions", type=int, default=100_000)
@@ -253,7 +254,7 @@ def main():
     print(f"[bench] timing {args.precision}: warmup {args.warmup}, steps "
           f"{args.steps}, window {args.window}", file=sys.stderr, flush=True)
     main_run = run_gpu(cfg, args.precision, args.warmup, args.steps,
-                       args.window, eng, bool(args.compact))
+                       args.window, eng, int(args.compact))
     value = main_run["ion_steps"] / main_run["wall_s"]
     ms_per_step = main_run["wall_s"] / args.steps * 1e3
     fp64 = None
@@ -261,7 +262,7 @@ def main():
         print("[bench] timing the plain FP64 kernel", file=sys.stderr,
               flush=True)
         r64 = run_gpu(cfg, "f64", 1, max(2, args.steps // 4), args.window,
-                      eng, bool(args.compact))
+                      eng, int(args.compact))
         fp64 = r64["ion_steps"] / r64["wall_s"]
     # roofline of the flight kernel: algorithmic bytes per launch / mean
     # HIP-event launch time on the engine's stream
@@ -308,7 +309,7 @@ def main():
         serving = run_serving(eng, args.serve_requests, args.serve_load,
                               args.serve_flight_us, args.serve_window,
                               args.serve_capacity, seed=11,
-                              compact=bool(args.compact))
+                              compact=int(args.compact))
     line = {
         "metric": "ion-steps/s at 1M ions (3-D funnel+HS), FP32 vs FP64 "
                   "plain; p50/p99 latency",
@@ -329,7 +330,7 @@ def main():
                    "field_precision": args.precision,
                    "state_precision": "f64", "dt_ns": cfg["dt"] * 1e3,
                    "live_ions_end": main_run["live_end"],
-                   "compact": bool(args.compact),
+                   "compact": int(args.compact),
                    "field_bytes": main_run["field_bytes"]},
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,

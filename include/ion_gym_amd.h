@@ -207,7 +207,12 @@ typedef struct {
   int32_t compact;     /* 1: gather the ensemble's live ions into a dense
                           index list (one wave ballot each) before the
                           window, so dead and empty slots take no lanes;
-                          results do not depend on it */
+                          2: the same list in cell order (a counting sort
+                          by the grid cell each ion sits in), so the
+                          lanes of a wave gather neighbouring field
+                          records; 3: the engine's choice per route (cell
+                          order for XYZ, slot order otherwise); results
+                          do not depend on it */
   /* the periodic transporter of the 3-D RK4 flight (tracer3d.py:433-458,
      fly3d's transporter argument, :722-735): tp_axis 0 = none, else
      1 + the axis (1 x, 2 y, 3 z); an ion crossing the accept plane in tp_direction within a

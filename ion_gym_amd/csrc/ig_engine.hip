@@ -146,6 +146,7 @@ struct Ensemble {
   DevBuf sums;          // live/steps reduction (device)
   DevBuf live_idx;      // compacted live-ion list (int32, capacity)
   DevBuf live_cnt;      // its length (uint32)
+  DevBuf live_key, live_sorted, live_hist; // its cell order (compact 2)
   int32_t *tune_idx = nullptr;
   std::vector<std::unique_ptr<DevBuf>> bufs;
   EnsK k{};
@@ -838,7 +839,8 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
   k.occupancy = p->occupancy;
   if (p->block != 0 && (p->block < 64 || p->block > 256 || p->block % 64 != 0))
     return fail(IG_E_ARG, "ig_fly: block must be 0 (default), 64, 128, 192 or 256");
-  if (p->compact != 0 && p->compact != 1) return fail(IG_E_ARG, "ig_fly: compact must be 0 or 1");
+  if (p->compact < 0 || p->compact > 3)
+    return fail(IG_E_ARG, "ig_fly: compact must be 0, 1, 2 or 3");
   k.tp_col = -1;
   if (p->tp_axis != 0) {
     if (p->tp_axis < 1 || p->tp_axis > 3)
@@ -879,29 +881,166 @@ __global__ void zero_u32_kernel(uint32_t *p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
+// Cell order of the live list (compact 2): a counting sort of the live
+// ions by the grid cell they sit in, so that the lanes of a wave gather
+// from the same or neighbouring field records.  The key is the linear
+// cell index (axial, radial for r-z; x, y for planar; x, y, z for 3-D,
+// the last fastest) modulo kMaxBlocks: exact cell order wherever the
+// occupied cells span fewer than kMaxBlocks indices, and a few far-apart
+// cells sharing a bucket elsewhere.  Within a cell the order is whatever
+// the atomics give (an ion's flight does not depend on its lane).
+constexpr int kMaxBlocks = 1 << 18;
+
+struct BlockGrid {
+  int32_t route, dims;
+  int64_t n[3];       // nodes per axis (r-z: the half grid)
+  double inv_h;       // node coordinates of the pack (1 / pitch)
+};
+
+static BlockGrid block_grid(const FieldsK &f) {
+  BlockGrid g{};
+  g.route = f.route;
+  g.inv_h = 1.0 / f.h;
+  g.n[0] = f.n0;
+  g.n[1] = f.route == IG_ROUTE_RZ ? (f.n1 + 1) / 2 : f.n1;
+  g.n[2] = f.route == IG_ROUTE_XYZ ? f.n2 : 1;
+  g.dims = f.route == IG_ROUTE_XYZ ? 3 : 2;
+  return g;
+}
+
+__device__ __forceinline__ uint32_t block_of(const BlockGrid &g, double x, double y, double z) {
+  double c[3];
+  c[0] = x * g.inv_h;
+  if (g.route == IG_ROUTE_RZ) {
+    c[1] = sqrt(y * y + z * z) * g.inv_h;
+    c[2] = 0.0;
+  } else {
+    c[1] = y * g.inv_h;
+    c[2] = g.dims == 3 ? z * g.inv_h : 0.0;
+  }
+  int64_t b[3];
+  for (int a = 0; a < 3; ++a) {
+    // outside the grid (or not a number): the nearest cell
+    const double v = c[a] >= 0.0 ? c[a] : 0.0;
+    const int64_t q = v < 4.0e18 ? (int64_t)v : g.n[a] - 1;
+    b[a] = q < g.n[a] ? q : g.n[a] - 1;
+  }
+  const int64_t cell = (b[0] * g.n[1] + b[1]) * g.n[2] + b[2];
+  return (uint32_t)(cell & (int64_t)(kMaxBlocks - 1));
+}
+
+__global__ void zero_blocks_kernel(uint32_t *__restrict__ hist, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) hist[i] = 0u;
+}
+
+__global__ void block_hist_kernel(const BlockGrid g, const EnsK e, const int32_t *__restrict__ idx,
+                                  const uint32_t *__restrict__ count, uint32_t *__restrict__ key,
+                                  uint32_t *__restrict__ hist) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)*count) return;
+  const int32_t i = idx[t];
+  const uint32_t k = block_of(g, e.x[i], e.y[i], e.z[i]);
+  key[t] = k;
+  atomicAdd(&hist[k], 1u);
+}
+
+// exclusive scan of n <= kMaxBlocks counts in one 1024-lane workgroup:
+// each lane sums a contiguous chunk, the chunk sums are scanned in LDS,
+// and each lane writes its chunk's offsets
+__global__ __launch_bounds__(1024) void block_scan_kernel(uint32_t *__restrict__ hist, int64_t n) {
+  __shared__ uint32_t part[1024];
+  const int tid = threadIdx.x;
+  const int64_t chunk = (n + 1023) / 1024;
+  const int64_t lo = tid * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  uint32_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += hist[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = tid >= off ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[tid] - s; // exclusive prefix of this chunk
+  for (int64_t i = lo; i < hi; ++i) {
+    const uint32_t c = hist[i];
+    hist[i] = run;
+    run += c;
+  }
+}
+
+__global__ void block_scatter_kernel(const int32_t *__restrict__ idx,
+                                     const uint32_t *__restrict__ count,
+                                     const uint32_t *__restrict__ key,
+                                     uint32_t *__restrict__ offsets, int32_t *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)*count) return;
+  out[atomicAdd(&offsets[key[t]], 1u)] = idx[t];
+}
+
 // the compaction buffers, sized for the ensemble's capacity (allocated
 // once, before any capture)
-static int live_bufs(Ensemble *E) {
+static int live_bufs(Ensemble *E, bool ordered) {
+  const size_t cap = (size_t)std::max<int64_t>(E->cap, 1);
   if (!E->live_idx.p)
-    if (int rc = dev_alloc(E->live_idx, (size_t)std::max<int64_t>(E->cap, 1) * sizeof(int32_t)))
-      return rc;
+    if (int rc = dev_alloc(E->live_idx, cap * sizeof(int32_t))) return rc;
   if (!E->live_cnt.p)
     if (int rc = dev_alloc(E->live_cnt, sizeof(uint32_t))) return rc;
+  if (ordered) {
+    if (!E->live_key.p)
+      if (int rc = dev_alloc(E->live_key, cap * sizeof(uint32_t))) return rc;
+    if (!E->live_sorted.p)
+      if (int rc = dev_alloc(E->live_sorted, cap * sizeof(int32_t))) return rc;
+    if (!E->live_hist.p)
+      if (int rc = dev_alloc(E->live_hist, (size_t)kMaxBlocks * sizeof(uint32_t))) return rc;
+  }
   return 0;
 }
 
 // the kernel arguments of one window: with compaction, the live list is
-// rebuilt on the stream first (kernels only, so a graph may capture it)
-static int window_ens(ig_engine *h, Ensemble *E, const ig_flight_params *p, EnsK &ek) {
+// rebuilt on the stream first, and with compact 2 put in cell order
+// (kernels only, so a graph may capture it)
+// the live-list mode of a flight: compact 3 lets the engine choose per
+// route -- cell order for the 3-D route, whose field gathers reach HBM and
+// gain most from neighbouring lanes sharing records (1.8x on the SLIM
+// board), slot order elsewhere, where the pack is cache-resident and the
+// lanes' coalesced state loads matter more (cell order costs 3 % on the
+// funnel)
+static int compact_mode(const ig_flight_params *p, const FieldsK &f) {
+  if (p->compact != 3) return p->compact;
+  return f.route == IG_ROUTE_XYZ ? 2 : 1;
+}
+
+static int window_ens(ig_engine *h, Ensemble *E, const FieldsK &f, const ig_flight_params *p,
+                      EnsK &ek) {
   ek = E->k;
-  if (!p->compact || E->n == 0) return 0;
-  if (int rc = live_bufs(E)) return rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, (uint32_t *)E->live_cnt.p);
-  hipLaunchKernelGGL(compact_live_kernel, dim3((unsigned)((E->n + 255) / 256)), dim3(256), 0,
-                     h->stream, E->k.fate, E->n, (int32_t *)E->live_idx.p,
-                     (uint32_t *)E->live_cnt.p);
+  const int mode = compact_mode(p, f);
+  if (!mode || E->n == 0) return 0;
+  const bool ordered = mode == 2;
+  if (int rc = live_bufs(E, ordered)) return rc;
+  const unsigned grid = (unsigned)((E->n + 255) / 256);
+  uint32_t *cnt = (uint32_t *)E->live_cnt.p;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, cnt);
+  hipLaunchKernelGGL(compact_live_kernel, dim3(grid), dim3(256), 0, h->stream, E->k.fate, E->n,
+                     (int32_t *)E->live_idx.p, cnt);
   ek.idx = (const int32_t *)E->live_idx.p;
-  ek.n_live = (const uint32_t *)E->live_cnt.p;
+  ek.n_live = (const uint32_t *)cnt;
+  if (ordered) {
+    const BlockGrid g = block_grid(f);
+    const int64_t nb = kMaxBlocks;
+    uint32_t *hist = (uint32_t *)E->live_hist.p, *key = (uint32_t *)E->live_key.p;
+    hipLaunchKernelGGL(zero_blocks_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                       h->stream, hist, nb);
+    hipLaunchKernelGGL(block_hist_kernel, dim3(grid), dim3(256), 0, h->stream, g, E->k,
+                       (const int32_t *)E->live_idx.p, (const uint32_t *)cnt, key, hist);
+    hipLaunchKernelGGL(block_scan_kernel, dim3(1), dim3(1024), 0, h->stream, hist, nb);
+    hipLaunchKernelGGL(block_scatter_kernel, dim3(grid), dim3(256), 0, h->stream,
+                       (const int32_t *)E->live_idx.p, (const uint32_t *)cnt,
+                       (const uint32_t *)key, hist, (int32_t *)E->live_sorted.p);
+    ek.idx = (const int32_t *)E->live_sorted.p;
+  }
   return 0;
 }
 
@@ -969,7 +1108,7 @@ int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensembl
   if (int rc = prepare_flight(h, fid, p, eid, "ig_fly", f, k, E)) return rc;
   HIPX(hipEventRecord(h->ev0, h->stream));
   EnsK ek;
-  if (int rc = window_ens(h, E, p, ek)) return rc;
+  if (int rc = window_ens(h, E, *f, p, ek)) return rc;
   int rc = launch_route(*f, k, ek, E->n, p->block > 0 ? p->block : h->block, h->stream);
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
@@ -986,8 +1125,8 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
   Ensemble *E = nullptr;
   if (int rc = prepare_flight(h, fid, p, eid, "ig_fly_graph", f, k, E)) return rc;
   const int block = p->block > 0 ? p->block : h->block;
-  if (p->compact)
-    if (int rc = live_bufs(E)) return rc;
+  if (compact_mode(p, *f))
+    if (int rc = live_bufs(E, compact_mode(p, *f) == 2)) return rc;
   // the graph is a function of the exact kernel arguments and launch shape
   std::vector<uint8_t> key(sizeof(FieldsK) + sizeof(FlightK) + sizeof(EnsK) + 3 * 8);
   {
@@ -998,7 +1137,7 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
     q += sizeof(FlightK);
     std::memcpy(q, &E->k, sizeof(EnsK));
     q += sizeof(EnsK);
-    const int64_t tail[3] = {E->n, block, n_windows + (p->compact ? (1ll << 40) : 0)};
+    const int64_t tail[3] = {E->n, block, n_windows + ((int64_t)p->compact << 40)};
     std::memcpy(q, tail, sizeof(tail));
   }
   hipGraphExec_t exec = nullptr;
@@ -1010,7 +1149,7 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
     int rc = 0;
     for (int32_t wdx = 0; wdx < n_windows && rc == 0; ++wdx) {
       EnsK ek;
-      rc = window_ens(h, E, p, ek);
+      rc = window_ens(h, E, *f, p, ek);
       if (rc == 0) rc = launch_route(*f, k, ek, E->n, block, h->stream);
     }
     hipError_t ce = hipStreamEndCapture(h->stream, &graph);

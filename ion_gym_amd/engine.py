@@ -207,7 +207,7 @@ class Flight:
 
     def __init__(self, dt, t_max, *, max_steps=0, rec_every=1, rec_rows=0,
                  channels=(), gas=None, bounds=None, planes=(), rng_key=0,
-                 coll_batch=0, occupancy=0, block=0, compact=True,
+                 coll_batch=0, occupancy=0, block=0, compact=3,
                  transporter=None):
         self.dt = float(dt)
         self.t_max = float(t_max)
@@ -222,7 +222,9 @@ class Flight:
         self.coll_batch = int(coll_batch)
         self.occupancy = int(occupancy)
         self.block = int(block)
-        self.compact = bool(compact)
+        # live-ion list before each window: 0 none, 1 in slot order, 2 in
+        # cell order, 3 the engine's choice per route
+        self.compact = int(compact)
         # the 3-D periodic transporter (tracer3d.py:722-735): a dict with
         # axis ('x'|'y'|'z'), accept_mm, emit_mm, direction (default -1)
         # and max_passes (default 10**9), or None
@@ -236,7 +238,7 @@ class Flight:
         for k, v in kw.items():
             if k not in ("block", "occupancy", "coll_batch", "compact"):
                 raise ValueError(f"not a launch setting: {k}")
-            setattr(f, k, bool(v) if k == "compact" else int(v))
+            setattr(f, k, int(v))
         return f
 
     def struct(self, keep):

@@ -1,7 +1,8 @@
 """Live-ion compaction (ig_flight_params.compact) against the same windows
 without it: a wave-ballot kernel packs the ensemble's flying ions into a
-dense index list before each window, so lanes map to other ions than in
-the plain launch.  An ion's flight does not depend on its lane, so every
+dense index list before each window (mode 1), optionally put in cell
+order by a counting sort over grid blocks (mode 2), so lanes map to other
+ions than in the plain launch.  An ion's flight does not depend on its lane, so every
 field of every ion stays bit-identical after every window -- on r-z with
 hard-sphere gas (FP64 and the FP32 basis), on the 3-D quadrupole with
 gas, on the planar guide, in a partly filled ensemble, and inside a
@@ -20,12 +21,12 @@ def _same(a, b):
         assert np.array_equal(a[k], b[k]), k
 
 
-def _windows(engine, fp, fl, ions, cap, n_win, graph_every=0):
+def _windows(engine, fp, fl, ions, cap, n_win, graph_every=0, mode=1):
     plain = engine.ensemble(cap)
     comp = engine.ensemble(cap)
     plain.load(ions)
     comp.load(ions)
-    flc = fl.launch(compact=True)
+    flc = fl.launch(compact=mode)
     fl = fl.launch(compact=False)
     lives = []
     for w in range(n_win):
@@ -58,13 +59,14 @@ def _einzel_gas(n):
     return c
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_compact_rz_hard_sphere_bitwise(engine, prec):
+@pytest.mark.parametrize("prec,mode", [("f64", 1), ("f32", 1), ("f64", 2),
+                                       ("f32", 2)])
+def test_compact_rz_hard_sphere_bitwise(engine, prec, mode):
     from ion_gym_amd.engine import Flight
     c = _einzel_gas(6000)
     fp = engine.upload_rz(c["model"].pack(), prec)
     fl = Flight(c["dt"], c["t_max"], gas=c["gas"], max_steps=500)
-    lives = _windows(engine, fp, fl, c["ions"], 6000, 16)
+    lives = _windows(engine, fp, fl, c["ions"], 6000, 16, mode=mode)
     # ions are lost during the windows, so the lists really were compacted
     assert lives[-1] < lives[0] / 2
     fp.release()
@@ -76,7 +78,8 @@ def test_compact_partly_filled_ensemble_and_graph(engine):
     fp = engine.upload_rz(c["model"].pack(), "f32")
     fl = Flight(c["dt"], c["t_max"], gas=c["gas"], max_steps=500)
     # capacity above the load: the empty slots are never listed
-    lives = _windows(engine, fp, fl, c["ions"], 5000, 16, graph_every=2)
+    lives = _windows(engine, fp, fl, c["ions"], 5000, 16, graph_every=2,
+                     mode=2)
     assert lives[-1] < lives[0] / 2
     fp.release()
 
@@ -95,6 +98,7 @@ def test_compact_3d_hard_sphere_bitwise(engine):
         fp = engine.upload_xyz(c["pack"], prec)
         fl = Flight(c["dt"], c["t_max"], gas=gas, max_steps=150)
         _windows(engine, fp, fl, ions, 2000, 6)
+        _windows(engine, fp, fl, ions, 2000, 6, mode=2)
         fp.release()
 
 
@@ -104,6 +108,6 @@ def test_compact_planar_bitwise(engine):
     c = configs.planar_quad(n_ions=3000, coarse=2, n_cycles=30, rf_v=700.0)
     fp = engine.upload_planar(c["pack"], "f64")
     fl = Flight(c["dt"], c["t_max"], max_steps=200)
-    lives = _windows(engine, fp, fl, c["ions"], 3000, 8)
+    lives = _windows(engine, fp, fl, c["ions"], 3000, 8, mode=2)
     assert lives[-1] < lives[0]
     fp.release()
