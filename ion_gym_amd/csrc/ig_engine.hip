@@ -67,9 +67,14 @@ constexpr int kDefaultCollBatch = 16;
 constexpr int kDefaultCollBatchFlat = 8;
 constexpr int kDefaultCollBatchFlat3D = 1;
 // default register-budget variant of the hard-sphere kernel per precision
-// (measured on the funnel workload: FP32 best at 3 waves/SIMD, FP64 and
+// and route (measured: the FP32 r-z and planar kernels on the funnel are
+// best at 3 waves/SIMD; the FP32 3-D kernel, whose step keeps its cell's
+// corner sums, at 2 on the SLIM board (3 spills inside the step); FP64 and
 // FP32-storage at the compiler's own budget)
-inline int default_occupancy(int prec) { return prec == IG_PREC_F32 ? 3 : 1; }
+inline int default_occupancy(int prec, int route) {
+  if (prec != IG_PREC_F32) return 1;
+  return route == IG_ROUTE_XYZ ? 2 : 3;
+}
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -688,13 +693,26 @@ int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_t
   return 0;
 }
 
+// live ions per tune slot: the lanes of a wave that share a tune add
+// once, led by their first lane (a served slab keeps each request's ions
+// contiguous, so a wave holds one or two tunes and the counters see one
+// atomic per wave instead of one per ion)
 __global__ void count_live_kernel(const int32_t *__restrict__ fate,
                                   const int32_t *__restrict__ tune, int64_t n, int32_t nt,
                                   unsigned long long *__restrict__ cnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t t = -1;
   if (i < n && fate[i] == IG_FATE_FLYING) {
-    const int32_t t = tune[i];
-    if (t >= 0 && t < nt) atomicAdd(&cnt[t], 1ull);
+    const int32_t v = tune[i];
+    if (v >= 0 && v < nt) t = v;
+  }
+  uint64_t todo = __ballot(t >= 0);
+  while (todo) {
+    const int lead = __ffsll((unsigned long long)todo) - 1;
+    const int32_t tl = __shfl(t, lead);
+    const uint64_t same = __ballot(t == tl);
+    if ((int)__lane_id() == lead) atomicAdd(&cnt[tl], (unsigned long long)__popcll(same));
+    todo &= ~same;
   }
 }
 
@@ -945,30 +963,55 @@ __global__ void block_hist_kernel(const BlockGrid g, const EnsK e, const int32_t
   atomicAdd(&hist[k], 1u);
 }
 
-// exclusive scan of n <= kMaxBlocks counts in one 1024-lane workgroup:
-// each lane sums a contiguous chunk, the chunk sums are scanned in LDS,
-// and each lane writes its chunk's offsets
-__global__ __launch_bounds__(1024) void block_scan_kernel(uint32_t *__restrict__ hist, int64_t n) {
-  __shared__ uint32_t part[1024];
-  const int tid = threadIdx.x;
-  const int64_t chunk = (n + 1023) / 1024;
-  const int64_t lo = tid * chunk, hi = lo + chunk < n ? lo + chunk : n;
-  uint32_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += hist[i];
-  part[tid] = s;
+// exclusive scan of the kMaxBlocks counts in two passes of kScanTiles
+// workgroups of 256 lanes, each lane holding four consecutive counts (one
+// 16-byte load): the tile sums, then each tile adds the sum of the tiles
+// before it to its own scan
+constexpr int kScanTile = 1024, kScanTiles = kMaxBlocks / kScanTile;
+static_assert(kScanTiles <= 256, "one lane per earlier tile");
+
+__device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[wv] = v;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = tid >= off ? part[tid - off] : 0u;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  const uint32_t tot = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return tot;
+}
+
+__global__ __launch_bounds__(256) void scan_sums_kernel(const uint32_t *__restrict__ hist,
+                                                        uint32_t *__restrict__ sums) {
+  __shared__ uint32_t red[4];
+  const uint4 v = reinterpret_cast<const uint4 *>(hist + (size_t)blockIdx.x * kScanTile)[threadIdx.x];
+  const uint32_t tot = block_sum_256(v.x + v.y + v.z + v.w, red);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void scan_apply_kernel(uint32_t *__restrict__ hist,
+                                                         const uint32_t *__restrict__ sums) {
+  __shared__ uint32_t red[4], wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const uint32_t base = block_sum_256(tid < (int)blockIdx.x ? sums[tid] : 0u, red);
+  uint4 *p = reinterpret_cast<uint4 *>(hist + (size_t)blockIdx.x * kScanTile) + tid;
+  const uint4 v = *p;
+  const uint32_t own = v.x + v.y + v.z + v.w;
+  // inclusive scan of the lanes' totals within the wave, then across waves
+  uint32_t inc = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o);
+    if (lane >= o) inc += u;
   }
-  uint32_t run = part[tid] - s; // exclusive prefix of this chunk
-  for (int64_t i = lo; i < hi; ++i) {
-    const uint32_t c = hist[i];
-    hist[i] = run;
-    run += c;
-  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t run = base + inc - own;
+  for (int k = 0; k < wv; ++k) run += wsum[k];
+  uint4 o;
+  o.x = run;
+  o.y = o.x + v.x;
+  o.z = o.y + v.y;
+  o.w = o.z + v.z;
+  *p = o;
 }
 
 __global__ void block_scatter_kernel(const int32_t *__restrict__ idx,
@@ -994,7 +1037,8 @@ static int live_bufs(Ensemble *E, bool ordered) {
     if (!E->live_sorted.p)
       if (int rc = dev_alloc(E->live_sorted, cap * sizeof(int32_t))) return rc;
     if (!E->live_hist.p)
-      if (int rc = dev_alloc(E->live_hist, (size_t)kMaxBlocks * sizeof(uint32_t))) return rc;
+      if (int rc = dev_alloc(E->live_hist, (size_t)(kMaxBlocks + kScanTiles) * sizeof(uint32_t)))
+        return rc;
   }
   return 0;
 }
@@ -1035,7 +1079,10 @@ static int window_ens(ig_engine *h, Ensemble *E, const FieldsK &f, const ig_flig
                        h->stream, hist, nb);
     hipLaunchKernelGGL(block_hist_kernel, dim3(grid), dim3(256), 0, h->stream, g, E->k,
                        (const int32_t *)E->live_idx.p, (const uint32_t *)cnt, key, hist);
-    hipLaunchKernelGGL(block_scan_kernel, dim3(1), dim3(1024), 0, h->stream, hist, nb);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(kScanTiles), dim3(256), 0, h->stream,
+                       (const uint32_t *)hist, hist + kMaxBlocks);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(kScanTiles), dim3(256), 0, h->stream, hist,
+                       (const uint32_t *)(hist + kMaxBlocks));
     hipLaunchKernelGGL(block_scatter_kernel, dim3(grid), dim3(256), 0, h->stream,
                        (const int32_t *)E->live_idx.p, (const uint32_t *)cnt,
                        (const uint32_t *)key, hist, (int32_t *)E->live_sorted.p);
@@ -1088,7 +1135,7 @@ static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params
     return fail(IG_E_ARG, w + ": records with the transporter need 11 columns (the pass count)");
   E->route = fk->route;
   E->clock = clk;
-  if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec);
+  if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec, fk->route);
   if (k.coll_batch == 0)
     k.coll_batch = fk->prec != IG_PREC_F32 ? kDefaultCollBatch
                    : (fk->route == IG_ROUTE_XYZ ? kDefaultCollBatchFlat3D : kDefaultCollBatchFlat);

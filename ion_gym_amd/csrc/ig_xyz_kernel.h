@@ -115,17 +115,16 @@ __device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy
   ez = e[2];
 }
 
-// The FP32-basis field: each corner record is read as float4s once and
-// its sets are composed with the per-set weights wt (waveform x scale)
-// before the trilinear blend, so an evaluation is 8 x 3 NS FMAs over
-// 8 x S/4 vector loads whatever the number of sets.  The clamps and cell
-// index are those of _tri in FP64; the sums are FP32 in a different order
-// than the reference's, within the FP32 mode's stated error.
-template <int NS>
-__device__ __forceinline__ void xyz_field_f32(const FieldsK &f, double gx, double gy, double gz,
-                                              const float (&wt)[NS], double &ex, double &ey,
-                                              double &ez) {
-  constexpr int S = RecLayout3<float, NS>::kStride, Q = S / 4;
+// The cell and fractions of _tri (tracer3d.py:121-151) in FP64 grid units:
+// coordinates clamped to the box (NaN passes the clamps and is kept
+// defined by the fmax), the index clamped to n - 2, the fractions rounded
+// to FP32 for the FP32-basis blend.
+struct Cell3 {
+  int32_t i, j, k; // a 3-D pack holds fewer than 2^31 nodes
+  float tx, ty, tz;
+};
+
+__device__ __forceinline__ Cell3 cell3(const FieldsK &f, double gx, double gy, double gz) {
   const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
   double x = gx, y = gy, z = gz;
   if (x < 0.0) x = 0.0;
@@ -138,9 +137,63 @@ __device__ __forceinline__ void xyz_field_f32(const FieldsK &f, double gx, doubl
   if (i > nx - 2) i = nx - 2;
   if (j > ny - 2) j = ny - 2;
   if (k > nz - 2) k = nz - 2;
-  const float tx = (float)(x - i), ty = (float)(y - j), tz = (float)(z - k);
+  Cell3 c;
+  c.i = (int32_t)i;
+  c.j = (int32_t)j;
+  c.k = (int32_t)k;
+  c.tx = (float)(x - i);
+  c.ty = (float)(y - j);
+  c.tz = (float)(z - k);
+  return c;
+}
+
+// trilinear weight of corner (i + di, j + dj, k + dk)
+__device__ __forceinline__ float corner_w(const Cell3 &c, int di, int dj, int dk) {
+  return (di ? c.tx : 1.0f - c.tx) * (dj ? c.ty : 1.0f - c.ty) * (dk ? c.tz : 1.0f - c.tz);
+}
+
+// One corner record (3 NS floats, read as float4s) composed with per-set
+// weights: acc[comp] = sum over sets of w[set] * E[set][comp], FMAs in
+// record order.
+template <int NS, int M>
+__device__ __forceinline__ void compose_rec(const float4 *rec, const float (*const (&w)[M])[NS],
+                                            float (&acc)[M][3]) {
+  constexpr int Q = RecLayout3<float, NS>::kStride / 4;
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0f;
+#pragma unroll
+  for (int q4 = 0; q4 < Q; ++q4) {
+    const float4 v = rec[q4];
+    const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int q = 4 * q4 + l;
+      if (q < 3 * NS) {
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          acc[m][q % 3] = __builtin_fmaf((*w[m])[q / 3], vv[l], acc[m][q % 3]);
+      }
+    }
+  }
+}
+
+// The FP32-basis field: each corner record is read as float4s once and
+// its sets are composed with the per-set weights wt (waveform x scale)
+// before the trilinear blend, so an evaluation is 8 x 3 NS FMAs over
+// 8 x S/4 vector loads whatever the number of sets.  The clamps and cell
+// index are those of _tri in FP64; the sums are FP32 in a different order
+// than the reference's, within the FP32 mode's stated error.
+template <int NS>
+__device__ __forceinline__ void xyz_field_f32(const FieldsK &f, double gx, double gy, double gz,
+                                              const float (&wt)[NS], double &ex, double &ey,
+                                              double &ez) {
+  constexpr int Q = RecLayout3<float, NS>::kStride / 4;
+  const int64_t ny = f.n1, nz = f.n2;
+  const Cell3 cl = cell3(f, gx, gy, gz);
   const float4 *g = reinterpret_cast<const float4 *>(f.grid);
-  const int64_t r00 = ((i * ny + j) * nz + k) * Q, dI = ny * nz * Q, dJ = nz * Q;
+  const int64_t r00 = (((int64_t)cl.i * ny + cl.j) * nz + cl.k) * Q, dI = ny * nz * Q,
+                dJ = nz * Q;
+  const float (*const ws[1])[NS] = {&wt};
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   // the four (i, j) columns one after another, each as its two z-adjacent
   // records (contiguous, 2 S floats): the loads of one column are in flight
@@ -148,34 +201,82 @@ __device__ __forceinline__ void xyz_field_f32(const FieldsK &f, double gx, doubl
 #pragma unroll 1
   for (int col = 0; col < 4; ++col) {
 #pragma unroll
-  for (int dk = 0; dk < 2; ++dk) {
-    const int di = (col >> 1) & 1, dj = col & 1;
-    const float wc =
-        (di ? tx : 1.0f - tx) * (dj ? ty : 1.0f - ty) * (dk ? tz : 1.0f - tz);
-    const float4 *rec = g + r00 + di * dI + dj * dJ + dk * Q;
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (int dk = 0; dk < 2; ++dk) {
+      const int di = (col >> 1) & 1, dj = col & 1;
+      const float wc = corner_w(cl, di, dj, dk);
+      float c[1][3];
+      compose_rec<NS, 1>(g + r00 + di * dI + dj * dJ + dk * Q, ws, c);
+      ax = __builtin_fmaf(wc, c[0][0], ax);
+      ay = __builtin_fmaf(wc, c[0][1], ay);
+      az = __builtin_fmaf(wc, c[0][2], az);
+    }
+  }
+  ex = ax;
+  ey = ay;
+  ez = az;
+}
+
+// One RK4 step's gathers from one cell (FP32 basis): the eight corner
+// records are read once and each is composed with the weights of the
+// step's three lab times t, t + h/2 and t + h.  The first stage's field,
+// at the cell's own point, is returned; the corner sums at t + h/2 and
+// t + h go to the lane's slice of LDS (kc[(6 n + m) bs], m = 0..2 for
+// t + h/2 and 3..5 for t + h, bs the block size, so a wave's accesses are
+// consecutive words) for the later stages, which blend them when their
+// point lies in the same cell (an ion moves a small fraction of a cell
+// per step).  The FMA chains and the corner order are those of
+// xyz_field_f32, so the field is bitwise the one it returns.
+constexpr int kCellLdsFloats = 48; // per lane: 8 corners x 2 times x 3
+template <int NS>
+__device__ __forceinline__ void xyz_cell_f32(const FieldsK &f, const Cell3 &cl,
+                                             const float (&w0)[NS], const float (&w1)[NS],
+                                             const float (&w2)[NS], double &ex, double &ey,
+                                             double &ez, float *kc, int bs) {
+  constexpr int Q = RecLayout3<float, NS>::kStride / 4;
+  const int64_t ny = f.n1, nz = f.n2;
+  const float4 *g = reinterpret_cast<const float4 *>(f.grid);
+  const int64_t r00 = (((int64_t)cl.i * ny + cl.j) * nz + cl.k) * Q, dI = ny * nz * Q,
+                dJ = nz * Q;
+  const float (*const ws[3])[NS] = {&w0, &w1, &w2};
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  // one corner record in flight at a time: the three weight sets and nine
+  // sums leave no room for a second
+#pragma unroll 1
+  for (int n = 0; n < 8; ++n) {
+    {
+      const int col = n >> 1, dk = n & 1, di = (col >> 1) & 1, dj = col & 1;
+      const float wc = corner_w(cl, di, dj, dk);
+      float c[3][3];
+      compose_rec<NS, 3>(g + r00 + di * dI + dj * dJ + dk * Q, ws, c);
+      ax = __builtin_fmaf(wc, c[0][0], ax);
+      ay = __builtin_fmaf(wc, c[0][1], ay);
+      az = __builtin_fmaf(wc, c[0][2], az);
 #pragma unroll
-    for (int q4 = 0; q4 < Q; ++q4) {
-      const float4 v = rec[q4];
-      const float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const int q = 4 * q4 + l;
-        if (q < 3 * NS) {
-          const int sidx = q / 3, comp = q % 3;
-          if (comp == 0)
-            cx = __builtin_fmaf(wt[sidx], vv[l], cx);
-          else if (comp == 1)
-            cy = __builtin_fmaf(wt[sidx], vv[l], cy);
-          else
-            cz = __builtin_fmaf(wt[sidx], vv[l], cz);
-        }
+      for (int m = 0; m < 3; ++m) {
+        kc[(6 * n + m) * bs] = c[1][m];
+        kc[(6 * n + 3 + m) * bs] = c[2][m];
       }
     }
-    ax = __builtin_fmaf(wc, cx, ax);
-    ay = __builtin_fmaf(wc, cy, ay);
-    az = __builtin_fmaf(wc, cz, az);
   }
+  ex = ax;
+  ey = ay;
+  ez = az;
+}
+
+// a later stage's field from the kept corner sums (t + h when last, else
+// t + h/2), blended at the stage's own fractions in xyz_field_f32's
+// corner order
+__device__ __forceinline__ void blend_cell_f32(const Cell3 &cl, const float *kc, int bs,
+                                               bool last, double &ex, double &ey, double &ez) {
+  const float *b = kc + (last ? 3 : 0) * bs;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int col = n >> 1, dk = n & 1, di = (col >> 1) & 1, dj = col & 1;
+    const float wc = corner_w(cl, di, dj, dk);
+    ax = __builtin_fmaf(wc, b[(6 * n + 0) * bs], ax);
+    ay = __builtin_fmaf(wc, b[(6 * n + 1) * bs], ay);
+    az = __builtin_fmaf(wc, b[(6 * n + 2) * bs], az);
   }
   ex = ax;
   ey = ay;
@@ -185,6 +286,11 @@ __device__ __forceinline__ void xyz_field_f32(const FieldsK &f, double gx, doubl
 template <int PREC, int NS, int GAS, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void xyz_fly_kernel(
     const FieldsK f, const FlightK p, const EnsK e) {
+  // FP32 mode: the lane's slice of the corner-sum cache, kCellLdsFloats
+  // words per lane of dynamic LDS
+  extern __shared__ float xyz_lds[];
+  float *const kc = xyz_lds + threadIdx.x;
+  const int bs = blockDim.x;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i;
   if (e.idx) {
@@ -238,33 +344,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
   constexpr int NW = NS > 1 ? NS - 1 : 1;
   double w[NW];
-  float wt[NS]; // FP32 mode: per-set composition weights
   double wtt = __builtin_nan("");
+  // FP32 mode: the per-set composition weights at lab time tt
+  auto wt_at = [&](double tt, float(&o)[NS]) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      double v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+      v = v * f.scale[q];
+      if (tscale) v = v * tscale[q];
+      o[q] = (float)v;
+    }
+  };
+  // FP64 modes: the waveform weights, kept while the lab time repeats
   auto weights = [&](double tt) {
     if (!(tt == wtt)) {
-      if constexpr (PREC == IG_PREC_F32) {
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-          double v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
-          v = v * f.scale[q];
-          if (tscale) v = v * tscale[q];
-          wt[q] = (float)v;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < NW; ++k)
-          if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
-      }
+      for (int k = 0; k < NW; ++k)
+        if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
       wtt = tt;
     }
   };
   auto efield = [&](double px, double py, double pz, double tt, double &a, double &b,
                     double &c) {
-    weights(tt);
-    if constexpr (PREC == IG_PREC_F32)
-      xyz_field_f32<NS>(f, px * inv_h, py * inv_h, pz * inv_h, wt, a, b, c);
-    else
+    if constexpr (PREC == IG_PREC_F32) {
+      // weights made afresh: FP32 steps gather through the cell cache, so
+      // this path (a stage that left its cell, a record row) is rare and
+      // keeps no weights live across steps
+      float wl[NS];
+      wt_at(tt, wl);
+      xyz_field_f32<NS>(f, px * inv_h, py * inv_h, pz * inv_h, wl, a, b, c);
+    } else {
+      weights(tt);
       xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
+    }
   };
   int32_t tpp = p.tp_col >= 0 ? e.tp_pass[i] : 0;
   // a record row; with the transporter the 11th column is the pass count
@@ -334,11 +446,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         double px = x, py = y, pz = z;
         double kx = vx, ky = vy, kz = vz;
         double skx = 0.0, sky = 0.0, skz = 0.0, sax = 0.0, say = 0.0, saz = 0.0;
+        // FP32 mode: the first stage reads the cell's corner records once
+        // for all three lab times; later stages in the same cell blend
+        // the kept sums, and only a stage that has left the cell gathers
+        Cell3 cl0{};
 #pragma unroll 1
         for (int st = 0; st < 4; ++st) {
           const double tt = st == 0 ? t_us : (st == 3 ? t_us + h_us : t_us + 0.5 * h_us);
           double ea, eb, ec;
-          efield(px, py, pz, tt, ea, eb, ec);
+          if constexpr (PREC == IG_PREC_F32) {
+            const Cell3 cl = cell3(f, px * inv_h, py * inv_h, pz * inv_h);
+            if (st == 0) {
+              float w0[NS], w1[NS], w2[NS];
+              wt_at(t_us, w0);
+              wt_at(t_us + 0.5 * h_us, w1);
+              wt_at(t_us + h_us, w2);
+              xyz_cell_f32<NS>(f, cl, w0, w1, w2, ea, eb, ec, kc, bs);
+              cl0 = cl;
+            } else if (cl.i == cl0.i && cl.j == cl0.j && cl.k == cl0.k) {
+              blend_cell_f32(cl, kc, bs, st == 3, ea, eb, ec);
+            } else {
+              efield(px, py, pz, tt, ea, eb, ec);
+            }
+          } else {
+            efield(px, py, pz, tt, ea, eb, ec);
+          }
           const double ax = qm * 1e3 * ea, ay = qm * 1e3 * eb, az = qm * 1e3 * ec;
           if (st == 0) {
             skx = kx;
@@ -788,19 +920,20 @@ template <int PREC, int NS>
 static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n_launch,
                          int block, hipStream_t st) {
   dim3 grid((unsigned)((n_launch + block - 1) / block));
+  const size_t lds = PREC == IG_PREC_F32 ? (size_t)kCellLdsFloats * sizeof(float) * block : 0;
   if (p.gas == IG_GAS_HS) {
     if (p.occupancy == 2)
-      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 2>), grid, dim3(block), 0, st, f, p,
-                         e);
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 2>), grid, dim3(block), lds, st, f,
+                         p, e);
     else if (p.occupancy == 3)
-      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 3>), grid, dim3(block), 0, st, f, p,
-                         e);
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 3>), grid, dim3(block), lds, st, f,
+                         p, e);
     else
-      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 1>), grid, dim3(block), 0, st, f, p,
-                         e);
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 1>), grid, dim3(block), lds, st, f,
+                         p, e);
   } else if (p.gas == IG_GAS_VACUUM) {
-    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 1>), grid, dim3(block), 0, st, f,
-                       p, e);
+    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 1>), grid, dim3(block), lds, st,
+                       f, p, e);
   } else {
     hipLaunchKernelGGL((xyz_sds_kernel<PREC, NS>), grid, dim3(block), 0, st, f, p, e);
   }
