@@ -473,6 +473,55 @@ int ig_ensemble_refuse_births(ig_engine *h, ig_ensemble_id id,
                               ig_fields_id f, int64_t offset, int64_t n,
                               int64_t *n_refused);
 
+/* ---- 3-D Laplace basis solve (upstream of the flight: the bases) ---- */
+
+#define IG_SOLVE_SOR 0    /* red-black SOR: ion_gym/physics/solver3d.py:250-293 */
+#define IG_SOLVE_VCYCLE 1 /* V-cycle multigrid: ion_gym/physics/multigrid3d.py:602-731 */
+#define IG_STENCIL_GHOST_LINEAR 0 /* solver3d.py:172-181 */
+#define IG_STENCIL_SW 1           /* Shortley-Weller, solver3d.py:182-190 */
+
+typedef struct {
+  int64_t n[3];          /* nx, ny, nz; arrays are C order, z fastest */
+  const uint8_t *fixed;  /* (nx, ny, nz) Dirichlet mask (host) */
+  const double *val;     /* potentials at the fixed nodes (host) */
+  const float *theta;    /* (6, nx, ny, nz) leg fractions in the order
+                            -x, +x, -y, +y, -z, +z, or NULL for all 1 */
+  const double *phi0;    /* initial guess or NULL for zeros */
+  int32_t method;        /* IG_SOLVE_* */
+  int32_t stencil;       /* IG_STENCIL_* */
+  double tol;            /* volts */
+  double omega;          /* SOR relaxation factor (solver3d.optimal_omega) */
+  int64_t max_sweeps;    /* SOR: red-black sweep pairs allowed */
+  int32_t max_cycles;    /* V-cycle: cycles allowed (reference 60) */
+  int32_t nu1, nu2;      /* V-cycle: pre/post smoothing pairs (3, 3) */
+  int32_t sym_axes;      /* V-cycle: bit a projects each cycle's iterate
+                            onto the mirror-symmetric subspace of axis a;
+                            the caller has verified the symmetry */
+} ig_solve3d_desc;
+
+typedef struct {
+  int64_t sweeps;  /* red-black pairs on the fine grid */
+  double delta;    /* SOR: the last pair's max |update|; V-cycle: the last
+                      cycle's max |change| */
+  int32_t status;  /* 0 converged (delta < tol), 1 the sweep or cycle
+                      limit was reached, 2 the V-cycle stagnated */
+} ig_solve3d_result;
+
+/* Solve Laplace for one basis and write the final iterate to phi (host,
+ * n nodes).  SOR is solve3d: pairs of colour sweeps (node colour
+ * (i + j + k) & 1, even reflection at every outer face) until a pair's
+ * max |update| < tol.  VCYCLE is solve3d_vmg: per cycle nu1 pairs of
+ * Gauss-Seidel sweeps of the same stencil, a plain 7-point error equation
+ * on a hierarchy of 2x-coarsened grids (fixed where any fine node of the
+ * 2x2x2 block is) solved to 5 % of its right-hand side
+ * (multigrid3d.py:553-599), its prolongation added, nu2 pairs, the
+ * symmetric projection, and convergence when the cycle moved no node by
+ * tol.  Replaces solve3d / solve3d_vmg under solve_bases and
+ * solve_bases_mg (solver3d.py:296-320, multigrid3d.py:233-302); the
+ * caller raises the reference's SolveNotConverged on status != 0. */
+int ig_solve3d(ig_engine *h, const ig_solve3d_desc *d, double *phi,
+               ig_solve3d_result *res);
+
 #ifdef __cplusplus
 }
 #endif

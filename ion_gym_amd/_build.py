@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 SOURCES = ("ig_engine.hip", "ig_rz_f64.hip", "ig_rz_f32s.hip", "ig_rz_f32.hip",
            "ig_xyz_f64.hip", "ig_xyz_f32s.hip", "ig_xyz_f32.hip",
            "ig_planar_f64.hip", "ig_planar_f32s.hip", "ig_planar_f32.hip",
-           "ig_births.hip")
+           "ig_births.hip", "ig_solve.hip")
 ARCH = os.environ.get("ION_GYM_AMD_ARCH", "gfx950")
 
 

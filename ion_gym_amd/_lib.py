@@ -122,6 +122,26 @@ class BirthsDesc(C.Structure):
                 ("zig_k", _u64p), ("zig_w", _dp), ("zig_f", _dp)]
 
 
+SOLVE_SOR, SOLVE_VCYCLE = 0, 1
+STENCIL_GHOST_LINEAR, STENCIL_SW = 0, 1
+STENCILS = {"ghost_linear": STENCIL_GHOST_LINEAR, "sw": STENCIL_SW}
+_fp = C.POINTER(C.c_float)
+
+
+class Solve3dDesc(C.Structure):
+    _fields_ = [("n", C.c_int64 * 3), ("fixed", _u8p), ("val", _dp),
+                ("theta", _fp), ("phi0", _dp), ("method", C.c_int32),
+                ("stencil", C.c_int32), ("tol", C.c_double),
+                ("omega", C.c_double), ("max_sweeps", C.c_int64),
+                ("max_cycles", C.c_int32), ("nu1", C.c_int32),
+                ("nu2", C.c_int32), ("sym_axes", C.c_int32)]
+
+
+class Solve3dResult(C.Structure):
+    _fields_ = [("sweeps", C.c_int64), ("delta", C.c_double),
+                ("status", C.c_int32)]
+
+
 # every entry point of include/ion_gym_amd.h: name -> (restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = {
@@ -172,6 +192,8 @@ SIGNATURES = {
     "ig_ensemble_refuse_births": (C.c_int, [_VP, C.c_int64, C.c_int64,
                                             C.c_int64, C.c_int64,
                                             C.POINTER(C.c_int64)]),
+    "ig_solve3d": (C.c_int, [_VP, C.POINTER(Solve3dDesc), _dp,
+                             C.POINTER(Solve3dResult)]),
 }
 
 

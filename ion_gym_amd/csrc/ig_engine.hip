@@ -17,6 +17,7 @@
 #include "ig_common.h"
 #include "ig_compose_kernel.h"
 #include "ig_metal.h"
+#include "ig_solve.h"
 
 using namespace ig;
 
@@ -1676,6 +1677,39 @@ int ig_fly_batch(ig_engine *h, ig_fields_id f, const ig_flight_params *p, const 
   ig_ensemble_release(h, e);
   if (rc) g_err = keep;
   return rc;
+}
+
+int ig_solve3d(ig_engine *h, const ig_solve3d_desc *d, double *phi, ig_solve3d_result *res) {
+  if (!h || !d || !phi || !res || !d->fixed || !d->val)
+    return fail(IG_E_ARG, "ig_solve3d: null argument");
+  for (int a = 0; a < 3; ++a)
+    if (d->n[a] < 1 || d->n[a] > 65535)
+      return fail(IG_E_ARG, "ig_solve3d: grid sizes must be in [1, 65535]");
+  if ((double)d->n[0] * d->n[1] * d->n[2] > 2.0e9)
+    return fail(IG_E_ARG, "ig_solve3d: more than 2e9 nodes");
+  if (d->method != IG_SOLVE_SOR && d->method != IG_SOLVE_VCYCLE)
+    return fail(IG_E_ARG, "ig_solve3d: method must be IG_SOLVE_SOR or IG_SOLVE_VCYCLE");
+  if (d->stencil != IG_STENCIL_GHOST_LINEAR && d->stencil != IG_STENCIL_SW)
+    return fail(IG_E_ARG, "ig_solve3d: unknown stencil (ghost_linear 0, sw 1)");
+  if (!(d->tol > 0.0) || !std::isfinite(d->tol))
+    return fail(IG_E_ARG, "ig_solve3d: tol must be positive and finite");
+  if (d->method == IG_SOLVE_SOR) {
+    if (!(d->omega > 0.0 && d->omega < 2.0))
+      return fail(IG_E_ARG, "ig_solve3d: SOR omega must lie in (0, 2)");
+    if (d->max_sweeps < 0) return fail(IG_E_ARG, "ig_solve3d: max_sweeps < 0");
+  } else {
+    if (d->max_cycles < 0 || d->nu1 < 0 || d->nu2 < 0)
+      return fail(IG_E_ARG, "ig_solve3d: max_cycles, nu1 and nu2 must be >= 0");
+    if (d->sym_axes < 0 || d->sym_axes > 7)
+      return fail(IG_E_ARG, "ig_solve3d: sym_axes is a 3-bit mask");
+    if (std::min(d->n[0], std::min(d->n[1], d->n[2])) < 2)
+      return fail(IG_E_ARG, "ig_solve3d: the V-cycle needs every axis of at least 2 nodes");
+  }
+  if (int rc = set_dev(h)) return rc;
+  std::string err;
+  if (int rc = ig::solve3d_run(h->stream, d, phi, res, err))
+    return fail(rc, "ig_solve3d: " + err);
+  return 0;
 }
 
 } // extern "C"
