@@ -209,6 +209,79 @@ def run_precision(eng, n_ions, n_cycles):
     return out
 
 
+def solver_scene(shape):
+    """One electrode's basis: a 3-node-thick plate at 1e4 V across the
+    middle half of x and three quarters of z, halfway up y, between
+    grounded walls at y = 0 and y = ny - 1 (the scene of the reference's
+    multigrid gates, tests/test_multigrid.py, at board size)."""
+    nx, ny, nz = shape
+    fixed = np.zeros(shape, bool)
+    fixed[nx // 4:3 * nx // 4, ny // 2 - 1:ny // 2 + 2,
+          nz // 8:7 * nz // 8] = True
+    val = np.zeros(shape)
+    val[fixed] = 1e4
+    fixed[:, 0, :] = True
+    fixed[:, -1, :] = True
+    return fixed, val
+
+
+def run_solver(eng, shape, cpu_shape, cpu):
+    """The basis solve leg: one V-cycle solve to 1e-6 V of the scene above
+    on a SLIM-sized grid, the red-black SOR pair time (the memory-bound
+    stencil) from the difference of a 1000- and a 200-pair run, and, with
+    cpu, the oracle's V-cycle on a smaller grid beside the engine's on the
+    same grid (bitwise compared)."""
+    from ion_gym_amd.physics import solver3d as S
+    fixed, val = solver_scene(shape)
+    n = fixed.size
+    S.solve3d_vmg(fixed, val, tol=1e-6, engine=eng)
+    t0 = time.perf_counter()
+    phi, sw, d = S.solve3d_vmg(fixed, val, tol=1e-6, engine=eng)
+    gpu_s = time.perf_counter() - t0
+    runs = {}
+    for pairs in (200, 1000):
+        t0 = time.perf_counter()
+        try:
+            S.solve3d(fixed, val, tol=1e-12, max_sweeps=pairs, engine=eng)
+        except S.SolveNotConverged:
+            pass
+        runs[pairs] = time.perf_counter() - t0
+    pair_s = (runs[1000] - runs[200]) / 800
+    # per red-black pair: both colours read every potential and the mask
+    # and write their half of the potentials
+    pair_bytes = 2 * n * (8 + 1 + 4)
+    out = {"workload": f"V-cycle basis solve {shape[0]}x{shape[1]}x"
+                       f"{shape[2]} (one plate at 1e4 V, grounded walls)",
+           "nodes": n, "tol_v": 1e-6, "fine_sweep_pairs": sw,
+           "cycle_delta_v": d, "gpu_s": gpu_s,
+           "sor_pair_us": pair_s * 1e6,
+           "sor_roofline": {"bound": "hbm", "achieved": pair_bytes / pair_s
+                            / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": pair_bytes / pair_s / 1e9
+                            / HBM_PEAK_GBS, "bytes_per_pair": pair_bytes}}
+    if cpu:
+        from oracle import solve3d as O
+        fc, vc = solver_scene(cpu_shape)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+        t0 = time.perf_counter()
+        pc, swc, dc, st = O.solve3d_vmg(fc, vc, tol=1e-6,
+                                        sym_axes=S.symmetric_axes(fc, vc))
+        cpu_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pg, swg, dg = S.solve3d_vmg(fc, vc, tol=1e-6, engine=eng)
+        gpu_small = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": cpu_s, "unit": "s per basis", "cores": threads,
+            "kind": "port",
+            "sample": f"the same scene at {cpu_shape[0]}x{cpu_shape[1]}x"
+                      f"{cpu_shape[2]}, V-cycle to 1e-6 V (FP64, OpenMP)"}
+        out["gpu_s_on_cpu_sample"] = gpu_small
+        out["speedup_vs_cpu"] = cpu_s / gpu_small
+        out["bitwise_equal_to_cpu"] = bool(st == 0 and swc == swg
+                                           and np.array_equal(pc, pg))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +311,8 @@ def main():
     ap.add_argument("--serve-flight-us", type=float, default=2.0)
     ap.add_argument("--serve-window", type=int, default=500)
     ap.add_argument("--serve-capacity", type=int, default=2_000_000)
+    ap.add_argument("--no-solver", action="store_true",
+                    help="skip the 3-D Laplace basis-solve leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
@@ -310,6 +385,12 @@ def main():
                               args.serve_flight_us, args.serve_window,
                               args.serve_capacity, seed=11,
                               compact=int(args.compact))
+    solver = None
+    if not args.no_solver:
+        print("[bench] solver: V-cycle basis solve 201x81x801",
+              file=sys.stderr, flush=True)
+        solver = run_solver(eng, (201, 81, 801), (101, 41, 401),
+                            not args.no_cpu)
     line = {
         "metric": "ion-steps/s at 1M ions (3-D funnel+HS), FP32 vs FP64 "
                   "plain; p50/p99 latency",
@@ -343,6 +424,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "precision": precision,
+        "solver": solver,
     }
     print(json.dumps(line), flush=True)
     eng.close()

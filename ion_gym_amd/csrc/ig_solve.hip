@@ -93,24 +93,40 @@ __device__ __forceinline__ void stencil(const double (&v)[6], const float *__res
 
 // the node of colour c of half-lattice slot t of a row list: rows (i, j),
 // k of the row's parity stepping by 2; false past the last row or the
-// row's end
-__device__ __forceinline__ bool colour_node_flat(const Grid &g, int c, int64_t t, int64_t &i,
+// row's end (a small grid: slots below 2^32)
+__device__ __forceinline__ bool colour_node_flat(const Grid &g, int c, uint32_t t, int64_t &i,
                                                  int64_t &j, int64_t &k) {
-  const int64_t half = (g.nz + 1) / 2;
-  const int64_t row = t / half, m = t - row * half;
-  i = row / g.ny;
-  j = row - i * g.ny;
-  k = ((i + j + c) & 1) + 2 * m;
+  const uint32_t half = (uint32_t)((g.nz + 1) / 2), ny = (uint32_t)g.ny;
+  const uint32_t row = t / half, m = t - row * half;
+  const uint32_t ii = row / ny;
+  i = ii;
+  j = row - ii * ny;
+  k = ((i + j + c) & 1) + 2 * (int64_t)m;
   return i < g.nx && k < g.nz;
 }
 
-// the same in a per-plane launch: plane i = blockIdx.y
+// XCD-aware tile of a per-plane launch: the dispatcher deals workgroups
+// to the 8 XCDs in turn, so workgroup b is renumbered to run r of XCD
+// b % 8, each XCD taking a contiguous run of planes whose neighbours are
+// then in its own L2
+__device__ __forceinline__ void xcd_tile(uint32_t &bx, uint32_t &by) {
+  const uint32_t nb = gridDim.x * gridDim.y;
+  const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+  const uint32_t q = nb / 8, r = nb % 8, x = b % 8, y = b / 8;
+  const uint32_t L = x * q + (x < r ? x : r) + y;
+  by = L / gridDim.x;
+  bx = L - by * gridDim.x;
+}
+
+// the same in a per-plane launch: plane i = the tile's row
 __device__ __forceinline__ bool colour_node(const Grid &g, int c, int64_t &i, int64_t &j,
                                             int64_t &k) {
+  uint32_t bx, by;
+  xcd_tile(bx, by);
   const uint32_t half = (uint32_t)((g.nz + 1) / 2);
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = bx * blockDim.x + threadIdx.x;
   const uint32_t jj = t / half, m = t - jj * half;
-  i = blockIdx.y;
+  i = by;
   j = jj;
   k = ((i + j + c) & 1) + 2 * (int64_t)m;
   return j < g.ny && k < g.nz;
@@ -262,18 +278,35 @@ __global__ __launch_bounds__(kBlock) void sweep_rhs_kernel(double *__restrict__ 
 
 // a whole bottom relaxation (multigrid3d.py:573-578) in one workgroup: up
 // to max_pairs pairs of colour sweeps, stopping after the first pair whose
-// max |update| < thr; the workgroup barrier orders the colours
-__global__ __launch_bounds__(1024) void bottom_kernel(double *__restrict__ e,
+// max |update| < thr; the workgroup barrier orders the colours.  With LDS
+// the level's e and f are staged in dynamic LDS (16 bytes a node) for the
+// whole relaxation and e written back at the end.
+template <bool LDS>
+__global__ __launch_bounds__(1024) void bottom_kernel(double *__restrict__ e_g,
                                                       const uint8_t *__restrict__ fixed,
-                                                      const double *__restrict__ f, const Grid g,
-                                                      double omega, double thr, int max_pairs) {
+                                                      const double *__restrict__ f_g,
+                                                      const Grid g, double omega, double thr,
+                                                      int max_pairs) {
+  extern __shared__ double bsh[];
   __shared__ unsigned long long red[16];
   __shared__ int stop;
+  const int64_t n = g.n();
+  double *e = e_g;
+  const double *f = f_g;
+  if constexpr (LDS) {
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+      bsh[q] = e_g[q];
+      bsh[n + q] = f_g[q];
+    }
+    __syncthreads();
+    e = bsh;
+    f = bsh + n;
+  }
   const int64_t half_total = g.nx * g.ny * ((g.nz + 1) / 2);
   for (int it = 0; it < max_pairs; ++it) {
     double dm = 0.0;
     for (int c = 0; c < 2; ++c) {
-      for (int64_t t = threadIdx.x; t < half_total; t += blockDim.x) {
+      for (uint32_t t = threadIdx.x; t < (uint32_t)half_total; t += blockDim.x) {
         int64_t i, j, k;
         if (!colour_node_flat(g, c, t, i, j, k)) continue;
         const int64_t q = ix(g, i, j, k);
@@ -300,6 +333,9 @@ __global__ __launch_bounds__(1024) void bottom_kernel(double *__restrict__ e,
     }
     __syncthreads();
     if (stop) break;
+  }
+  if constexpr (LDS) {
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) e_g[q] = bsh[q];
   }
 }
 
@@ -533,6 +569,7 @@ struct Solver {
   unsigned long long *slot; // a reduced maximum
   int32_t *done;
   int nu1, nu2;
+  size_t lds_max; // dynamic LDS a bottom workgroup may take
 
   // _solve_err at level l (multigrid3d.py:553-599): lv[l].f given, lv[l].e
   // solved
@@ -580,8 +617,14 @@ struct Solver {
   bool bottom(Level &L, double tol) {
     const double om = host_omega(L.g), thr = 0.02 * tol;
     if (L.g.n() <= kSmallNodes) {
-      hipLaunchKernelGGL(bottom_kernel, dim3(1), dim3(1024), 0, x.st, L.e, L.fixed, L.f, L.g,
-                         om, thr, kBottomPairs);
+      const size_t bytes = (size_t)16 * L.g.n();
+      if (bytes <= lds_max) {
+        hipLaunchKernelGGL(bottom_kernel<true>, dim3(1), dim3(1024), bytes, x.st, L.e, L.fixed,
+                           L.f, L.g, om, thr, kBottomPairs);
+      } else {
+        hipLaunchKernelGGL(bottom_kernel<false>, dim3(1), dim3(1024), 0, x.st, L.e, L.fixed,
+                           L.f, L.g, om, thr, kBottomPairs);
+      }
       return x.launched();
     }
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, x.st,
@@ -675,7 +718,21 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
   } else {
     // the error hierarchy, fine -> coarse, down to the first level with an
     // axis of at most kMinCoarse nodes (multigrid3d.py:569-582)
-    Solver S{x, {}, part, slot, done, d->nu1, d->nu2};
+    // the LDS a workgroup may allocate, less the bottom kernel's own words
+    int dev = 0, lds = 0;
+    if (!x.ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !x.ok(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+              "LDS size"))
+      return x.rc;
+    size_t lds_max = lds > 1024 ? (size_t)lds - 1024 : 0;
+    if (lds_max > 64512 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&bottom_kernel<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_max) != hipSuccess) {
+      (void)hipGetLastError(); // keep to the 64 KiB every workgroup may take
+      lds_max = 64512;
+    }
+    Solver S{x, {}, part, slot, done, d->nu1, d->nu2, lds_max};
     S.lv.emplace_back();
     S.lv[0].g = g;
     S.lv[0].fixed = fixed;
