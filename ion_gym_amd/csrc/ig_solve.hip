@@ -186,6 +186,29 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const unsigned long long *
   if (threadIdx.x == 0) *out = m;
 }
 
+// an error level's tolerances from the maxima of |f| (multigrid3d.py:567-
+// 569, 577): tolv[0] = rel max(max|f|, 1e-300) / 6, tolv[1] = 0.02 tolv[0]
+__global__ __launch_bounds__(1024) void err_tol_kernel(const unsigned long long *part,
+                                                       int64_t count, double rel,
+                                                       double *tolv) {
+  double scale = __longlong_as_double((long long)reduce_parts(part, count));
+  if (threadIdx.x != 0) return;
+  if (scale < 1e-300) scale = 1e-300;
+  const double tol = rel * scale / 6.0;
+  tolv[0] = tol;
+  tolv[1] = 0.02 * tol;
+}
+
+// a cycle's decision data: out[0] = the maximum of part, out[1] = tol
+__global__ __launch_bounds__(1024) void check_kernel(const unsigned long long *part,
+                                                     int64_t count, const double *tolv,
+                                                     double *out) {
+  const unsigned long long m = reduce_parts(part, count);
+  if (threadIdx.x != 0) return;
+  out[0] = __longlong_as_double((long long)m);
+  out[1] = tolv ? tolv[0] : 0.0;
+}
+
 // one colour sweep of the validated stencil (solver3d._sweep); done, when
 // given, ends the launch at once (a converged SOR)
 __global__ __launch_bounds__(kBlock) void sweep_kernel(double *__restrict__ phi,
@@ -213,15 +236,17 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(double *__restrict__ phi,
 }
 
 // after pair s (the partials of both colour sweeps): convergence when its
-// max |update| < tol (solver3d.py:281-287); last and pairs, when given,
-// record the pair
+// max |update| < tol, or < *tol_p when given (solver3d.py:281-287); last
+// and pairs, when given, record the pair
 __global__ __launch_bounds__(1024) void pair_finish_kernel(const unsigned long long *part,
                                                            int64_t count, int32_t *done,
-                                                           int64_t s, double tol, double *last,
+                                                           int64_t s, double tol,
+                                                           const double *tol_p, double *last,
                                                            int64_t *pairs) {
   if (*done) return;
   const double d = __longlong_as_double((long long)reduce_parts(part, count));
   if (threadIdx.x != 0) return;
+  if (tol_p) tol = *tol_p;
   if (last) *last = d;
   if (pairs) *pairs = s + 1;
   if (d < tol) *done = 1;
@@ -285,9 +310,10 @@ template <bool LDS>
 __global__ __launch_bounds__(1024) void bottom_kernel(double *__restrict__ e_g,
                                                       const uint8_t *__restrict__ fixed,
                                                       const double *__restrict__ f_g,
-                                                      const Grid g, double omega, double thr,
-                                                      int max_pairs) {
+                                                      const Grid g, double omega,
+                                                      const double *thr_p, int max_pairs) {
   extern __shared__ double bsh[];
+  const double thr = *thr_p;
   __shared__ unsigned long long red[16];
   __shared__ int stop;
   const int64_t n = g.n();
@@ -356,12 +382,10 @@ __global__ void residual_rhs_kernel(const double *__restrict__ e,
 }
 
 // full weighting (multigrid3d.py:420-455) followed by the caller's
-// rc *= 4 and negation (multigrid3d.py:586-589, 676-679): out = -(4 R r)
-__global__ void restrict_kernel(const double *__restrict__ fine, const Grid g,
-                                double *__restrict__ out, const Grid c) {
-  int64_t ic, jc, kc;
-  if (!plane_node(c, ic, jc, kc)) return;
-  const int64_t q = ix(c, ic, jc, kc);
+// rc *= 4 and negation (multigrid3d.py:586-589, 676-679): -(4 R r) at
+// coarse node (ic, jc, kc)
+__device__ __forceinline__ double restrict_at(const double *__restrict__ fine, const Grid &g,
+                                              int64_t ic, int64_t jc, int64_t kc) {
   const int64_t i = 2 * ic, j = 2 * jc, k = 2 * kc;
   double acc = 0.0;
   for (int di = -1; di <= 1; ++di) {
@@ -383,7 +407,14 @@ __global__ void restrict_kernel(const double *__restrict__ fine, const Grid g,
       }
     }
   }
-  out[q] = -((acc / 8.0) * 4.0);
+  return -((acc / 8.0) * 4.0);
+}
+
+__global__ void restrict_kernel(const double *__restrict__ fine, const Grid g,
+                                double *__restrict__ out, const Grid c) {
+  int64_t ic, jc, kc;
+  if (!plane_node(c, ic, jc, kc)) return;
+  out[ix(c, ic, jc, kc)] = restrict_at(fine, g, ic, jc, kc);
 }
 
 __device__ __forceinline__ void pro_axis(int64_t i, int64_t c, int64_t &a0, int64_t &a1,
@@ -397,14 +428,10 @@ __device__ __forceinline__ void pro_axis(int64_t i, int64_t c, int64_t &a0, int6
   t = xf - x0;
 }
 
-// trilinear prolongation added into the free nodes (multigrid3d.py:458-499)
-__global__ void prolong_add_kernel(const double *__restrict__ co, const Grid c,
-                                   double *__restrict__ out, const uint8_t *__restrict__ fixed,
-                                   const Grid g) {
-  int64_t i, j, k;
-  if (!plane_node(g, i, j, k)) return;
-  const int64_t q = ix(g, i, j, k);
-  if (fixed[q]) return;
+// the trilinear prolongation of co at fine node (i, j, k)
+// (multigrid3d.py:458-499)
+__device__ __forceinline__ double prolong_at(const double *__restrict__ co, const Grid &c,
+                                             int64_t i, int64_t j, int64_t k) {
   int64_t x0, x1, y0, y1, z0, z1;
   double tx, ty, tz;
   pro_axis(i, c.nx, x0, x1, tx);
@@ -417,7 +444,18 @@ __global__ void prolong_add_kernel(const double *__restrict__ co, const Grid c,
   const double c11 = C(x0, y1, z1) * (1 - tx) + C(x1, y1, z1) * tx;
   const double c0 = c00 * (1 - ty) + c10 * ty;
   const double c1 = c01 * (1 - ty) + c11 * ty;
-  out[q] += c0 * (1 - tz) + c1 * tz;
+  return c0 * (1 - tz) + c1 * tz;
+}
+
+// the prolongation added into the free nodes
+__global__ void prolong_add_kernel(const double *__restrict__ co, const Grid c,
+                                   double *__restrict__ out, const uint8_t *__restrict__ fixed,
+                                   const Grid g) {
+  int64_t i, j, k;
+  if (!plane_node(g, i, j, k)) return;
+  const int64_t q = ix(g, i, j, k);
+  if (fixed[q]) return;
+  out[q] += prolong_at(co, c, i, j, k);
 }
 
 // coarse node fixed iff any fine node of its 2 x 2 x 2 block is
@@ -486,6 +524,17 @@ struct Buf {
   }
 };
 
+// pinned host words for readbacks
+struct PinBuf {
+  void *p = nullptr;
+  PinBuf() = default;
+  PinBuf(const PinBuf &) = delete;
+  PinBuf &operator=(const PinBuf &) = delete;
+  ~PinBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 struct Ctx {
   hipStream_t st;
   std::string *err;
@@ -532,41 +581,44 @@ double host_omega(const Grid &g) {
   return 2.0 / (1.0 + std::sqrt(1.0 - rho * rho));
 }
 
-// read back the maximum of part[0, count)
-bool read_max(Ctx &x, const unsigned long long *part, int64_t count, unsigned long long *slot,
-              double &out) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, x.st, part, count, slot);
+// read back the maximum of part[0, count) and, when given, tolv[0]: one
+// 16-byte copy into pinned host memory
+bool read_check(Ctx &x, const unsigned long long *part, int64_t count, const double *tolv,
+                double *dev2, double *host2, double &m, double &tol) {
+  hipLaunchKernelGGL(check_kernel, dim3(1), dim3(1024), 0, x.st, part, count, tolv, dev2);
   if (!x.launched()) return false;
-  unsigned long long h = 0;
-  if (!x.ok(hipMemcpyAsync(&h, slot, 8, hipMemcpyDeviceToHost, x.st), "read max")) return false;
-  if (!x.ok(hipStreamSynchronize(x.st), "sync")) return false;
-  long long bits = (long long)h;
-  std::memcpy(&out, &bits, 8);
+  if (!x.ok(hipMemcpyAsync(host2, dev2, 16, hipMemcpyDeviceToHost, x.st), "read max") ||
+      !x.ok(hipStreamSynchronize(x.st), "sync"))
+    return false;
+  m = host2[0];
+  tol = host2[1];
   return true;
 }
 
-// device max |a - b| (b may be null for max |a|), read back
-bool dev_max(Ctx &x, const double *a, const double *b, int64_t n, unsigned long long *part,
-             unsigned long long *slot, double &out) {
+// the per-workgroup maxima of |a - b| (b may be null for |a|); returns
+// how many
+int64_t max_parts(Ctx &x, const double *a, const double *b, int64_t n,
+                  unsigned long long *part) {
   const unsigned g = (unsigned)std::min<int64_t>(blocks(n), kMaxDiffBlocks);
   hipLaunchKernelGGL(maxdiff_kernel, dim3(g), dim3(kBlock), 0, x.st, a, b, n, part);
-  return x.launched() && read_max(x, part, g, slot, out);
+  return x.launched() ? (int64_t)g : -1;
 }
 
 // one level of the error hierarchy: its mask, right-hand side, solution
 // and residual scratch
 struct Level {
   Grid g;
-  Buf fixed_b, f_b, e_b, r_b;
+  Buf fixed_b, f_b, e_b, r_b, tol_b;
   uint8_t *fixed = nullptr;
   double *f = nullptr, *e = nullptr, *r = nullptr;
+  double *tolv = nullptr; // the level's tol and bottom threshold
 };
 
 struct Solver {
   Ctx &x;
   std::vector<Level> lv; // lv[0] is the fine grid (its fixed mask only)
   unsigned long long *part; // per-workgroup maxima
-  unsigned long long *slot; // a reduced maximum
+  double *dev2, *host2;     // a cycle's (max, tol) on the device / pinned host
   int32_t *done;
   int nu1, nu2;
   size_t lds_max; // dynamic LDS a bottom workgroup may take
@@ -577,13 +629,13 @@ struct Solver {
     Level &L = lv[l];
     const int64_t n = L.g.n();
     hipLaunchKernelGGL(fill_kernel, dim3(blocks(n)), dim3(kBlock), 0, x.st, L.e, n, 0.0);
+    const int64_t np = max_parts(x, L.f, nullptr, n, part);
+    if (np < 0) return false;
+    hipLaunchKernelGGL(err_tol_kernel, dim3(1), dim3(1024), 0, x.st, part, np, kErrRelTol,
+                       L.tolv);
     if (!x.launched()) return false;
-    double scale;
-    if (!dev_max(x, L.f, nullptr, n, part, slot, scale)) return false;
-    if (scale < 1e-300) scale = 1e-300;
-    const double tol = kErrRelTol * scale / 6.0;
     const int64_t mn = std::min(L.g.nx, std::min(L.g.ny, L.g.nz));
-    if (mn <= kMinCoarse) return bottom(L, tol);
+    if (mn <= kMinCoarse) return bottom(L);
     for (int cyc = 0; cyc < kErrCycles; ++cyc) {
       for (int s = 0; s < nu1; ++s)
         for (int c = 0; c < 2; ++c)
@@ -605,8 +657,8 @@ struct Solver {
           hipLaunchKernelGGL(sweep_rhs_kernel, hg, dim3(kBlock), 0, x.st, L.e, L.fixed, L.f, L.g,
                              1.0, c, nullptr, s == nu2 - 1 ? part + c * nb : nullptr);
       if (!x.launched()) return false;
-      double d = 0.0;
-      if (nu2 > 0 && !read_max(x, part, 2 * nb, slot, d)) return false;
+      double d = 0.0, tol = 0.0;
+      if (!read_check(x, part, nu2 > 0 ? 2 * nb : 0, L.tolv, dev2, host2, d, tol)) return false;
       if (d < tol) break;
     }
     return true;
@@ -614,8 +666,9 @@ struct Solver {
 
   // the coarsest level: SOR with the level's optimal omega until a pair's
   // max |update| < 0.02 tol, at most kBottomPairs pairs (multigrid3d.py:570-577)
-  bool bottom(Level &L, double tol) {
-    const double om = host_omega(L.g), thr = 0.02 * tol;
+  bool bottom(Level &L) {
+    const double om = host_omega(L.g);
+    const double *thr = L.tolv + 1;
     if (L.g.n() <= kSmallNodes) {
       const size_t bytes = (size_t)16 * L.g.n();
       if (bytes <= lds_max) {
@@ -638,14 +691,14 @@ struct Solver {
           hipLaunchKernelGGL(sweep_rhs_kernel, hg, dim3(kBlock), 0, x.st, L.e, L.fixed, L.f,
                              L.g, om, c, done, part + c * nb);
         hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, x.st, part, 2 * nb, done,
-                           s, thr, nullptr, nullptr);
+                           s, 0.0, thr, nullptr, nullptr);
       }
       if (!x.launched()) return false;
-      int32_t h = 0;
-      if (!x.ok(hipMemcpyAsync(&h, done, 4, hipMemcpyDeviceToHost, x.st), "read flag") ||
+      int32_t *h = reinterpret_cast<int32_t *>(host2);
+      if (!x.ok(hipMemcpyAsync(h, done, 4, hipMemcpyDeviceToHost, x.st), "read flag") ||
           !x.ok(hipStreamSynchronize(x.st), "sync"))
         return false;
-      if (h) break;
+      if (*h) break;
     }
     return true;
   }
@@ -669,15 +722,20 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
   double *phi = x.alloc<double>(phi_b, n);
   uint8_t *fixed = x.alloc<uint8_t>(fixed_b, n);
   float *th = d->theta ? x.alloc<float>(th_b, 6 * n) : nullptr;
-  // control words: a reduced maximum, the done flag, last delta, pairs
+  // control words: [2, 4) a cycle's (max, tol), the done flag, last
+  // delta, pairs
   unsigned long long *ctl = x.alloc<unsigned long long>(ctl_b, 8);
+  PinBuf pin;
+  if (!x.rc && !x.ok(hipHostMalloc(&pin.p, 64, hipHostMallocDefault), "hipHostMalloc"))
+    return x.rc;
+  double *host2 = (double *)pin.p;
   // per-workgroup maxima: both colours of a fine sweep pair at most
   const int64_t nb_fine = n_blocks(half_grid(g));
   Buf part_b;
   unsigned long long *part =
       x.alloc<unsigned long long>(part_b, 2 * std::max<int64_t>(nb_fine, kMaxDiffBlocks));
   if (x.rc) return x.rc;
-  unsigned long long *slot = ctl;
+  double *dev2 = (double *)(ctl + 2);
   int32_t *done = (int32_t *)(ctl + 4);
   double *last = (double *)(ctl + 5);
   int64_t *pairs = (int64_t *)(ctl + 6);
@@ -698,14 +756,14 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
           hipLaunchKernelGGL(sweep_kernel, half_grid(g), dim3(kBlock), 0, st, phi, fixed,
                              th, g, d->omega, c, gl, done, part + c * nb_fine);
         hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, st, part, 2 * nb_fine,
-                           done, s, d->tol, last, pairs);
+                           done, s, d->tol, nullptr, last, pairs);
       }
       if (!x.launched()) return x.rc;
-      int32_t h = 0;
-      if (!x.ok(hipMemcpyAsync(&h, done, 4, hipMemcpyDeviceToHost, st), "read flag") ||
+      int32_t *hp = reinterpret_cast<int32_t *>(host2);
+      if (!x.ok(hipMemcpyAsync(hp, done, 4, hipMemcpyDeviceToHost, st), "read flag") ||
           !x.ok(hipStreamSynchronize(st), "sync"))
         return x.rc;
-      if (h) break;
+      if (*hp) break;
     }
     int32_t h = 0;
     x.ok(hipMemcpyAsync(&h, done, 4, hipMemcpyDeviceToHost, st), "read flag");
@@ -732,7 +790,7 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
       (void)hipGetLastError(); // keep to the 64 KiB every workgroup may take
       lds_max = 64512;
     }
-    Solver S{x, {}, part, slot, done, d->nu1, d->nu2, lds_max};
+    Solver S{x, {}, part, dev2, host2, done, d->nu1, d->nu2, lds_max};
     S.lv.emplace_back();
     S.lv[0].g = g;
     S.lv[0].fixed = fixed;
@@ -747,6 +805,7 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
       L.f = x.alloc<double>(L.f_b, m);
       L.e = x.alloc<double>(L.e_b, m);
       L.r = x.alloc<double>(L.r_b, m);
+      L.tolv = x.alloc<double>(L.tol_b, 2);
       if (x.rc) return x.rc;
       hipLaunchKernelGGL(coarsen_any_kernel, plane_grid(L.g), dim3(kBlock), 0, st,
                          S.lv.back().fixed, p, L.fixed, L.g);
@@ -782,7 +841,9 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
       for (int a = 0; a < 3; ++a)
         if ((d->sym_axes >> a) & 1)
           hipLaunchKernelGGL(symmetrize_kernel, plane_grid(g), dim3(kBlock), 0, st, phi, g, a);
-      if (!x.launched() || !dev_max(x, phi, c0, n, part, slot, cd)) return x.rc;
+      const int64_t np = max_parts(x, phi, c0, n, part);
+      double unused;
+      if (np < 0 || !read_check(x, part, np, nullptr, dev2, host2, cd, unused)) return x.rc;
       if (cd < d->tol) {
         status = 0;
         break;
