@@ -25,6 +25,18 @@ struct Waves {
   int64_t tab_off[IG_MAX_CHANNELS + 1];
 };
 
+// A tune's drive of one field set packed for the FP32-basis 3-D weights:
+// the phase terms, and the affine amplitude and offset already multiplied
+// by the tune's set scale, in 16-byte words (three loads a set).
+constexpr int kWaveSlow = 5;  // a square of duty != 0.5 (wave_eval_slow)
+constexpr int kWaveConst = 6; // the static set: weight a
+struct alignas(16) WaveF {
+  double om, ph;
+  double a, b;
+  int32_t mode; // IG_WAVE_SIN / _COS / _SQUARE (duty 0.5), kWaveSlow, kWaveConst
+  int32_t pad[3];
+};
+
 // One tune: the per-request / per-ensemble parameters an ion of a batched
 // launch flies with (ig_tune of the C ABI, on the device).
 struct Tune {
@@ -32,6 +44,7 @@ struct Tune {
   double dt, t_max;
   double scale[IG_MAX_CHANNELS + 2]; // multiplies each set's field
   uint64_t rng_key;
+  WaveF wf[IG_MAX_CHANNELS + 1];     // set 0 (static) and the drive channels
 };
 
 struct FieldsK {

@@ -345,25 +345,28 @@ __device__ __noinline__ double wave_eval_slow(const Waves *__restrict__ W, const
   return wave_eval<AFFINE>(W, f, k, t);
 }
 
-// FP32-basis waveform: sin, cos and square at duty 0.5 (sin >= 0, i.e. the
-// reduced phase >= 0) from the FP64-reduced phase; the rest as in FP64
+// the unit waveform in FP32 from the FP64-reduced phase om t + ph: kind
+// IG_WAVE_SIN, IG_WAVE_COS or IG_WAVE_SQUARE at duty 0.5 (sin >= 0, i.e.
+// the reduced phase >= 0)
+__device__ __forceinline__ float fast_unit(int kind, double om, double ph, double t) {
+  float r = phase_reduce(om * t + ph);
+  if (kind == IG_WAVE_SQUARE) return r >= 0.0f ? 1.0f : -1.0f;
+  if (kind == IG_WAVE_COS) { // cos(r) = sin(r + pi/2), back into [-pi, pi]
+    r = r + 1.57079632679489662f;
+    if (r > 3.14159265358979324f) r = r - 6.28318530717958648f;
+  }
+  return sin_pi_range(r);
+}
+
+// FP32-basis waveform: sin, cos and square at duty 0.5 by fast_unit; the
+// rest as in FP64
 template <bool AFFINE>
 __device__ __forceinline__ double wave_eval_fast(const Waves *__restrict__ W, const FieldsK &f,
                                                  int k, double t) {
   const int kind = W->kind[k];
   const bool sq = kind == IG_WAVE_SQUARE && W->duty[k] == 0.5;
   if (kind == IG_WAVE_SIN || kind == IG_WAVE_COS || sq) {
-    float r = phase_reduce(W->om[k] * t + W->ph[k]);
-    float v;
-    if (sq) {
-      v = r >= 0.0f ? 1.0f : -1.0f;
-    } else {
-      if (kind == IG_WAVE_COS) { // cos(r) = sin(r + pi/2), back into [-pi, pi]
-        r = r + 1.57079632679489662f;
-        if (r > 3.14159265358979324f) r = r - 6.28318530717958648f;
-      }
-      v = sin_pi_range(r);
-    }
+    const float v = fast_unit(kind, W->om[k], W->ph[k], t);
     return AFFINE ? W->amp[k] * (double)v + W->off[k] : (double)v;
   }
   return wave_eval_slow<AFFINE>(W, f, k, t);

@@ -674,7 +674,15 @@ int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_t
       b.w.duty[c] = a.ch_duty[c];
       b.w.amp[c] = a.ch_amp[c];
       b.w.off[c] = a.ch_off[c];
+      WaveF &w = b.wf[c + 1];
+      w.om = a.ch_om[c];
+      w.ph = a.ch_ph[c];
+      w.a = a.ch_amp[c] * a.set_scale[c + 1];
+      w.b = a.ch_off[c] * a.set_scale[c + 1];
+      w.mode = a.ch_kind[c] == IG_WAVE_SQUARE && a.ch_duty[c] != 0.5 ? kWaveSlow : a.ch_kind[c];
     }
+    b.wf[0].mode = kWaveConst;
+    b.wf[0].a = a.set_scale[0];
   }
   // loaded ions must keep a valid tune index
   if (E->n > 0) {

@@ -345,13 +345,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   constexpr int NW = NS > 1 ? NS - 1 : 1;
   double w[NW];
   double wtt = __builtin_nan("");
-  // FP32 mode: the per-set composition weights at lab time tt
+  // FP32 mode: the per-set composition weights at lab time tt; a tuned
+  // ion reads its tune's packed records (three loads a set, the tune's set
+  // scale already in a and b)
+  const WaveF *twf = e.tunes ? e.tunes[e.tune[i]].wf : nullptr;
   auto wt_at = [&](double tt, float(&o)[NS]) {
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      double v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+      double v;
+      if (twf) {
+        const WaveF &w = twf[q];
+        if (q == 0)
+          v = w.a;
+        else if (q > K)
+          v = 0.0;
+        else if (w.mode == kWaveSlow)
+          v = wave_eval_slow<true>(W, f, q - 1, tt) * tscale[q];
+        else
+          v = w.a * (double)fast_unit(w.mode, w.om, w.ph, tt) + w.b;
+      } else {
+        v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+      }
       v = v * f.scale[q];
-      if (tscale) v = v * tscale[q];
+      if (!twf && tscale) v = v * tscale[q];
       o[q] = (float)v;
     }
   };
