@@ -27,15 +27,19 @@ struct Waves {
 
 // A tune's drive of one field set packed for the FP32-basis 3-D weights:
 // the phase terms, and the affine amplitude and offset already multiplied
-// by the tune's set scale, in 16-byte words (three loads a set).
+// by the tune's set scale, in two 16-byte words; the sets' modes
+// (IG_WAVE_SIN / _COS / _SQUARE at duty 0.5, kWaveSlow, kWaveConst) sit
+// four bits each in one 16-byte word of the tune.
 constexpr int kWaveSlow = 5;  // a square of duty != 0.5 (wave_eval_slow)
 constexpr int kWaveConst = 6; // the static set: weight a
 struct alignas(16) WaveF {
   double om, ph;
   double a, b;
-  int32_t mode; // IG_WAVE_SIN / _COS / _SQUARE (duty 0.5), kWaveSlow, kWaveConst
-  int32_t pad[3];
 };
+
+__host__ __device__ inline int wave_mode(const uint32_t *m, int q) {
+  return (int)((m[q >> 3] >> (4 * (q & 7))) & 15u);
+}
 
 // One tune: the per-request / per-ensemble parameters an ion of a batched
 // launch flies with (ig_tune of the C ABI, on the device).
@@ -45,6 +49,7 @@ struct Tune {
   double scale[IG_MAX_CHANNELS + 2]; // multiplies each set's field
   uint64_t rng_key;
   WaveF wf[IG_MAX_CHANNELS + 1];     // set 0 (static) and the drive channels
+  alignas(16) uint32_t wmode[4];     // wf's modes, four bits a set
 };
 
 struct FieldsK {

@@ -679,9 +679,11 @@ int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_t
       w.ph = a.ch_ph[c];
       w.a = a.ch_amp[c] * a.set_scale[c + 1];
       w.b = a.ch_off[c] * a.set_scale[c + 1];
-      w.mode = a.ch_kind[c] == IG_WAVE_SQUARE && a.ch_duty[c] != 0.5 ? kWaveSlow : a.ch_kind[c];
+      const uint32_t mode =
+          a.ch_kind[c] == IG_WAVE_SQUARE && a.ch_duty[c] != 0.5 ? kWaveSlow : a.ch_kind[c];
+      b.wmode[(c + 1) >> 3] |= mode << (4 * ((c + 1) & 7));
     }
-    b.wf[0].mode = kWaveConst;
+    b.wmode[0] |= (uint32_t)kWaveConst;
     b.wf[0].a = a.set_scale[0];
   }
   // loaded ions must keep a valid tune index

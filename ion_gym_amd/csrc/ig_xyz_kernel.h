@@ -348,27 +348,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // FP32 mode: the per-set composition weights at lab time tt; a tuned
   // ion reads its tune's packed records (three loads a set, the tune's set
   // scale already in a and b)
-  const WaveF *twf = e.tunes ? e.tunes[e.tune[i]].wf : nullptr;
+  const Tune *tun = e.tunes ? e.tunes + e.tune[i] : nullptr;
+  const WaveF *twf = tun ? tun->wf : nullptr;
   auto wt_at = [&](double tt, float(&o)[NS]) {
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       double v;
       if (twf) {
-        const WaveF &w = twf[q];
+        const int mode = wave_mode(tun->wmode, q);
         if (q == 0)
-          v = w.a;
+          v = twf[0].a;
         else if (q > K)
           v = 0.0;
-        else if (w.mode == kWaveSlow)
+        else if (mode == kWaveSlow)
           v = wave_eval_slow<true>(W, f, q - 1, tt) * tscale[q];
         else
-          v = w.a * (double)fast_unit(w.mode, w.om, w.ph, tt) + w.b;
+          v = twf[q].a * (double)fast_unit(mode, twf[q].om, twf[q].ph, tt) + twf[q].b;
       } else {
         v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
       }
       v = v * f.scale[q];
       if (!twf && tscale) v = v * tscale[q];
       o[q] = (float)v;
+    }
+  };
+  // the weights of a step's three lab times, each tuned set's record read
+  // once for all three (the same arithmetic as three wt_at calls)
+  auto wt_at3 = [&](double t0, double t1, double t2, float(&o0)[NS], float(&o1)[NS],
+                    float(&o2)[NS]) {
+    if (!twf) {
+      wt_at(t0, o0);
+      wt_at(t1, o1);
+      wt_at(t2, o2);
+      return;
+    }
+    const uint4 mw = *reinterpret_cast<const uint4 *>(tun->wmode);
+    const uint32_t mws[4] = {mw.x, mw.y, mw.z, mw.w};
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const int mode = wave_mode(mws, q);
+      const double tt[3] = {t0, t1, t2};
+      double v[3];
+      WaveF w{};
+      if (q == 0)
+        w.a = twf[0].a;
+      else if (q <= K)
+        w = twf[q];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        if (q == 0)
+          v[m] = w.a;
+        else if (q > K)
+          v[m] = 0.0;
+        else if (mode == kWaveSlow)
+          v[m] = wave_eval_slow<true>(W, f, q - 1, tt[m]) * tscale[q];
+        else
+          v[m] = w.a * (double)fast_unit(mode, w.om, w.ph, tt[m]) + w.b;
+        v[m] = v[m] * f.scale[q];
+      }
+      o0[q] = (float)v[0];
+      o1[q] = (float)v[1];
+      o2[q] = (float)v[2];
     }
   };
   // FP64 modes: the waveform weights, kept while the lab time repeats
@@ -474,9 +514,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
             const Cell3 cl = cell3(f, px * inv_h, py * inv_h, pz * inv_h);
             if (st == 0) {
               float w0[NS], w1[NS], w2[NS];
-              wt_at(t_us, w0);
-              wt_at(t_us + 0.5 * h_us, w1);
-              wt_at(t_us + h_us, w2);
+              wt_at3(t_us, t_us + 0.5 * h_us, t_us + h_us, w0, w1, w2);
               xyz_cell_f32<NS>(f, cl, w0, w1, w2, ea, eb, ec, kc, bs);
               cl0 = cl;
             } else if (cl.i == cl0.i && cl.j == cl0.j && cl.k == cl0.k) {
