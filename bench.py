@@ -209,6 +209,62 @@ def run_precision(eng, n_ions, n_cycles):
     return out
 
 
+def run_configs(eng):
+    """The other flight configurations of the scope table, each measured as
+    ion-steps/s in FP64 and in the FP32 basis: the einzel lens (10,000
+    ions in vacuum, flown to fate in one launch) and the batched drift
+    tube (64 SDS ensembles of log-uniform 1e3-2e5 ions, each its own field
+    strength, m/z and dt, in one slab: one warm-up window, then two timed
+    500-step windows)."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.batch import EnsembleBatch
+    from ion_gym_amd.engine import Flight
+    out = {}
+    c = configs.einzel_rz(n_ions=10_000)
+    res = {}
+    for prec in ("f64", "f32"):
+        fp = eng.upload_rz(c["model"].pack(), prec)
+        ens = eng.ensemble(len(c["ions"]["x"]))
+        ens.load(c["ions"])
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.fly(fp, Flight(c["dt"], c["t_max"], max_steps=0), ens)
+        eng.sync()
+        el = time.perf_counter() - t0
+        res[prec] = ens.steps() / el
+        out[f"einzel_live_end_{prec}"] = ens.live()
+        ens.release()
+        fp.release()
+    out["einzel"] = {"workload": "einzel_rz (401x101 @ 0.1 mm, 0/+600/0 V, "
+                                 "10,000 ions m/z 100, vacuum, to fate)",
+                     "f64_ion_steps_per_s": res["f64"],
+                     "f32_ion_steps_per_s": res["f32"]}
+    b = configs.drift_batch_rz()
+    res = {}
+    for prec in ("f64", "f32"):
+        fp = eng.upload_rz(b["model"].pack(), prec)
+        fl = Flight(1.0, 2000.0, gas=b["gas"], max_steps=500)
+        batch = EnsembleBatch(eng, fp, fl, [(e["ions"], e["tune"])
+                                            for e in b["ensembles"]])
+        batch.fly(1, graph=False)
+        eng.sync()
+        s0 = batch.slab.steps()
+        t0 = time.perf_counter()
+        batch.fly(2, graph=False)
+        eng.sync()
+        el = time.perf_counter() - t0
+        res[prec] = (batch.slab.steps() - s0) / el
+        n = batch.n
+        batch.release()
+        fp.release()
+    out["drift_batch"] = {"workload": "drift_batch_rz (801x101 @ 0.1 mm, N2 "
+                                      "4 Torr SDS, 64 ensembles)",
+                          "ions": n, "window_steps": 500,
+                          "f64_ion_steps_per_s": res["f64"],
+                          "f32_ion_steps_per_s": res["f32"]}
+    return out
+
+
 def solver_scene(shape):
     """One electrode's basis: a 3-node-thick plate at 1e4 V across the
     middle half of x and three quarters of z, halfway up y, between
@@ -313,6 +369,8 @@ def main():
     ap.add_argument("--serve-capacity", type=int, default=2_000_000)
     ap.add_argument("--no-solver", action="store_true",
                     help="skip the 3-D Laplace basis-solve leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the einzel and batched drift-tube legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
@@ -385,6 +443,11 @@ def main():
                               args.serve_flight_us, args.serve_window,
                               args.serve_capacity, seed=11,
                               compact=int(args.compact))
+    others = None
+    if not args.no_configs:
+        print("[bench] configurations: einzel lens, batched drift tube",
+              file=sys.stderr, flush=True)
+        others = run_configs(eng)
     solver = None
     if not args.no_solver:
         print("[bench] solver: V-cycle basis solve 201x81x801",
@@ -425,6 +488,7 @@ def main():
         "serving": serving,
         "precision": precision,
         "solver": solver,
+        "configs": others,
     }
     print(json.dumps(line), flush=True)
     eng.close()
