@@ -67,14 +67,17 @@ constexpr int kDefaultCollBatch = 16;
 // 1; the 11-set SLIM 3-D flight, whose step is longer, is best at 1)
 constexpr int kDefaultCollBatchFlat = 8;
 constexpr int kDefaultCollBatchFlat3D = 1;
-// default register-budget variant of the hard-sphere kernel per precision
-// and route (measured: the FP32 r-z and planar kernels on the funnel are
+// default register-budget variant of the flight kernels per precision,
+// route and gas (measured: the FP32 r-z and planar kernels on the funnel are
 // best at 3 waves/SIMD; the FP32 3-D kernel, whose step keeps its cell's
 // corner sums, at 2 on the SLIM board (3 spills inside the step); FP64 and
 // FP32-storage at the compiler's own budget)
-inline int default_occupancy(int prec, int route) {
+inline int default_occupancy(int prec, int route, int gas) {
   if (prec != IG_PREC_F32) return 1;
-  return route == IG_ROUTE_XYZ ? 2 : 3;
+  if (route != IG_ROUTE_XYZ) return 3;
+  // the 3-D vacuum kernel (222 VGPRs whatever the budget) is fastest with
+  // the compiler's own schedule: quadrupole 6.8e9 at 1 against 5.5e9 at 2
+  return gas == IG_GAS_VACUUM ? 1 : 2;
 }
 
 int fail(int code, const std::string &msg) {
@@ -1146,7 +1149,7 @@ static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params
     return fail(IG_E_ARG, w + ": records with the transporter need 11 columns (the pass count)");
   E->route = fk->route;
   E->clock = clk;
-  if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec, fk->route);
+  if (k.occupancy == 0) k.occupancy = default_occupancy(fk->prec, fk->route, k.gas);
   if (k.coll_batch == 0)
     k.coll_batch = fk->prec != IG_PREC_F32 ? kDefaultCollBatch
                    : (fk->route == IG_ROUTE_XYZ ? kDefaultCollBatchFlat3D : kDefaultCollBatchFlat);

@@ -986,8 +986,15 @@ static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int6
       hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 1>), grid, dim3(block), lds, st, f,
                          p, e);
   } else if (p.gas == IG_GAS_VACUUM) {
-    hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 1>), grid, dim3(block), lds, st,
-                       f, p, e);
+    if (p.occupancy == 2)
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 2>), grid, dim3(block), lds,
+                         st, f, p, e);
+    else if (p.occupancy == 3)
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 3>), grid, dim3(block), lds,
+                         st, f, p, e);
+    else
+      hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 1>), grid, dim3(block), lds,
+                         st, f, p, e);
   } else {
     hipLaunchKernelGGL((xyz_sds_kernel<PREC, NS>), grid, dim3(block), 0, st, f, p, e);
   }
