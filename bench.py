@@ -210,12 +210,13 @@ def run_precision(eng, n_ions, n_cycles):
 
 
 def run_configs(eng):
-    """The other flight configurations of the scope table, each measured as
-    ion-steps/s in FP64 and in the FP32 basis: the einzel lens (10,000
-    ions in vacuum, flown to fate in one launch) and the batched drift
-    tube (64 SDS ensembles of log-uniform 1e3-2e5 ions, each its own field
-    strength, m/z and dt, in one slab: one warm-up window, then two timed
-    500-step windows)."""
+    """The other flight configurations, each measured as ion-steps/s in
+    FP64 and in the FP32 basis: the einzel lens (10,000 ions in vacuum,
+    flown to fate in one launch), the batched drift tube (64 SDS ensembles
+    of log-uniform 1e3-2e5 ions, each its own field strength, m/z and dt,
+    in one slab: one warm-up window, then two timed 500-step windows), the
+    planar route's quadrupole guide and the 3-D SDS drift cell (1M ions
+    each, a warm-up window and a timed one)."""
     from ion_gym_amd import configs
     from ion_gym_amd.batch import EnsembleBatch
     from ion_gym_amd.engine import Flight
@@ -224,6 +225,13 @@ def run_configs(eng):
     res = {}
     for prec in ("f64", "f32"):
         fp = eng.upload_rz(c["model"].pack(), prec)
+        # a short flight of a few ions first: the kernel's code is loaded
+        # on its first launch
+        warm = eng.ensemble(64)
+        warm.load({k: v[:64] for k, v in c["ions"].items()})
+        eng.fly(fp, Flight(c["dt"], c["t_max"], max_steps=10), warm)
+        eng.sync()
+        warm.release()
         ens = eng.ensemble(len(c["ions"]["x"]))
         ens.load(c["ions"])
         eng.sync()
@@ -262,6 +270,37 @@ def run_configs(eng):
                           "ions": n, "window_steps": 500,
                           "f64_ion_steps_per_s": res["f64"],
                           "f32_ion_steps_per_s": res["f32"]}
+    # the planar route and the 3-D SDS route: 1M ions, one warm-up window,
+    # one timed window
+    pq = configs.planar_quad(n_ions=1_000_000)
+    d3 = configs.drift_3d(n_ions=1_000_000)
+    legs = (("planar_quad", eng.upload_planar, pq["pack"], pq["ions"],
+             Flight(pq["dt"], pq["t_max"], max_steps=2000),
+             "planar_quad (161x161 @ 0.05 mm linear quadrupole guide, RF "
+             "1 MHz, vacuum, 1M ions)", 2000),
+            ("drift_3d", eng.upload_xyz, d3["pack"], d3["ions"],
+             Flight(d3["dt"], d3["t_max"], gas=d3["gas"], planes=d3["planes"],
+                    max_steps=1000),
+             "drift_3d (41x41x301 @ 0.1 mm SDS drift cell, N2 4 Torr, 1M "
+             "ions)", 1000))
+    for name, upload, pack, ions, fl, desc, win in legs:
+        res = {}
+        for prec in ("f64", "f32"):
+            fp = upload(pack, prec)
+            ens = eng.ensemble(len(ions["x"]))
+            ens.load(ions)
+            eng.fly(fp, fl, ens)
+            eng.sync()
+            s0 = ens.steps()
+            t0 = time.perf_counter()
+            eng.fly(fp, fl, ens)
+            eng.sync()
+            res[prec] = (ens.steps() - s0) / (time.perf_counter() - t0)
+            ens.release()
+            fp.release()
+        out[name] = {"workload": desc, "window_steps": win,
+                     "f64_ion_steps_per_s": res["f64"],
+                     "f32_ion_steps_per_s": res["f32"]}
     return out
 
 
@@ -370,7 +409,8 @@ def main():
     ap.add_argument("--no-solver", action="store_true",
                     help="skip the 3-D Laplace basis-solve leg")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the einzel and batched drift-tube legs")
+                    help="skip the einzel, batched drift-tube, planar and "
+                         "3-D SDS legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
@@ -445,7 +485,8 @@ def main():
                               compact=int(args.compact))
     others = None
     if not args.no_configs:
-        print("[bench] configurations: einzel lens, batched drift tube",
+        print("[bench] configurations: einzel lens, batched drift tube, "
+              "planar quadrupole, 3-D SDS drift cell",
               file=sys.stderr, flush=True)
         others = run_configs(eng)
     solver = None
