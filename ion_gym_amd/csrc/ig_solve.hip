@@ -210,13 +210,16 @@ __global__ __launch_bounds__(1024) void check_kernel(const unsigned long long *p
 }
 
 // one colour sweep of the validated stencil (solver3d._sweep); done, when
-// given, ends the launch at once (a converged SOR)
+// given, ends the launch at once (a converged SOR); snap, when given,
+// receives each updated node's value before the update (a cycle's first
+// pair fills the V-cycle's snapshot of the free nodes this way)
 __global__ __launch_bounds__(kBlock) void sweep_kernel(double *__restrict__ phi,
                                                        const uint8_t *__restrict__ fixed,
                                                        const float *__restrict__ th,
                                                        const Grid g, double omega, int colour,
                                                        int gl, const int32_t *done,
-                                                       unsigned long long *part) {
+                                                       unsigned long long *part,
+                                                       double *__restrict__ snap) {
   if (done && *done) return;
   int64_t i, j, k;
   double d = 0.0;
@@ -230,6 +233,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(double *__restrict__ phi,
       const double nw = old + omega * (num / den - old);
       d = fabs(nw - old);
       phi[q] = nw;
+      if (snap) snap[q] = old;
     }
   }
   if (part) block_max(d, part);
@@ -754,7 +758,7 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
       for (int64_t s = base; s < end; ++s) {
         for (int c = 0; c < 2; ++c)
           hipLaunchKernelGGL(sweep_kernel, half_grid(g), dim3(kBlock), 0, st, phi, fixed,
-                             th, g, d->omega, c, gl, done, part + c * nb_fine);
+                             th, g, d->omega, c, gl, done, part + c * nb_fine, nullptr);
         hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, st, part, 2 * nb_fine,
                            done, s, d->tol, nullptr, last, pairs);
       }
@@ -820,13 +824,20 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
     int64_t sw = 0;
     int status = 1;
     const Level &C = S.lv[1];
+    // the cycle-start snapshot: fixed nodes never change, so one copy of
+    // the whole field up front, then each cycle's first pre-smoothing pair
+    // writes the free nodes' values before it updates them (a whole copy
+    // per cycle when there is no pre-smoothing)
+    if (!x.ok(hipMemcpyAsync(c0, phi, n * 8, hipMemcpyDeviceToDevice, st), "snapshot"))
+      return x.rc;
     for (int cyc = 0; cyc < d->max_cycles; ++cyc) {
-      if (!x.ok(hipMemcpyAsync(c0, phi, n * 8, hipMemcpyDeviceToDevice, st), "snapshot"))
+      if (cyc > 0 && d->nu1 == 0 &&
+          !x.ok(hipMemcpyAsync(c0, phi, n * 8, hipMemcpyDeviceToDevice, st), "snapshot"))
         return x.rc;
       for (int s = 0; s < d->nu1; ++s, ++sw)
         for (int c = 0; c < 2; ++c)
           hipLaunchKernelGGL(sweep_kernel, half_grid(g), dim3(kBlock), 0, st, phi, fixed,
-                             th, g, 1.0, c, gl, nullptr, nullptr);
+                             th, g, 1.0, c, gl, nullptr, nullptr, s == 0 ? c0 : nullptr);
       hipLaunchKernelGGL(residual_kernel, plane_grid(g), dim3(kBlock), 0, st, phi, fixed, th, g,
                          gl, r);
       hipLaunchKernelGGL(restrict_kernel, plane_grid(C.g), dim3(kBlock), 0, st, r, g, C.f,
@@ -837,7 +848,7 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
       for (int s = 0; s < d->nu2; ++s, ++sw)
         for (int c = 0; c < 2; ++c)
           hipLaunchKernelGGL(sweep_kernel, half_grid(g), dim3(kBlock), 0, st, phi, fixed,
-                             th, g, 1.0, c, gl, nullptr, nullptr);
+                             th, g, 1.0, c, gl, nullptr, nullptr, nullptr);
       for (int a = 0; a < 3; ++a)
         if ((d->sym_axes >> a) & 1)
           hipLaunchKernelGGL(symmetrize_kernel, plane_grid(g), dim3(kBlock), 0, st, phi, g, a);
