@@ -477,6 +477,7 @@ int ig_ensemble_refuse_births(ig_engine *h, ig_ensemble_id id,
 
 #define IG_SOLVE_SOR 0    /* red-black SOR: ion_gym/physics/solver3d.py:250-293 */
 #define IG_SOLVE_VCYCLE 1 /* V-cycle multigrid: ion_gym/physics/multigrid3d.py:602-731 */
+#define IG_SOLVE_CASCADE 2 /* cascadic multigrid: ion_gym/physics/multigrid3d.py:175-230 */
 #define IG_STENCIL_GHOST_LINEAR 0 /* solver3d.py:172-181 */
 #define IG_STENCIL_SW 1           /* Shortley-Weller, solver3d.py:182-190 */
 
@@ -486,17 +487,26 @@ typedef struct {
   const double *val;     /* potentials at the fixed nodes (host) */
   const float *theta;    /* (6, nx, ny, nz) leg fractions in the order
                             -x, +x, -y, +y, -z, +z, or NULL for all 1 */
-  const double *phi0;    /* initial guess or NULL for zeros */
+  const double *phi0;    /* initial guess or NULL for zeros (NULL for
+                            the cascade) */
   int32_t method;        /* IG_SOLVE_* */
   int32_t stencil;       /* IG_STENCIL_* */
   double tol;            /* volts */
-  double omega;          /* SOR relaxation factor (solver3d.optimal_omega) */
-  int64_t max_sweeps;    /* SOR: red-black sweep pairs allowed */
+  double omega;          /* SOR relaxation factor (solver3d.optimal_omega);
+                            the cascade's fine grid */
+  int64_t max_sweeps;    /* SOR and every cascade grid: red-black sweep
+                            pairs allowed */
   int32_t max_cycles;    /* V-cycle: cycles allowed (reference 60) */
   int32_t nu1, nu2;      /* V-cycle: pre/post smoothing pairs (3, 3) */
   int32_t sym_axes;      /* V-cycle: bit a projects each cycle's iterate
                             onto the mirror-symmetric subspace of axis a;
                             the caller has verified the symmetry */
+  int32_t min_coarse;    /* cascade: stop coarsening at an axis of at most
+                            this many nodes (reference 9) */
+  int32_t max_levels;    /* cascade: grids in the hierarchy, the fine one
+                            included (8) */
+  double level_tol_factor; /* cascade: intermediate grids converge to
+                              tol times this (30) */
 } ig_solve3d_desc;
 
 typedef struct {
@@ -505,6 +515,9 @@ typedef struct {
                       cycle's max |change| */
   int32_t status;  /* 0 converged (delta < tol), 1 the sweep or cycle
                       limit was reached, 2 the V-cycle stagnated */
+  int32_t level;   /* cascade: the last grid solved (0 the fine one, l
+                      the l-th coarsening); with status 1, the grid whose
+                      SOR ran out of sweeps, and sweeps and delta are its */
 } ig_solve3d_result;
 
 /* Solve Laplace for one basis and write the final iterate to phi (host,
@@ -516,9 +529,16 @@ typedef struct {
  * 2x2x2 block is) solved to 5 % of its right-hand side
  * (multigrid3d.py:553-599), its prolongation added, nu2 pairs, the
  * symmetric projection, and convergence when the cycle moved no node by
- * tol.  Replaces solve3d / solve3d_vmg under solve_bases and
- * solve_bases_mg (solver3d.py:296-320, multigrid3d.py:233-302); the
- * caller raises the reference's SolveNotConverged on status != 0. */
+ * tol.  CASCADE is solve3d_mg: (fixed, val) coarsened by injection with
+ * the block's largest-|val| fixed node rescuing a free injection point,
+ * while every axis exceeds min_coarse and halves to at least 3 nodes;
+ * the coarsest grid solved by SOR from zeros to tol, each finer one from
+ * the trilinear prolongation of the coarser solution (to tol times
+ * level_tol_factor, unit fractions, the grid's optimal omega), the fine
+ * grid last with its fractions, omega and tol.  Replaces solve3d /
+ * solve3d_vmg / solve3d_mg under solve_bases and solve_bases_mg
+ * (solver3d.py:296-320, multigrid3d.py:233-302); the caller raises the
+ * reference's SolveNotConverged on status != 0. */
 int ig_solve3d(ig_engine *h, const ig_solve3d_desc *d, double *phi,
                ig_solve3d_result *res);
 

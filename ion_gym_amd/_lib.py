@@ -122,7 +122,7 @@ class BirthsDesc(C.Structure):
                 ("zig_k", _u64p), ("zig_w", _dp), ("zig_f", _dp)]
 
 
-SOLVE_SOR, SOLVE_VCYCLE = 0, 1
+SOLVE_SOR, SOLVE_VCYCLE, SOLVE_CASCADE = 0, 1, 2
 STENCIL_GHOST_LINEAR, STENCIL_SW = 0, 1
 STENCILS = {"ghost_linear": STENCIL_GHOST_LINEAR, "sw": STENCIL_SW}
 _fp = C.POINTER(C.c_float)
@@ -134,12 +134,14 @@ class Solve3dDesc(C.Structure):
                 ("stencil", C.c_int32), ("tol", C.c_double),
                 ("omega", C.c_double), ("max_sweeps", C.c_int64),
                 ("max_cycles", C.c_int32), ("nu1", C.c_int32),
-                ("nu2", C.c_int32), ("sym_axes", C.c_int32)]
+                ("nu2", C.c_int32), ("sym_axes", C.c_int32),
+                ("min_coarse", C.c_int32), ("max_levels", C.c_int32),
+                ("level_tol_factor", C.c_double)]
 
 
 class Solve3dResult(C.Structure):
     _fields_ = [("sweeps", C.c_int64), ("delta", C.c_double),
-                ("status", C.c_int32)]
+                ("status", C.c_int32), ("level", C.c_int32)]
 
 
 # every entry point of include/ion_gym_amd.h: name -> (restype, argtypes)

@@ -1700,16 +1700,22 @@ int ig_solve3d(ig_engine *h, const ig_solve3d_desc *d, double *phi, ig_solve3d_r
       return fail(IG_E_ARG, "ig_solve3d: grid sizes must be in [1, 65535]");
   if ((double)d->n[0] * d->n[1] * d->n[2] > 2.0e9)
     return fail(IG_E_ARG, "ig_solve3d: more than 2e9 nodes");
-  if (d->method != IG_SOLVE_SOR && d->method != IG_SOLVE_VCYCLE)
-    return fail(IG_E_ARG, "ig_solve3d: method must be IG_SOLVE_SOR or IG_SOLVE_VCYCLE");
+  if (d->method != IG_SOLVE_SOR && d->method != IG_SOLVE_VCYCLE && d->method != IG_SOLVE_CASCADE)
+    return fail(IG_E_ARG,
+                "ig_solve3d: method must be IG_SOLVE_SOR, IG_SOLVE_VCYCLE or IG_SOLVE_CASCADE");
   if (d->stencil != IG_STENCIL_GHOST_LINEAR && d->stencil != IG_STENCIL_SW)
     return fail(IG_E_ARG, "ig_solve3d: unknown stencil (ghost_linear 0, sw 1)");
   if (!(d->tol > 0.0) || !std::isfinite(d->tol))
     return fail(IG_E_ARG, "ig_solve3d: tol must be positive and finite");
-  if (d->method == IG_SOLVE_SOR) {
+  if (d->method != IG_SOLVE_VCYCLE) {
     if (!(d->omega > 0.0 && d->omega < 2.0))
       return fail(IG_E_ARG, "ig_solve3d: SOR omega must lie in (0, 2)");
     if (d->max_sweeps < 0) return fail(IG_E_ARG, "ig_solve3d: max_sweeps < 0");
+    if (d->method == IG_SOLVE_CASCADE) {
+      if (d->phi0) return fail(IG_E_ARG, "ig_solve3d: the cascade starts from zeros (phi0 NULL)");
+      if (!(d->level_tol_factor > 0.0) || !std::isfinite(d->level_tol_factor))
+        return fail(IG_E_ARG, "ig_solve3d: level_tol_factor must be positive and finite");
+    }
   } else {
     if (d->max_cycles < 0 || d->nu1 < 0 || d->nu2 < 0)
       return fail(IG_E_ARG, "ig_solve3d: max_cycles, nu1 and nu2 must be >= 0");

@@ -432,15 +432,36 @@ __device__ __forceinline__ void pro_axis(int64_t i, int64_t c, int64_t &a0, int6
   t = xf - x0;
 }
 
-// the trilinear prolongation of co at fine node (i, j, k)
-// (multigrid3d.py:458-499)
+// one axis of the cascade's prolongation (multigrid3d.py:144-156): fine
+// index i sits at coarse coordinate min(i / 2, c - 1), its upper
+// neighbour clamped to the last coarse node
+__device__ __forceinline__ void cas_axis(int64_t i, int64_t c, int64_t &a0, int64_t &a1,
+                                         double &t) {
+  double xf = i / 2.0;
+  if (xf > (double)(c - 1)) xf = (double)(c - 1);
+  const int64_t x0 = (int64_t)floor(xf);
+  a0 = x0;
+  a1 = x0 + 1 < c - 1 ? x0 + 1 : c - 1;
+  t = xf - (double)x0;
+}
+
+// the trilinear prolongation of co at fine node (i, j, k): the V-cycle's
+// (multigrid3d.py:458-499) or, with CAS, the cascade's
+// (multigrid3d.py:138-172); the same interpolant, x first, then y, then z
+template <bool CAS = false>
 __device__ __forceinline__ double prolong_at(const double *__restrict__ co, const Grid &c,
                                              int64_t i, int64_t j, int64_t k) {
   int64_t x0, x1, y0, y1, z0, z1;
   double tx, ty, tz;
-  pro_axis(i, c.nx, x0, x1, tx);
-  pro_axis(j, c.ny, y0, y1, ty);
-  pro_axis(k, c.nz, z0, z1, tz);
+  if (CAS) {
+    cas_axis(i, c.nx, x0, x1, tx);
+    cas_axis(j, c.ny, y0, y1, ty);
+    cas_axis(k, c.nz, z0, z1, tz);
+  } else {
+    pro_axis(i, c.nx, x0, x1, tx);
+    pro_axis(j, c.ny, y0, y1, ty);
+    pro_axis(k, c.nz, z0, z1, tz);
+  }
   auto C = [&](int64_t a, int64_t b, int64_t d) { return co[(a * c.ny + b) * c.nz + d]; };
   const double c00 = C(x0, y0, z0) * (1 - tx) + C(x1, y0, z0) * tx;
   const double c10 = C(x0, y1, z0) * (1 - tx) + C(x1, y1, z0) * tx;
@@ -460,6 +481,56 @@ __global__ void prolong_add_kernel(const double *__restrict__ co, const Grid c,
   const int64_t q = ix(g, i, j, k);
   if (fixed[q]) return;
   out[q] += prolong_at(co, c, i, j, k);
+}
+
+// the cascade's starting guess: the prolongated coarser solution at every
+// free node, the fixed nodes keeping their values
+__global__ void prolongate_kernel(const double *__restrict__ co, const Grid c,
+                                  double *__restrict__ out, const uint8_t *__restrict__ fixed,
+                                  const Grid g) {
+  int64_t i, j, k;
+  if (!plane_node(g, i, j, k)) return;
+  const int64_t q = ix(g, i, j, k);
+  if (fixed[q]) return;
+  out[q] = prolong_at<true>(co, c, i, j, k);
+}
+
+// the cascade's coarsening (multigrid3d.py:84-135): coarse node c takes
+// fine node 2c by injection; when that node is free, c becomes fixed if any
+// node of the fine 2 x 2 x 2 block at 2c is, with the value of the block's
+// fixed node of largest |value| (the first in the order di, dj, dk, dk
+// fastest).  pf holds the values at the fixed fine nodes; pc receives the
+// coarse values at fixed nodes and zero at free ones (the coarsest
+// solve's starting guess)
+__global__ void coarsen_mask_kernel(const uint8_t *__restrict__ ff,
+                                    const double *__restrict__ pf, const Grid g,
+                                    uint8_t *__restrict__ fc, double *__restrict__ pc,
+                                    const Grid c) {
+  int64_t i, j, k;
+  if (!plane_node(c, i, j, k)) return;
+  const int64_t q = ix(c, i, j, k);
+  const int64_t f0 = ix(g, 2 * i, 2 * j, 2 * k);
+  uint8_t fx = ff[f0];
+  double v = pf[f0];
+  if (!fx) {
+    double best = -1.0;
+    for (int di = 0; di < 2; ++di)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int dk = 0; dk < 2; ++dk) {
+          const int64_t a = 2 * i + di, b = 2 * j + dj, e = 2 * k + dk;
+          if (a >= g.nx || b >= g.ny || e >= g.nz) continue;
+          const int64_t f = ix(g, a, b, e);
+          if (!ff[f]) continue;
+          const double m = fabs(pf[f]);
+          if (m > best) {
+            best = m;
+            v = pf[f];
+          }
+        }
+    fx = best >= 0.0;
+  }
+  fc[q] = fx;
+  pc[q] = fx ? v : 0.0;
 }
 
 // coarse node fixed iff any fine node of its 2 x 2 x 2 block is
@@ -608,6 +679,55 @@ int64_t max_parts(Ctx &x, const double *a, const double *b, int64_t n,
   return x.launched() ? (int64_t)g : -1;
 }
 
+struct SorOut {
+  int64_t sweeps = 0;
+  double delta = INFINITY;
+  int status = 1;
+};
+
+// red-black SOR of phi until a pair's max |update| < tol, at most
+// max_sweeps pairs (solver3d.py:276-293): the pair maximum and the done
+// flag are kept on the device, the host reads the flag every kSorChunk
+// pairs.  ctl[4, 7) holds the done flag, the last pair's delta and the
+// pair count; part takes both colours' per-workgroup maxima.
+bool sor_run(Ctx &x, double *phi, const uint8_t *fixed, const float *th, const Grid &g, int gl,
+             double omega, double tol, int64_t max_sweeps, unsigned long long *part,
+             unsigned long long *ctl, double *host2, SorOut &o) {
+  int32_t *done = (int32_t *)(ctl + 4);
+  double *last = (double *)(ctl + 5);
+  int64_t *pairs = (int64_t *)(ctl + 6);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, x.st, ctl + 4, 3);
+  const dim3 hg = half_grid(g);
+  const int64_t nb = n_blocks(hg);
+  int32_t *hp = reinterpret_cast<int32_t *>(host2);
+  for (int64_t base = 0; base < max_sweeps; base += kSorChunk) {
+    const int64_t end = std::min<int64_t>(base + kSorChunk, max_sweeps);
+    for (int64_t s = base; s < end; ++s) {
+      for (int c = 0; c < 2; ++c)
+        hipLaunchKernelGGL(sweep_kernel, hg, dim3(kBlock), 0, x.st, phi, fixed, th, g, omega, c,
+                           gl, done, part + c * nb, nullptr);
+      hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, x.st, part, 2 * nb, done, s,
+                         tol, nullptr, last, pairs);
+    }
+    if (!x.launched()) return false;
+    if (!x.ok(hipMemcpyAsync(hp, done, 4, hipMemcpyDeviceToHost, x.st), "read flag") ||
+        !x.ok(hipStreamSynchronize(x.st), "sync"))
+      return false;
+    if (*hp) break;
+  }
+  // done, delta and pairs in one read
+  unsigned long long *hw = reinterpret_cast<unsigned long long *>(host2);
+  if (!x.ok(hipMemcpyAsync(hw, ctl + 4, 24, hipMemcpyDeviceToHost, x.st), "read result") ||
+      !x.ok(hipStreamSynchronize(x.st), "sync"))
+    return false;
+  const int32_t fl = *reinterpret_cast<int32_t *>(hw);
+  std::memcpy(&o.delta, hw + 1, 8);
+  std::memcpy(&o.sweeps, hw + 2, 8);
+  o.status = fl ? 0 : 1;
+  if (o.sweeps == 0) o.delta = INFINITY;
+  return true;
+}
+
 // one level of the error hierarchy: its mask, right-hand side, solution
 // and residual scratch
 struct Level {
@@ -741,8 +861,6 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
   if (x.rc) return x.rc;
   double *dev2 = (double *)(ctl + 2);
   int32_t *done = (int32_t *)(ctl + 4);
-  double *last = (double *)(ctl + 5);
-  int64_t *pairs = (int64_t *)(ctl + 6);
   x.ok(hipMemcpyAsync(phi, init.data(), n * 8, hipMemcpyHostToDevice, st), "upload phi");
   x.ok(hipMemcpyAsync(fixed, d->fixed, n, hipMemcpyHostToDevice, st), "upload mask");
   if (th) x.ok(hipMemcpyAsync(th, d->theta, 6 * n * 4, hipMemcpyHostToDevice, st), "upload theta");
@@ -751,32 +869,63 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
   res->sweeps = 0;
   res->delta = INFINITY;
   res->status = 1;
+  res->level = 0;
 
   if (d->method == IG_SOLVE_SOR) {
-    for (int64_t base = 0; base < d->max_sweeps; base += kSorChunk) {
-      const int64_t end = std::min<int64_t>(base + kSorChunk, d->max_sweeps);
-      for (int64_t s = base; s < end; ++s) {
-        for (int c = 0; c < 2; ++c)
-          hipLaunchKernelGGL(sweep_kernel, half_grid(g), dim3(kBlock), 0, st, phi, fixed,
-                             th, g, d->omega, c, gl, done, part + c * nb_fine, nullptr);
-        hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, st, part, 2 * nb_fine,
-                           done, s, d->tol, nullptr, last, pairs);
-      }
+    SorOut o;
+    if (!sor_run(x, phi, fixed, th, g, gl, d->omega, d->tol, d->max_sweeps, part, ctl, host2, o))
+      return x.rc;
+    res->sweeps = o.sweeps;
+    res->delta = o.delta;
+    res->status = o.status;
+  } else if (d->method == IG_SOLVE_CASCADE) {
+    // the hierarchy, fine -> coarse (multigrid3d.py:194-201); each grid's
+    // phi holds its fixed values and zeros until its solve
+    struct Grd {
+      Grid g;
+      Buf f_b, p_b;
+      uint8_t *fixed = nullptr;
+      double *phi = nullptr;
+    };
+    std::vector<Grd> gr(1);
+    gr[0].g = g;
+    gr[0].fixed = fixed;
+    gr[0].phi = phi;
+    while ((int64_t)gr.size() < d->max_levels) {
+      const Grid p = gr.back().g;
+      const int64_t mn = std::min(p.nx, std::min(p.ny, p.nz));
+      if (mn <= d->min_coarse || (mn + 1) / 2 < 3) break;
+      Grd L;
+      L.g = Grid{(p.nx + 1) / 2, (p.ny + 1) / 2, (p.nz + 1) / 2};
+      L.fixed = x.alloc<uint8_t>(L.f_b, L.g.n());
+      L.phi = x.alloc<double>(L.p_b, L.g.n());
+      if (x.rc) return x.rc;
+      hipLaunchKernelGGL(coarsen_mask_kernel, plane_grid(L.g), dim3(kBlock), 0, st,
+                         gr.back().fixed, gr.back().phi, p, L.fixed, L.phi, L.g);
       if (!x.launched()) return x.rc;
-      int32_t *hp = reinterpret_cast<int32_t *>(host2);
-      if (!x.ok(hipMemcpyAsync(hp, done, 4, hipMemcpyDeviceToHost, st), "read flag") ||
-          !x.ok(hipStreamSynchronize(st), "sync"))
-        return x.rc;
-      if (*hp) break;
+      gr.push_back(std::move(L));
     }
-    int32_t h = 0;
-    x.ok(hipMemcpyAsync(&h, done, 4, hipMemcpyDeviceToHost, st), "read flag");
-    x.ok(hipMemcpyAsync(&res->delta, last, 8, hipMemcpyDeviceToHost, st), "read delta");
-    x.ok(hipMemcpyAsync(&res->sweeps, pairs, 8, hipMemcpyDeviceToHost, st), "read pairs");
-    x.ok(hipStreamSynchronize(st), "sync");
-    if (x.rc) return x.rc;
-    res->status = h ? 0 : 1;
-    if (res->sweeps == 0) res->delta = INFINITY;
+    // coarsest to tol from zeros, the middle grids to tol * factor from
+    // the prolongated coarser solution, the fine grid with its fractions
+    // (multigrid3d.py:207-229)
+    const int nl = (int)gr.size();
+    SorOut o;
+    for (int l = nl - 1; l >= 0; --l) {
+      Grd &L = gr[l];
+      if (l < nl - 1)
+        hipLaunchKernelGGL(prolongate_kernel, plane_grid(L.g), dim3(kBlock), 0, st, gr[l + 1].phi,
+                           gr[l + 1].g, L.phi, L.fixed, L.g);
+      const double lt = (l == nl - 1 || l == 0) ? d->tol : d->tol * d->level_tol_factor;
+      const double om = l == 0 ? d->omega : host_omega(L.g);
+      if (!sor_run(x, L.phi, L.fixed, l == 0 ? th : nullptr, L.g, gl, om, lt, d->max_sweeps, part,
+                   ctl, host2, o))
+        return x.rc;
+      res->level = l;
+      if (o.status) break;
+    }
+    res->sweeps = o.sweeps;
+    res->delta = o.delta;
+    res->status = o.status;
   } else {
     // the error hierarchy, fine -> coarse, down to the first level with an
     // axis of at most kMinCoarse nodes (multigrid3d.py:569-582)

@@ -12,6 +12,10 @@ The reference's native solvers, with their call signatures and refusals:
   5 % on 2x-coarsened grids, projects onto verified mirror symmetries and
   stops when a whole cycle moved no node by ``tol``
   (multigrid3d.py:602-731);
+* ``solve3d_mg`` -- the cascade: the mask and values coarsened with a
+  rescue of thin metal, SOR from the coarsest grid up, each grid started
+  from the trilinear prolongation of the one below, the fine grid solved
+  by the same SOR as ``solve3d`` (multigrid3d.py:84-230);
 * ``solve_bases`` / ``solve_bases_mg`` -- one solve per electrode at
   ``v_basis`` with every other electrode grounded
   (solver3d.py:296-320, multigrid3d.py:233-302).
@@ -73,7 +77,8 @@ def _check_stencil(stencil):
 
 
 def _run(fixed, val, theta, phi0, stencil, tol, *, method, omega=0.0,
-         max_sweeps=0, max_cycles=0, nu1=0, nu2=0, sym_axes=0, engine=None):
+         max_sweeps=0, max_cycles=0, nu1=0, nu2=0, sym_axes=0, min_coarse=0,
+         max_levels=0, level_tol_factor=0.0, engine=None):
     fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
     if fixed.ndim != 3:
         raise ValueError(f"fixed must be 3-D, got shape {fixed.shape}")
@@ -104,12 +109,27 @@ def _run(fixed, val, theta, phi0, stencil, tol, *, method, omega=0.0,
     d.max_cycles = int(max_cycles)
     d.nu1, d.nu2 = int(nu1), int(nu2)
     d.sym_axes = int(sym_axes)
+    d.min_coarse = int(min_coarse)
+    d.max_levels = int(max_levels)
+    d.level_tol_factor = float(level_tol_factor)
     phi = np.empty(shape, np.float64)
     res = L.Solve3dResult()
     eng = _engine(engine)
     L.check(L.lib().ig_solve3d(eng.h, C.byref(d), L.ptr(phi), C.byref(res)),
             "ig_solve3d")
-    return phi, int(res.sweeps), float(res.delta), int(res.status)
+    return phi, int(res.sweeps), float(res.delta), int(res.status), \
+        int(res.level)
+
+
+def _sor_cap(shape, max_sweeps, d, tol):
+    nx, ny, nz = shape
+    return SolveNotConverged(
+        f"SOR hit its cap of {max_sweeps} sweeps on a {nx}x{ny}x{nz} "
+        f"grid without reaching tol: final delta {d:.3e} vs tol "
+        f"{tol:.3e} ({d / max(tol, 1e-30):.0f}x too loose). SOR stalls "
+        f"on long thin domains; solve this geometry with "
+        f"solve_bases_mg, or raise max_sweeps if you have reason to "
+        f"think it will actually converge.")
 
 
 def solve3d(fixed, val, theta=None, mirror=(False, False, False), tol=1e-6,
@@ -123,18 +143,11 @@ def solve3d(fixed, val, theta=None, mirror=(False, False, False), tol=1e-6,
     shape = np.shape(fixed)
     if omega is None:
         omega = optimal_omega(shape)
-    phi, sw, d, st = _run(fixed, val, theta, phi0, stencil, tol,
-                          method=L.SOLVE_SOR, omega=omega,
-                          max_sweeps=max_sweeps, engine=engine)
+    phi, sw, d, st, _ = _run(fixed, val, theta, phi0, stencil, tol,
+                             method=L.SOLVE_SOR, omega=omega,
+                             max_sweeps=max_sweeps, engine=engine)
     if st != 0:
-        nx, ny, nz = shape
-        raise SolveNotConverged(
-            f"SOR hit its cap of {max_sweeps} sweeps on a {nx}x{ny}x{nz} "
-            f"grid without reaching tol: final delta {d:.3e} vs tol "
-            f"{tol:.3e} ({d / max(tol, 1e-30):.0f}x too loose). SOR stalls "
-            f"on long thin domains; solve this geometry with "
-            f"solve_bases_mg, or raise max_sweeps if you have reason to "
-            f"think it will actually converge.")
+        raise _sor_cap(shape, max_sweeps, d, tol)
     if verbose:
         print(f"  SOR: {sw} sweeps, max|d| {d:.3e}")
     return phi, sw, d
@@ -158,6 +171,38 @@ def symmetric_axes(fixed, val, theta=None):
     return out
 
 
+def solve3d_mg(fixed, val, theta=None, mirror=(False, False, False),
+               tol=1e-6, max_sweeps=60000, verbose=False,
+               stencil="ghost_linear", min_coarse=9, max_levels=8,
+               level_tol_factor=30.0, engine=None):
+    """Cascadic multigrid (multigrid3d.py:175-230): (fixed, val) coarsened
+    while every axis exceeds ``min_coarse`` (at most ``max_levels`` grids),
+    the coarsest solved by SOR from zeros, each finer grid from the
+    prolongated coarser solution to ``tol * level_tol_factor``, the fine
+    grid with its fractions to ``tol``, so the result is the plain SOR
+    fixed point.  Returns (phi, fine sweep pairs, final delta); raises
+    SolveNotConverged, naming the grid, when any grid's SOR runs out of
+    ``max_sweeps``."""
+    _check_stencil(stencil)
+    shape = np.shape(fixed)
+    phi, sw, d, st, lv = _run(fixed, val, theta, None, stencil, tol,
+                              method=L.SOLVE_CASCADE,
+                              omega=optimal_omega(shape),
+                              max_sweeps=max_sweeps, min_coarse=min_coarse,
+                              max_levels=max_levels,
+                              level_tol_factor=level_tol_factor,
+                              engine=engine)
+    if st != 0:
+        g = tuple(shape)
+        for _ in range(lv):
+            g = tuple((n + 1) // 2 for n in g)
+        lt = tol if lv == 0 else tol * level_tol_factor
+        raise _sor_cap(g, max_sweeps, d, lt)
+    if verbose:
+        print(f"[mg] fine: {sw} sweeps, max|d| {d:.3e}")
+    return phi, sw, d
+
+
 def solve3d_vmg(fixed, val, theta=None, mirror=(False, False, False),
                 tol=1e-6, max_cycles=60, nu1=3, nu2=3, verbose=False,
                 stencil="ghost_linear", phi0=None, accept_stagnation=False,
@@ -169,10 +214,10 @@ def solve3d_vmg(fixed, val, theta=None, mirror=(False, False, False),
     _check_stencil(stencil)
     fixed = np.asarray(fixed, bool)
     sym = symmetric_axes(fixed, val, theta)
-    phi, sw, d, st = _run(fixed, val, theta, phi0, stencil, tol,
-                          method=L.SOLVE_VCYCLE, max_cycles=max_cycles,
-                          nu1=nu1, nu2=nu2,
-                          sym_axes=sum(1 << a for a in sym), engine=engine)
+    phi, sw, d, st, _ = _run(fixed, val, theta, phi0, stencil, tol,
+                             method=L.SOLVE_VCYCLE, max_cycles=max_cycles,
+                             nu1=nu1, nu2=nu2,
+                             sym_axes=sum(1 << a for a in sym), engine=engine)
     if verbose:
         print(f"[vmg] {sw} fine sweeps, cycle delta {d:.3e}")
     if st == 0:

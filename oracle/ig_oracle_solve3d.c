@@ -12,6 +12,9 @@
  *   or_optimal_omega solver3d.optimal_omega    (solver3d.py:201-226), unit spacing
  *   or_solve_err    multigrid3d._solve_err     (multigrid3d.py:553-599)
  *   or_solve3d_vmg  multigrid3d.solve3d_vmg    (multigrid3d.py:602-731)
+ *   or_coarsen_mask multigrid3d._coarsen_mask  (multigrid3d.py:84-135)
+ *   or_prolongate   multigrid3d._prolongate    (multigrid3d.py:138-172)
+ *   or_solve3d_mg   multigrid3d.solve3d_mg     (multigrid3d.py:175-230)
  *
  * Arrays are C order (nx, ny, nz), z fastest; the leg fractions theta are
  * (6, nx, ny, nz) float32 in the order -x, +x, -y, +y, -z, +z, or NULL for
@@ -415,4 +418,149 @@ int or_solve3d_vmg(double *phi, const uint8_t *fixed, const double *val, const f
   free(ec);
   free(fcm);
   return rc_status;
+}
+
+/* _coarsen_mask (multigrid3d.py:84-135): coarse node c takes the fine node
+ * 2c by injection; a coarse node whose injection point is vacuum becomes
+ * fixed when any node of the fine 2 x 2 x 2 block at 2c is, with the value
+ * of the block's fixed node of largest |val| (the first such in the order
+ * di, dj, dk with dk fastest, as argmax takes it).  Free coarse nodes keep
+ * the injected value, which no solve reads. */
+void or_coarsen_mask(const uint8_t *fixed, const double *val, int64_t nx, int64_t ny,
+                     int64_t nz, uint8_t *fc, double *vc, int64_t cx, int64_t cy, int64_t cz) {
+  for (int64_t ic = 0; ic < cx; ++ic)
+    for (int64_t jc = 0; jc < cy; ++jc)
+      for (int64_t kc = 0; kc < cz; ++kc) {
+        const int64_t c = (ic * cy + jc) * cz + kc;
+        const int64_t q = IX(2 * ic, 2 * jc, 2 * kc);
+        fc[c] = fixed[q];
+        vc[c] = val[q];
+        if (fixed[q]) continue;
+        double best = -1.0, v = 0.0;
+        for (int di = 0; di < 2; ++di)
+          for (int dj = 0; dj < 2; ++dj)
+            for (int dk = 0; dk < 2; ++dk) {
+              const int64_t i = 2 * ic + di, j = 2 * jc + dj, k = 2 * kc + dk;
+              if (i >= nx || j >= ny || k >= nz || !fixed[IX(i, j, k)]) continue;
+              const double m = fabs(val[IX(i, j, k)]);
+              if (m > best) {
+                best = m;
+                v = val[IX(i, j, k)];
+              }
+            }
+        if (best >= 0.0) {
+          fc[c] = 1;
+          vc[c] = v;
+        }
+      }
+}
+
+/* one axis of _prolongate (multigrid3d.py:144-156): fine index i sits at
+ * coarse coordinate min(i / 2, c - 1) */
+static inline void cas_axis(int64_t i, int64_t c, int64_t *a0, int64_t *a1, double *t) {
+  double xf = i / 2.0;
+  if (xf > (double)(c - 1)) xf = (double)(c - 1);
+  const int64_t x0 = (int64_t)floor(xf);
+  *a0 = x0;
+  *a1 = x0 + 1 < c - 1 ? x0 + 1 : c - 1;
+  *t = xf - (double)x0;
+}
+
+/* _prolongate (multigrid3d.py:138-172): the trilinear interpolant of the
+ * coarse field at every fine node, x first, then y, then z */
+void or_prolongate(const double *co, int64_t cx, int64_t cy, int64_t cz, double *out,
+                   int64_t nx, int64_t ny, int64_t nz) {
+  for (int64_t i = 0; i < nx; ++i) {
+    int64_t x0, x1;
+    double tx;
+    cas_axis(i, cx, &x0, &x1, &tx);
+    for (int64_t j = 0; j < ny; ++j) {
+      int64_t y0, y1;
+      double ty;
+      cas_axis(j, cy, &y0, &y1, &ty);
+      for (int64_t k = 0; k < nz; ++k) {
+        int64_t z0, z1;
+        double tz;
+        cas_axis(k, cz, &z0, &z1, &tz);
+#define C(a, b, c) co[((a) * cy + (b)) * cz + (c)]
+        const double c00 = C(x0, y0, z0) * (1 - tx) + C(x1, y0, z0) * tx;
+        const double c10 = C(x0, y1, z0) * (1 - tx) + C(x1, y1, z0) * tx;
+        const double c01 = C(x0, y0, z1) * (1 - tx) + C(x1, y0, z1) * tx;
+        const double c11 = C(x0, y1, z1) * (1 - tx) + C(x1, y1, z1) * tx;
+#undef C
+        const double c0 = c00 * (1 - ty) + c10 * ty;
+        const double c1 = c01 * (1 - ty) + c11 * ty;
+        out[IX(i, j, k)] = c0 * (1 - tz) + c1 * tz;
+      }
+    }
+  }
+}
+
+/* solve3d_mg (multigrid3d.py:175-230), the cascade: coarsen (fixed, val)
+ * while every axis exceeds min_coarse and halves to at least 3 nodes (at
+ * most max_levels grids), solve the coarsest from zeros to tol, each
+ * intermediate grid from the prolongated coarser solution to
+ * tol * level_tol_factor (all with plain unit fractions and the grid's own
+ * optimal omega), then the fine grid with its fractions, omega_fine and
+ * tol.  Returns 0 converged, or 1 with *level the grid (0 = fine) whose
+ * SOR ran out of sweeps; *sweeps and *delta are the fine solve's (or the
+ * failing grid's). */
+int or_solve3d_mg(double *phi, const uint8_t *fixed, const double *val, const float *th,
+                  int64_t nx, int64_t ny, int64_t nz, double tol, int64_t max_sweeps,
+                  double omega_fine, int gl, int min_coarse, int max_levels,
+                  double level_tol_factor, int64_t *sweeps, double *delta, int *level) {
+  enum { kMaxGrids = 64 };
+  int64_t dims[kMaxGrids][3];
+  uint8_t *fl[kMaxGrids];
+  double *vl[kMaxGrids];
+  int nl = 1;
+  dims[0][0] = nx;
+  dims[0][1] = ny;
+  dims[0][2] = nz;
+  fl[0] = (uint8_t *)fixed;
+  vl[0] = (double *)val;
+  while (nl < max_levels && nl < kMaxGrids) {
+    const int64_t *s = dims[nl - 1];
+    int64_t mn = s[0] < s[1] ? s[0] : s[1];
+    mn = mn < s[2] ? mn : s[2];
+    if (mn <= min_coarse || (mn + 1) / 2 < 3) break;
+    for (int a = 0; a < 3; ++a) dims[nl][a] = (s[a] + 1) / 2;
+    const int64_t m = dims[nl][0] * dims[nl][1] * dims[nl][2];
+    fl[nl] = (uint8_t *)malloc(m);
+    vl[nl] = (double *)malloc(m * sizeof(double));
+    or_coarsen_mask(fl[nl - 1], vl[nl - 1], s[0], s[1], s[2], fl[nl], vl[nl], dims[nl][0],
+                    dims[nl][1], dims[nl][2]);
+    ++nl;
+  }
+  int st = 0;
+  double *pc = NULL;
+  for (int l = nl - 1; l >= 0 && !st; --l) {
+    const int64_t *s = dims[l];
+    const int64_t m = s[0] * s[1] * s[2];
+    double *p = l == 0 ? phi : (double *)malloc(m * sizeof(double));
+    if (pc) {
+      const int64_t *c = dims[l + 1];
+      or_prolongate(pc, c[0], c[1], c[2], p, s[0], s[1], s[2]);
+      free(pc);
+      pc = NULL;
+    } else {
+      memset(p, 0, m * sizeof(double));
+    }
+    const double lt = (l == nl - 1 || l == 0) ? tol : tol * level_tol_factor;
+    const double om = l == 0 ? omega_fine : or_optimal_omega(s[0], s[1], s[2]);
+    int64_t sw = 0;
+    double d = INFINITY;
+    st = or_solve3d(p, fl[l], vl[l], l == 0 ? th : NULL, s[0], s[1], s[2], lt, max_sweeps, om,
+                    gl, &sw, &d);
+    *sweeps = sw;
+    *delta = d;
+    *level = l;
+    if (l > 0) pc = p;
+  }
+  if (pc) free(pc);
+  for (int l = 1; l < nl; ++l) {
+    free(fl[l]);
+    free(vl[l]);
+  }
+  return st;
 }

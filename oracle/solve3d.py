@@ -4,7 +4,7 @@
 TEST INFRASTRUCTURE ONLY: the checker the solver tests compare the
 engine's ``ig_solve3d`` against; the product package never imports it.
 Restates ion_gym/physics/solver3d.py:131-293 and
-ion_gym/physics/multigrid3d.py:305-731 (see the C file for the map).
+ion_gym/physics/multigrid3d.py:84-731 (see the C file for the map).
 """
 import ctypes as C
 
@@ -53,6 +53,16 @@ def lib():
                                      C.c_int, C.POINTER(_i64),
                                      C.POINTER(C.c_double)]
         L.or_solve3d_vmg.restype = C.c_int
+        L.or_coarsen_mask.argtypes = [_u8p, _dp, *g3, _u8p, _dp, *g3]
+        L.or_coarsen_mask.restype = None
+        L.or_prolongate.argtypes = [_dp, *g3, _dp, *g3]
+        L.or_prolongate.restype = None
+        L.or_solve3d_mg.argtypes = [_dp, _u8p, _dp, _fp, *g3, C.c_double,
+                                    _i64, C.c_double, C.c_int, C.c_int,
+                                    C.c_int, C.c_double, C.POINTER(_i64),
+                                    C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]
+        L.or_solve3d_mg.restype = C.c_int
         _L = L
     return _L
 
@@ -147,3 +157,42 @@ def solve3d_vmg(fixed, val, theta=None, tol=1e-6, max_cycles=60, nu1=3,
                               sum(1 << a for a in sym_axes), C.byref(sw),
                               C.byref(d))
     return phi, sw.value, d.value, st
+
+
+def coarsen_mask(fixed, val):
+    """(fixed, val) of the 2x-coarsened grid (multigrid3d._coarsen_mask)."""
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    val = np.ascontiguousarray(np.broadcast_to(val, fixed.shape), np.float64)
+    cs = tuple((n + 1) // 2 for n in fixed.shape)
+    fc = np.empty(cs, np.uint8)
+    vc = np.empty(cs)
+    lib().or_coarsen_mask(_p(fixed, _u8p), _p(val), *fixed.shape,
+                          _p(fc, _u8p), _p(vc), *cs)
+    return fc.astype(bool), vc
+
+
+def prolongate(coarse, shape):
+    """Trilinear prolongation onto a fine grid (multigrid3d._prolongate)."""
+    coarse = np.ascontiguousarray(coarse, np.float64)
+    out = np.empty(tuple(int(n) for n in shape))
+    lib().or_prolongate(_p(coarse), *coarse.shape, _p(out), *out.shape)
+    return out
+
+
+def solve3d_mg(fixed, val, theta=None, tol=1e-6, max_sweeps=60000,
+               omega=None, stencil="ghost_linear", min_coarse=9,
+               max_levels=8, level_tol_factor=30.0):
+    """(phi, fine sweep pairs, delta, status 0 converged / 1 a grid's SOR
+    ran out of sweeps, the grid: 0 the fine one, l the l-th coarsening)."""
+    fixed, val, th = _prep(fixed, val, theta)
+    phi = np.zeros(fixed.shape)
+    if omega is None:
+        omega = optimal_omega(fixed.shape)
+    sw, d, lv = _i64(), C.c_double(), C.c_int()
+    st = lib().or_solve3d_mg(_p(phi), _p(fixed, _u8p), _p(val), _p(th, _fp),
+                             *fixed.shape, float(tol), int(max_sweeps),
+                             float(omega), int(stencil == "ghost_linear"),
+                             int(min_coarse), int(max_levels),
+                             float(level_tol_factor), C.byref(sw),
+                             C.byref(d), C.byref(lv))
+    return phi, sw.value, d.value, st, lv.value

@@ -1,7 +1,8 @@
 """The engine's 3-D Laplace solver (ig_solve3d) against the oracle, bit for
 bit: SOR with both fractional-leg stencils, the V-cycle (thick and 1-node
 plates, a mirror-symmetric scene, leg fractions, a coarsest grid too big
-for one workgroup), plus the basis drivers and the refusals."""
+for one workgroup), the cascade (plates, leg fractions, odd and even
+axes), plus the basis drivers and the refusals."""
 import numpy as np
 import pytest
 
@@ -108,6 +109,38 @@ def test_vcycle_large_coarsest_grid_matches_oracle(eng):
                                      sym_axes=S.symmetric_axes(fixed, val))
     assert st == 0 and sw == sw_o and d == d_o
     _same(p, q)
+
+
+@pytest.mark.parametrize("thin", [False, True])
+def test_cascade_matches_oracle_bitwise(eng, thin):
+    from ion_gym_amd.physics import solver3d as S
+    fixed, val = _plate(n=48 if thin else 49, thin=thin)
+    print(f"cascade plate {fixed.shape} thin={thin}", flush=True)
+    p, sw, d = S.solve3d_mg(fixed, val, tol=1e-7, engine=eng)
+    q, sw_o, d_o, st, lv = O.solve3d_mg(fixed, val, tol=1e-7,
+                                        omega=S.optimal_omega(fixed.shape))
+    assert st == 0 and lv == 0 and sw == sw_o and d == d_o
+    _same(p, q)
+
+
+@pytest.mark.parametrize("stencil", ["ghost_linear", "sw"])
+def test_cascade_fractions_match_oracle_bitwise(eng, stencil):
+    from ion_gym_amd.physics import solver3d as S
+    inner, ring, exact, _ = _coax(81, 40.0, 6.3, 21)
+    val = np.where(inner, 100.0, 0.0) + np.where(ring, exact, 0.0)
+    fixed, th = inner | ring, coax_theta(inner, 40.0, 6.3)
+    print(f"cascade coax 81x81x21 {stencil}", flush=True)
+    p, sw, d = S.solve3d_mg(fixed, val, theta=th, tol=1e-8, stencil=stencil,
+                            engine=eng)
+    q, sw_o, d_o, st, lv = O.solve3d_mg(fixed, val, theta=th, tol=1e-8,
+                                        omega=S.optimal_omega(fixed.shape),
+                                        stencil=stencil)
+    assert st == 0 and sw == sw_o and d == d_o
+    _same(p, q)
+    with pytest.raises(S.SolveNotConverged,
+                       match="cap of 3 sweeps on a 21x21x6 grid"):
+        S.solve3d_mg(fixed, val, theta=th, tol=1e-8, max_sweeps=3,
+                     engine=eng)
 
 
 def test_exact_harmonic_and_bases(eng):

@@ -299,3 +299,119 @@ def test_optimal_omega_formula():
     assert abs(S.optimal_omega((n, n, n)) - want) < 1e-14
     assert abs(S.optimal_omega((n, n, 1)) - want) < 1e-14
     assert S.optimal_omega((1, 1, 1)) == 1.0
+
+
+# ---- the cascade (multigrid3d.solve3d_mg) ------------------------------
+
+def _coarsen_mask_numpy(fixed, val):
+    """The reference's vectorised coarsening (multigrid3d.py:84-135):
+    injection, then block-any by reshape with the block's largest-|val|
+    fixed node by argmax over the (di, dj, dk) axes moved last."""
+    nx, ny, nz = fixed.shape
+    cs = tuple((n + 1) // 2 for n in fixed.shape)
+    fc = np.zeros(cs, bool)
+    vc = np.zeros(cs)
+    fc[...] = fixed[::2, ::2, ::2]
+    vc[...] = val[::2, ::2, ::2]
+    px, py, pz = [n + (n & 1) for n in fixed.shape]
+    fpad = np.zeros((px, py, pz), bool)
+    fpad[:nx, :ny, :nz] = fixed
+    vpad = np.zeros((px, py, pz))
+    vpad[:nx, :ny, :nz] = np.where(fixed, val, 0.0)
+    fb = fpad.reshape(px // 2, 2, py // 2, 2, pz // 2, 2)
+    vb = vpad.reshape(px // 2, 2, py // 2, 2, pz // 2, 2)
+    anyf = fb.any(axis=(1, 3, 5))
+    mag = np.where(fb, np.abs(vb), -1.0).transpose(0, 2, 4, 1, 3, 5)
+    flat = vb.transpose(0, 2, 4, 1, 3, 5).reshape(cs + (8,))
+    idx = np.argmax(mag.reshape(cs + (8,)), axis=-1)
+    vany = np.take_along_axis(flat, idx[..., None], axis=-1)[..., 0]
+    rescue = anyf & ~fc
+    fc[rescue] = True
+    vc[rescue] = vany[rescue]
+    return fc, vc
+
+
+@pytest.mark.parametrize("shape", [(9, 8, 7), (10, 5, 12), (3, 3, 2)])
+def test_cascade_transfers_match_numpy_forms(shape):
+    rng = np.random.default_rng(7 + sum(shape))
+    fixed = rng.random(shape) < 0.15
+    # integer values with repeated magnitudes, so argmax ties occur
+    val = rng.integers(-3, 4, shape).astype(float)
+    fc, vc = S.coarsen_mask(fixed, val)
+    wf, wv = _coarsen_mask_numpy(fixed, val)
+    assert np.array_equal(fc, wf)
+    assert np.array_equal(vc[fc], wv[wf])
+    cs = fc.shape
+    coarse = rng.standard_normal(cs)
+    assert np.array_equal(S.prolongate(coarse, shape),
+                          _prolongate_numpy(coarse, shape))
+
+
+def test_coarsen_mask_cases():
+    """Injection wins when the injection node is metal; a free injection
+    node takes the block's largest |val|, the first in (di, dj, dk) order
+    on a tie; a 1-node plate on an odd plane survives coarsening."""
+    f = np.zeros((4, 4, 4), bool)
+    v = np.zeros(f.shape)
+    f[0, 0, 0], v[0, 0, 0] = True, 1.0          # injection node
+    f[1, 1, 1], v[1, 1, 1] = True, -9.0         # larger, ignored
+    f[2, 0, 1], v[2, 0, 1] = True, 4.0          # block (1,0,0): tie,
+    f[3, 0, 0], v[3, 0, 0] = True, -4.0         # the first (dk=1) wins
+    f[0, 2, 2], v[0, 2, 2] = True, 2.0          # block (0,1,1): injection
+    f[1, 3, 3], v[1, 3, 3] = True, -6.0         # node metal, keeps 2
+    fc, vc = S.coarsen_mask(f, v)
+    assert fc[0, 0, 0] and vc[0, 0, 0] == 1.0
+    assert fc[1, 0, 0] and vc[1, 0, 0] == 4.0
+    assert fc[0, 1, 1] and vc[0, 1, 1] == 2.0
+    assert fc.sum() == 3
+    plate = np.zeros((9, 7, 5), bool)
+    plate[5, 1:6, 1:4] = True
+    fc, vc = S.coarsen_mask(plate, np.where(plate, 7.0, 0.0))
+    assert fc.shape == (5, 4, 3)
+    assert fc[2, 1:3, 1:2].all() and fc.sum() == fc[2].sum()
+    assert (vc[fc] == 7.0).all()
+
+
+@pytest.mark.parametrize("shape", [(9, 7, 5), (8, 6, 4)])
+def test_prolongate_reproduces_affine_fields(shape):
+    """A coarse field affine in the coarse coordinates comes back exactly
+    at (i/2, j/2, k/2), each clamped to the last coarse node (dyadic
+    values, so the arithmetic is exact)."""
+    cs = tuple((n + 1) // 2 for n in shape)
+    ci, cj, ck = np.meshgrid(*[np.arange(c, dtype=float) for c in cs],
+                             indexing="ij")
+    a = 3.0 + 2.0 * ci - 5.0 * cj + 0.5 * ck
+    got = S.prolongate(a, shape)
+    xs = [np.minimum(np.arange(n) / 2.0, c - 1) for n, c in zip(shape, cs)]
+    fi, fj, fk = np.meshgrid(*xs, indexing="ij")
+    assert np.array_equal(got, 3.0 + 2.0 * fi - 5.0 * fj + 0.5 * fk)
+
+
+def test_cascade_reaches_the_sor_fixed_point():
+    """solve3d_mg converges to plain SOR's answer at the same tolerance
+    with fewer fine sweeps; on a grid too small to coarsen it is plain
+    SOR, bit for bit."""
+    fixed, val = _plate()
+    p, sw, d, st, lv = S.solve3d_mg(fixed, val, tol=1e-8)
+    q, sw_s, d_s, st_s = S.solve3d(fixed, val, tol=1e-8)
+    assert st == 0 and st_s == 0 and lv == 0 and d < 1e-8
+    assert sw < sw_s
+    assert np.abs(p - q).max() < 1e-5
+    f9, v9 = _plate(n=21, nz=9)
+    p, sw, d, st, lv = S.solve3d_mg(f9, v9, tol=1e-7)
+    q, sw_s, d_s, st_s = S.solve3d(f9, v9, tol=1e-7)
+    assert (sw, d, st) == (sw_s, d_s, st_s) and np.array_equal(p, q)
+
+
+def test_cascade_exact_quadratic_and_cap():
+    n = 33
+    i, j, k = np.meshgrid(*[np.arange(n, dtype=float)] * 3, indexing="ij")
+    exact = i * i + j * j - 2 * k * k
+    fixed = shell(n, n, n)
+    phi, sw, d, st, lv = S.solve3d_mg(fixed, np.where(fixed, exact, 0.0),
+                                      tol=1e-10)
+    assert st == 0 and np.abs(phi - exact)[~fixed].max() < 1e-6
+    fixed, val = _plate()
+    _, sw, d, st, lv = S.solve3d_mg(fixed, val, tol=1e-9, max_sweeps=3)
+    # the coarsest of 49x49x25 -> 25x25x13 -> 13x13x7 fails first
+    assert st == 1 and lv == 2 and sw == 3 and d >= 1e-9
