@@ -321,8 +321,8 @@ def solver_scene(shape):
 
 
 def run_solver(eng, shape, cpu_shape, cpu):
-    """The basis solve leg: one V-cycle solve to 1e-6 V of the scene above
-    on a SLIM-sized grid, the red-black SOR pair time (the memory-bound
+    """The basis solve leg: one V-cycle solve and one cascade solve to 1e-6
+    V of the scene above on a SLIM-sized grid, the red-black SOR pair time (the memory-bound
     stencil) from the difference of a 1000- and a 200-pair run, and, with
     cpu, the oracle's V-cycle on a smaller grid beside the engine's on the
     same grid (bitwise compared)."""
@@ -333,6 +333,11 @@ def run_solver(eng, shape, cpu_shape, cpu):
     t0 = time.perf_counter()
     phi, sw, d = S.solve3d_vmg(fixed, val, tol=1e-6, engine=eng)
     gpu_s = time.perf_counter() - t0
+    # the cascade (solve3d_mg) on the same basis, warm
+    S.solve3d_mg(fixed, val, tol=1e-6, engine=eng)
+    t0 = time.perf_counter()
+    _, sw_c, d_c = S.solve3d_mg(fixed, val, tol=1e-6, engine=eng)
+    cas_s = time.perf_counter() - t0
     runs = {}
     for pairs in (200, 1000):
         t0 = time.perf_counter()
@@ -349,6 +354,8 @@ def run_solver(eng, shape, cpu_shape, cpu):
                        f"{shape[2]} (one plate at 1e4 V, grounded walls)",
            "nodes": n, "tol_v": 1e-6, "fine_sweep_pairs": sw,
            "cycle_delta_v": d, "gpu_s": gpu_s,
+           "cascade": {"gpu_s": cas_s, "fine_sweep_pairs": sw_c,
+                       "delta_v": d_c},
            "sor_pair_us": pair_s * 1e6,
            "sor_roofline": {"bound": "hbm", "achieved": pair_bytes / pair_s
                             / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",

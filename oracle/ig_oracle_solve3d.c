@@ -27,8 +27,9 @@
  * the reference's numba build cannot run here; pinned by the reference's
  * own gates restated in tests (exact quadratic harmonic, second order on a
  * smooth harmonic, folded against full domain, fractional legs against
- * the coaxial closed form, V-cycle against SOR at the same tolerance, the
- * size-1 axis limits).
+ * the coaxial closed form, V-cycle and cascade against SOR at the same
+ * tolerance, the size-1 axis limits; the cascade's transfers against the
+ * reference's numpy forms and affine fields).
  */
 #include <math.h>
 #include <stdint.h>
