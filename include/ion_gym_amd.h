@@ -542,6 +542,46 @@ typedef struct {
 int ig_solve3d(ig_engine *h, const ig_solve3d_desc *d, double *phi,
                ig_solve3d_result *res);
 
+/* ---- 2-D Laplace solve (r-z and planar bases) ---- */
+
+#define IG_SYM_PLANAR 0      /* phi_zz + phi_uu = 0 */
+#define IG_SYM_CYLINDRICAL 1 /* phi_zz + phi_rr + phi_r / r = 0, u = r >= 0 */
+#define IG_EDGE_Z0 1
+#define IG_EDGE_Z1 2
+#define IG_EDGE_U0 4
+#define IG_EDGE_U1 8
+
+typedef struct {
+  int64_t nz, nu;         /* grid (nz, nu), u fastest; both at least 2 */
+  const uint8_t *fixed;   /* Dirichlet mask (host) */
+  const double *val;      /* potentials at the fixed nodes (host) */
+  const double *phi0;     /* initial guess or NULL for zeros */
+  double h;               /* node spacing, both axes */
+  int32_t symmetry;       /* IG_SYM_* */
+  int32_t neumann;        /* open edges, IG_EDGE_* bits (u0 ignored when
+                             cylindrical: the axis needs no ghost) */
+  int32_t ghost_linear;   /* open edges whose missing neighbour folds onto
+                             the centre node ('ghost_linear'); the others
+                             fold onto the opposite neighbour ('mirror') */
+  double tol;             /* stop when a red-black pair moves no node by
+                             tol volts */
+  double omega;           /* SOR relaxation factor in (0, 2) */
+  int64_t max_sweeps;     /* red-black pairs allowed */
+} ig_solve2d_desc;
+
+/* Solve the 2-D system of solver2d.assemble_laplace
+ * (ion_gym/physics/solver2d.py:31-124): planar interior centre -4 with
+ * unit neighbours, cylindrical interior radial neighbours 1 +- h/(2r),
+ * the axis centre -6 with the first off-axis node 4, open edges folded
+ * per edge_ghost, fixed nodes Dirichlet.  The reference factors it by a
+ * sparse LU (laplace_factor / solve_laplace, solver2d.py:127-183); here
+ * red-black SOR iterates it on the device to tol, the result written to
+ * phi (host, nz * nu).  A free node on a closed edge is refused, as the
+ * reference's assembly refuses it.  res->status 0 converged, 1 the sweep
+ * limit was reached. */
+int ig_solve2d(ig_engine *h, const ig_solve2d_desc *d, double *phi,
+               ig_solve3d_result *res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,6 +144,18 @@ class Solve3dResult(C.Structure):
                 ("status", C.c_int32), ("level", C.c_int32)]
 
 
+class Solve2dDesc(C.Structure):
+    _fields_ = [("nz", C.c_int64), ("nu", C.c_int64), ("fixed", _u8p),
+                ("val", _dp), ("phi0", _dp), ("h", C.c_double),
+                ("symmetry", C.c_int32), ("neumann", C.c_int32),
+                ("ghost_linear", C.c_int32), ("tol", C.c_double),
+                ("omega", C.c_double), ("max_sweeps", C.c_int64)]
+
+
+SYM_PLANAR, SYM_CYLINDRICAL = 0, 1
+EDGE_BITS = {"z0": 1, "z1": 2, "u0": 4, "u1": 8}
+
+
 # every entry point of include/ion_gym_amd.h: name -> (restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = {
@@ -195,6 +207,8 @@ SIGNATURES = {
                                             C.c_int64, C.c_int64,
                                             C.POINTER(C.c_int64)]),
     "ig_solve3d": (C.c_int, [_VP, C.POINTER(Solve3dDesc), _dp,
+                             C.POINTER(Solve3dResult)]),
+    "ig_solve2d": (C.c_int, [_VP, C.POINTER(Solve2dDesc), _dp,
                              C.POINTER(Solve3dResult)]),
 }
 

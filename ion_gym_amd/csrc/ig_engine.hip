@@ -1731,4 +1731,49 @@ int ig_solve3d(ig_engine *h, const ig_solve3d_desc *d, double *phi, ig_solve3d_r
   return 0;
 }
 
+int ig_solve2d(ig_engine *h, const ig_solve2d_desc *d, double *phi, ig_solve3d_result *res) {
+  if (!h || !d || !phi || !res || !d->fixed || !d->val)
+    return fail(IG_E_ARG, "ig_solve2d: null argument");
+  if (d->nz < 2 || d->nz > 65535 || d->nu < 2 || d->nu > 65535)
+    return fail(IG_E_ARG, "ig_solve2d: grid sizes must be in [2, 65535]");
+  if (d->symmetry != IG_SYM_PLANAR && d->symmetry != IG_SYM_CYLINDRICAL)
+    return fail(IG_E_ARG, "ig_solve2d: symmetry must be IG_SYM_PLANAR or IG_SYM_CYLINDRICAL");
+  if ((d->neumann & ~15) || (d->ghost_linear & ~15))
+    return fail(IG_E_ARG, "ig_solve2d: edge masks are 4-bit (z0 1, z1 2, u0 4, u1 8)");
+  if (!(d->h > 0.0) || !std::isfinite(d->h))
+    return fail(IG_E_ARG, "ig_solve2d: h must be positive and finite");
+  if (!(d->tol > 0.0) || !std::isfinite(d->tol))
+    return fail(IG_E_ARG, "ig_solve2d: tol must be positive and finite");
+  if (!(d->omega > 0.0 && d->omega < 2.0))
+    return fail(IG_E_ARG, "ig_solve2d: SOR omega must lie in (0, 2)");
+  if (d->max_sweeps < 0) return fail(IG_E_ARG, "ig_solve2d: max_sweeps < 0");
+  // a free node whose missing neighbour has a nonzero coefficient needs
+  // its edge open (solver2d.py:104-106); on the cylindrical axis the
+  // missing coefficient is zero
+  const int64_t nz = d->nz, nu = d->nu;
+  const bool cyl = d->symmetry == IG_SYM_CYLINDRICAL;
+  const struct {
+    int bit;
+    const char *name;
+  } edges[4] = {{IG_EDGE_Z0, "z0"}, {IG_EDGE_Z1, "z1"}, {IG_EDGE_U0, "u0"}, {IG_EDGE_U1, "u1"}};
+  for (const auto &e : edges) {
+    if (d->neumann & e.bit) continue;
+    if (cyl && e.bit == IG_EDGE_U0) continue;
+    const bool zedge = e.bit <= IG_EDGE_Z1;
+    const int64_t cnt = zedge ? nu : nz;
+    for (int64_t t = 0; t < cnt; ++t) {
+      const int64_t i = zedge ? (e.bit == IG_EDGE_Z0 ? 0 : nz - 1) : t;
+      const int64_t j = zedge ? t : (e.bit == IG_EDGE_U0 ? 0 : nu - 1);
+      if (!d->fixed[i * nu + j])
+        return fail(IG_E_ARG, std::string("ig_solve2d: free node with missing neighbour on "
+                                          "closed edge ") + e.name);
+    }
+  }
+  if (int rc = set_dev(h)) return rc;
+  std::string err;
+  if (int rc = ig::solve2d_run(h->stream, d, phi, res, err))
+    return fail(rc, "ig_solve2d: " + err);
+  return 0;
+}
+
 } // extern "C"

@@ -239,6 +239,66 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(double *__restrict__ phi,
   if (part) block_max(d, part);
 }
 
+// the 2-D five-point stencil of solver2d.assemble_laplace
+// (solver2d.py:31-124): u the radius when cyl (centre -6 and the first
+// off-axis node 4 on the axis, 1 +- h/2r off it); at an open edge the
+// missing neighbour's coefficient folds onto the opposite neighbour, or
+// onto the centre for the edges in ghost (bits: 1 z0, 2 z1, 4 u0, 8 u1)
+struct Lap2 {
+  double h;
+  int cyl;
+  int ghost;
+};
+
+// one colour sweep of SOR on that system, the grid held as (nz, 1, nu);
+// the caller has refused free nodes on closed edges
+__global__ __launch_bounds__(kBlock) void sweep2d_kernel(double *__restrict__ phi,
+                                                         const uint8_t *__restrict__ fixed,
+                                                         const Grid g, const Lap2 s, double omega,
+                                                         int colour, const int32_t *done,
+                                                         unsigned long long *part) {
+  if (done && *done) return;
+  int64_t i, j0, j;
+  double d = 0.0;
+  if (colour_node(g, colour, i, j0, j)) {
+    const int64_t nz = g.nx, nu = g.nz;
+    const int64_t q = i * nu + j;
+    if (!fixed[q]) {
+      double cup = 1.0, cdn = 1.0, centre = -4.0;
+      if (s.cyl) {
+        if (j == 0) {
+          cup = 4.0;
+          cdn = 0.0;
+          centre = -6.0;
+        } else {
+          const double r = (double)j * s.h;
+          cup = 1.0 + s.h / (2.0 * r);
+          cdn = 1.0 - s.h / (2.0 * r);
+        }
+      }
+      double acc = 0.0, diag = centre;
+      auto link = [&](double coef, bool exists, int64_t tq, int64_t mq, int edge) {
+        if (coef == 0.0) return;
+        if (exists)
+          acc += coef * phi[tq];
+        else if (s.ghost & edge)
+          diag += coef;
+        else
+          acc += coef * phi[mq];
+      };
+      link(1.0, i > 0, q - nu, q + nu, 1);
+      link(1.0, i < nz - 1, q + nu, q - nu, 2);
+      link(cdn, j > 0, q - 1, q + 1, 4);
+      link(cup, j < nu - 1, q + 1, q - 1, 8);
+      const double old = phi[q];
+      const double nw = old + omega * (-acc / diag - old);
+      d = fabs(nw - old);
+      phi[q] = nw;
+    }
+  }
+  if (part) block_max(d, part);
+}
+
 // after pair s (the partials of both colour sweeps): convergence when its
 // max |update| < tol, or < *tol_p when given (solver3d.py:281-287); last
 // and pairs, when given, record the pair
@@ -685,27 +745,24 @@ struct SorOut {
   int status = 1;
 };
 
-// red-black SOR of phi until a pair's max |update| < tol, at most
-// max_sweeps pairs (solver3d.py:276-293): the pair maximum and the done
-// flag are kept on the device, the host reads the flag every kSorChunk
-// pairs.  ctl[4, 7) holds the done flag, the last pair's delta and the
-// pair count; part takes both colours' per-workgroup maxima.
-bool sor_run(Ctx &x, double *phi, const uint8_t *fixed, const float *th, const Grid &g, int gl,
-             double omega, double tol, int64_t max_sweeps, unsigned long long *part,
-             unsigned long long *ctl, double *host2, SorOut &o) {
+// red-black SOR until a pair's max |update| < tol, at most max_sweeps
+// pairs (solver3d.py:276-293): sweep(c, done, part) launches colour c;
+// the pair maximum and the done flag are kept on the device, the host
+// reads the flag every kSorChunk pairs.  ctl[4, 7) holds the done flag,
+// the last pair's delta and the pair count; part takes both colours'
+// per-workgroup maxima, nb each.
+template <class Sweep>
+bool sor_drive(Ctx &x, Sweep sweep, int64_t nb, double tol, int64_t max_sweeps,
+               unsigned long long *part, unsigned long long *ctl, double *host2, SorOut &o) {
   int32_t *done = (int32_t *)(ctl + 4);
   double *last = (double *)(ctl + 5);
   int64_t *pairs = (int64_t *)(ctl + 6);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, x.st, ctl + 4, 3);
-  const dim3 hg = half_grid(g);
-  const int64_t nb = n_blocks(hg);
   int32_t *hp = reinterpret_cast<int32_t *>(host2);
   for (int64_t base = 0; base < max_sweeps; base += kSorChunk) {
     const int64_t end = std::min<int64_t>(base + kSorChunk, max_sweeps);
     for (int64_t s = base; s < end; ++s) {
-      for (int c = 0; c < 2; ++c)
-        hipLaunchKernelGGL(sweep_kernel, hg, dim3(kBlock), 0, x.st, phi, fixed, th, g, omega, c,
-                           gl, done, part + c * nb, nullptr);
+      for (int c = 0; c < 2; ++c) sweep(c, done, part + c * nb);
       hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, x.st, part, 2 * nb, done, s,
                          tol, nullptr, last, pairs);
     }
@@ -726,6 +783,18 @@ bool sor_run(Ctx &x, double *phi, const uint8_t *fixed, const float *th, const G
   o.status = fl ? 0 : 1;
   if (o.sweeps == 0) o.delta = INFINITY;
   return true;
+}
+
+// solve3d's SOR of phi on the 3-D stencil
+bool sor_run(Ctx &x, double *phi, const uint8_t *fixed, const float *th, const Grid &g, int gl,
+             double omega, double tol, int64_t max_sweeps, unsigned long long *part,
+             unsigned long long *ctl, double *host2, SorOut &o) {
+  const dim3 hg = half_grid(g);
+  auto sweep = [&](int c, const int32_t *done, unsigned long long *pt) {
+    hipLaunchKernelGGL(sweep_kernel, hg, dim3(kBlock), 0, x.st, phi, fixed, th, g, omega, c, gl,
+                       done, pt, nullptr);
+  };
+  return sor_drive(x, sweep, n_blocks(hg), tol, max_sweeps, part, ctl, host2, o);
 }
 
 // one level of the error hierarchy: its mask, right-hand side, solution
@@ -1018,6 +1087,48 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
     res->delta = cd;
     res->status = status;
   }
+  if (!x.ok(hipMemcpyAsync(phi_out, phi, n * 8, hipMemcpyDeviceToHost, st), "download phi") ||
+      !x.ok(hipStreamSynchronize(st), "sync"))
+    return x.rc;
+  return 0;
+}
+
+int solve2d_run(hipStream_t st, const ig_solve2d_desc *d, double *phi_out,
+                ig_solve3d_result *res, std::string &err) {
+  Ctx x{st, &err};
+  const Grid g{d->nz, 1, d->nu};
+  const int64_t n = g.n();
+  std::vector<double> init(n);
+  for (int64_t q = 0; q < n; ++q) {
+    const double v0 = d->phi0 ? d->phi0[q] : 0.0;
+    init[q] = d->fixed[q] ? d->val[q] : v0;
+  }
+  Buf phi_b, fixed_b, ctl_b, part_b;
+  double *phi = x.alloc<double>(phi_b, n);
+  uint8_t *fixed = x.alloc<uint8_t>(fixed_b, n);
+  unsigned long long *ctl = x.alloc<unsigned long long>(ctl_b, 8);
+  const dim3 hg = half_grid(g);
+  const int64_t nb = n_blocks(hg);
+  unsigned long long *part = x.alloc<unsigned long long>(part_b, 2 * nb);
+  PinBuf pin;
+  if (!x.rc && !x.ok(hipHostMalloc(&pin.p, 64, hipHostMallocDefault), "hipHostMalloc"))
+    return x.rc;
+  if (x.rc) return x.rc;
+  x.ok(hipMemcpyAsync(phi, init.data(), n * 8, hipMemcpyHostToDevice, st), "upload phi");
+  x.ok(hipMemcpyAsync(fixed, d->fixed, n, hipMemcpyHostToDevice, st), "upload mask");
+  if (x.rc) return x.rc;
+  const Lap2 lap{d->h, d->symmetry == IG_SYM_CYLINDRICAL, d->ghost_linear};
+  auto sweep = [&](int c, const int32_t *done, unsigned long long *pt) {
+    hipLaunchKernelGGL(sweep2d_kernel, hg, dim3(kBlock), 0, st, phi, fixed, g, lap, d->omega, c,
+                       done, pt);
+  };
+  SorOut o;
+  if (!sor_drive(x, sweep, nb, d->tol, d->max_sweeps, part, ctl, (double *)pin.p, o))
+    return x.rc;
+  res->sweeps = o.sweeps;
+  res->delta = o.delta;
+  res->status = o.status;
+  res->level = 0;
   if (!x.ok(hipMemcpyAsync(phi_out, phi, n * 8, hipMemcpyDeviceToHost, st), "download phi") ||
       !x.ok(hipStreamSynchronize(st), "sync"))
     return x.rc;
