@@ -381,6 +381,36 @@ def run_solver(eng, shape, cpu_shape, cpu):
         out["speedup_vs_cpu"] = cpu_s / gpu_small
         out["bitwise_equal_to_cpu"] = bool(st == 0 and swc == swg
                                            and np.array_equal(pc, pg))
+    out["rz_2d"] = run_solver_2d(eng, cpu)
+    return out
+
+
+def run_solver_2d(eng, cpu):
+    """The 2-D leg: the funnel's 1201 x 201 r-z grid (100 rings on an
+    axial ladder of 100 -> 1 V, open ends and outer radius), device SOR
+    to 1e-12 of the scale, warm; with cpu, the oracle's sparse LU of the
+    same system on the host and the largest difference."""
+    from ion_gym_amd.configs import _grid_labels
+    from ion_gym_amd.physics import solver2d as S2
+    h = 0.1
+    elec = [(i + 1, 10.0 + i, 10.5 + i, 12.5 - 11.5 * i / 99.0,
+             15.5 - 11.5 * i / 99.0) for i in range(100)]
+    lab = _grid_labels(1201, 201, h, elec)
+    val = np.where(lab > 0, 100.0 - (lab - 1) * 1.0, 0.0)
+    edges = ("z0", "z1", "u1")
+    solve = S2.laplace_factor(lab > 0, h, "cylindrical", edges, engine=eng)
+    solve(val)
+    t0 = time.perf_counter()
+    p = solve(val)
+    out = {"workload": "r-z basis solve 1201x201 (funnel rings, ladder "
+                       "100 -> 1 V)", "gpu_s": time.perf_counter() - t0,
+           "sor_pairs": solve.sweeps}
+    if cpu:
+        from oracle import solve2d as O2
+        t0 = time.perf_counter()
+        q, _ = O2.solve(lab > 0, val, h, "cylindrical", edges)
+        out["cpu_lu_s"] = time.perf_counter() - t0
+        out["max_abs_diff_v"] = float(np.abs(p - q).max())
     return out
 
 
