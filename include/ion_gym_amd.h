@@ -577,8 +577,9 @@ typedef struct {
  * sparse LU (laplace_factor / solve_laplace, solver2d.py:127-183); here
  * red-black SOR iterates it on the device to tol, the result written to
  * phi (host, nz * nu).  A free node on a closed edge is refused, as the
- * reference's assembly refuses it.  res->status 0 converged, 1 the sweep
- * limit was reached. */
+ * reference's assembly refuses it.  Convergence is measured every 8th
+ * pair, so res->sweeps is a multiple of 8 unless the limit ended the run.
+ * res->status 0 converged, 1 the sweep limit was reached. */
 int ig_solve2d(ig_engine *h, const ig_solve2d_desc *d, double *phi,
                ig_solve3d_result *res);
 
