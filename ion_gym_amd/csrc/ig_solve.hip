@@ -180,6 +180,19 @@ __device__ __forceinline__ unsigned long long reduce_parts(const unsigned long l
   return t;
 }
 
+// the first stage of a large pair maximum: workgroup b reduces its
+// contiguous share of part[0, count) to out[b] (nothing once done is set)
+__global__ __launch_bounds__(1024) void chunk_max_kernel(const unsigned long long *part,
+                                                         int64_t count, const int32_t *done,
+                                                         unsigned long long *out) {
+  if (*done) return;
+  const int64_t chunk = (count + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t len = lo < count ? (count - lo < chunk ? count - lo : chunk) : 0;
+  const unsigned long long m = reduce_parts(part + lo, len);
+  if (threadIdx.x == 0) out[blockIdx.x] = m;
+}
+
 __global__ __launch_bounds__(1024) void reduce_kernel(const unsigned long long *part,
                                                       int64_t count, unsigned long long *out) {
   const unsigned long long m = reduce_parts(part, count);
@@ -762,10 +775,17 @@ struct SorOut {
 // per-workgroup maxima, nb each.  With every > 1 only every every-th pair
 // (and the last allowed) is measured, so the pair count is a multiple of
 // every unless the limit ends the run.
+// A pair of more than kTwoStage partials is reduced in two stages when
+// stage2 (kStage2 words) is given: kStage2 workgroups share the partials,
+// one reads their maxima (one workgroup reading 51k partials of the 13M-node
+// board takes 15 us).
+constexpr int64_t kTwoStage = 4096;
+constexpr unsigned kStage2 = 64;
 template <class Sweep>
 bool sor_drive(Ctx &x, Sweep sweep, int64_t nb, double tol, int64_t max_sweeps,
                unsigned long long *part, unsigned long long *ctl, double *host2, SorOut &o,
-               int64_t every = 1) {
+               int64_t every = 1, unsigned long long *stage2 = nullptr) {
+  const bool two = stage2 && 2 * nb > kTwoStage;
   int32_t *done = (int32_t *)(ctl + 4);
   double *last = (double *)(ctl + 5);
   int64_t *pairs = (int64_t *)(ctl + 6);
@@ -776,9 +796,15 @@ bool sor_drive(Ctx &x, Sweep sweep, int64_t nb, double tol, int64_t max_sweeps,
     for (int64_t s = base; s < end; ++s) {
       const bool chk = (s + 1) % every == 0 || s + 1 == max_sweeps;
       for (int c = 0; c < 2; ++c) sweep(c, done, chk ? part + c * nb : nullptr);
-      if (chk)
+      if (chk && two) {
+        hipLaunchKernelGGL(chunk_max_kernel, dim3(kStage2), dim3(1024), 0, x.st, part, 2 * nb,
+                           done, stage2);
+        hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, x.st, stage2,
+                           (int64_t)kStage2, done, s, tol, nullptr, last, pairs);
+      } else if (chk) {
         hipLaunchKernelGGL(pair_finish_kernel, dim3(1), dim3(1024), 0, x.st, part, 2 * nb, done,
                            s, tol, nullptr, last, pairs);
+      }
     }
     if (!x.launched()) return false;
     if (!x.ok(hipMemcpyAsync(hp, done, 4, hipMemcpyDeviceToHost, x.st), "read flag") ||
@@ -802,13 +828,13 @@ bool sor_drive(Ctx &x, Sweep sweep, int64_t nb, double tol, int64_t max_sweeps,
 // solve3d's SOR of phi on the 3-D stencil
 bool sor_run(Ctx &x, double *phi, const uint8_t *fixed, const float *th, const Grid &g, int gl,
              double omega, double tol, int64_t max_sweeps, unsigned long long *part,
-             unsigned long long *ctl, double *host2, SorOut &o) {
+             unsigned long long *ctl, double *host2, SorOut &o, unsigned long long *stage2) {
   const dim3 hg = half_grid(g);
   auto sweep = [&](int c, const int32_t *done, unsigned long long *pt) {
     hipLaunchKernelGGL(sweep_kernel, hg, dim3(kBlock), 0, x.st, phi, fixed, th, g, omega, c, gl,
                        done, pt, nullptr);
   };
-  return sor_drive(x, sweep, n_blocks(hg), tol, max_sweeps, part, ctl, host2, o);
+  return sor_drive(x, sweep, n_blocks(hg), tol, max_sweeps, part, ctl, host2, o, 1, stage2);
 }
 
 // one level of the error hierarchy: its mask, right-hand side, solution
@@ -939,8 +965,8 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
   // per-workgroup maxima: both colours of a fine sweep pair at most
   const int64_t nb_fine = n_blocks(half_grid(g));
   Buf part_b;
-  unsigned long long *part =
-      x.alloc<unsigned long long>(part_b, 2 * std::max<int64_t>(nb_fine, kMaxDiffBlocks));
+  const int64_t n_part = 2 * std::max<int64_t>(nb_fine, kMaxDiffBlocks);
+  unsigned long long *part = x.alloc<unsigned long long>(part_b, n_part + kStage2);
   if (x.rc) return x.rc;
   double *dev2 = (double *)(ctl + 2);
   int32_t *done = (int32_t *)(ctl + 4);
@@ -956,7 +982,8 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
 
   if (d->method == IG_SOLVE_SOR) {
     SorOut o;
-    if (!sor_run(x, phi, fixed, th, g, gl, d->omega, d->tol, d->max_sweeps, part, ctl, host2, o))
+    if (!sor_run(x, phi, fixed, th, g, gl, d->omega, d->tol, d->max_sweeps, part, ctl, host2, o,
+                 part + n_part))
       return x.rc;
     res->sweeps = o.sweeps;
     res->delta = o.delta;
@@ -1001,7 +1028,7 @@ int solve3d_run(hipStream_t st, const ig_solve3d_desc *d, double *phi_out,
       const double lt = (l == nl - 1 || l == 0) ? d->tol : d->tol * d->level_tol_factor;
       const double om = l == 0 ? d->omega : host_omega(L.g);
       if (!sor_run(x, L.phi, L.fixed, l == 0 ? th : nullptr, L.g, gl, om, lt, d->max_sweeps, part,
-                   ctl, host2, o))
+                   ctl, host2, o, part + n_part))
         return x.rc;
       res->level = l;
       if (o.status) break;
