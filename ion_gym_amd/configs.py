@@ -290,7 +290,9 @@ def _drift_ions(n_ions, seed, mz, e_td, K0, P_torr, T_k, dt_frac, stats,
     n_gas = P_torr * (101325.0 / 760.0) / (K_B * T_k)   # 1/m^3
     E_vpm = e_td * 1e-21 * n_gas                         # V/m
     scale = E_vpm * 1e-3 / 1.0                           # base is 1 V/mm
-    p = sds_mod.transport_params(mz, 1, m_gas, T_k, P_torr, K0, stats=stats)
+    p = sds_mod.transport_params(
+        mz, 1, m_gas, 0.366, T_k, P_torr, gas_name="N2",
+        species=sds_mod.declared_K0(mz, "N2", K0), stats=stats)
     dt = dt_frac / p["damping"]
     rng = np.random.default_rng(seed)
     m = np.full(n_ions, mz)
@@ -460,8 +462,9 @@ def drift_3d(n_ions=4000, seed=6000, coarse=1, e_vpmm=5.0, mz=500.0, K0=1.5,
                     EzK=np.zeros((0,) + shape))
     g = gas_mod.gas_kernel_scalars("N2", T_k=T_k, P_torr=P_torr)
     stats = sds_mod.default_statistics()
-    p = sds_mod.transport_params(mz, 1, g["m_gas"], T_k, P_torr, K0,
-                                 stats=stats)
+    p = sds_mod.transport_params(
+        mz, 1, g["m_gas"], 0.366, T_k, P_torr, gas_name="N2",
+        species=sds_mod.declared_K0(mz, "N2", K0), stats=stats)
     dt = dt_frac / p["damping"]
     rng = np.random.default_rng(seed)
     m = np.full(n_ions, mz)

@@ -18,13 +18,13 @@ import math
 
 import numpy as np
 
-from .. import _lib as L
 from ..engine import BASE_CHANNELS, Flight
 from .build_rz import RECORD_BUDGET_BYTES, shared_engine
-from .collision3d import gas_kernel_scalars
 from .constants import E_CHG, KG_AMU
 from .drift_extension import extend_ballistic, xyz_edge_fields, xyz_escape_face
-from .ion_envelope import birth_splat, make_summary, per_ion, xyz_birth_refusal
+from .flight_inputs import DeckGas, ion_arrays
+from .ion_envelope import (birth_splat, make_summary, mz_of, run_seed_base,
+                           xyz_birth_refusal)
 from .stations import all_planes
 
 # the optional channels the 3-D route fills, in the reference's
@@ -47,7 +47,8 @@ def kernel_planes(spec):
     return out
 
 
-def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
+def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
+                   verbose=False):
     """(fly_fn, col_names) for a 3-D pack (EAx, EAy, EAz, ExK, EyK, EzK
     V/mm, ele, h_mm, ch_*), births (N, 7) and a FlightSpec."""
     opt = [c for c in XYZ_CHANNELS if c in spec.integration.record_channels]
@@ -62,13 +63,8 @@ def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
         raise ValueError("charge = 0: an uncharged ion has no electric "
                          "acceleration. State the charge state "
                          "(source.charge).")
-    col = spec.collisions if (spec.collisions is not None
-                              and spec.collisions.enabled) else None
-    model_name = str(col.model).lower() if col is not None else ""
-    if col is not None and model_name not in ("hs", "sds"):
-        raise ValueError(f"unknown collisions.model {model_name!r} on the "
-                         f"3-D route; known models are 'hs' and 'sds'")
-    sds_on = model_name == "sds"
+    gas = DeckGas(spec, sds_stats)
+    sds_on = gas.sds_on
     if sds_on and "wrap_passes" in opt:
         raise ValueError("record channel 'wrap_passes' was requested but "
                          "the collisions.model='sds' kernel does not "
@@ -76,35 +72,26 @@ def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
     if sds_on and spec.transporter is not None:
         raise ValueError("the transporter belongs to the 3-D RK4 flight; "
                          "the SDS flight has none")
-    g = gas_kernel_scalars(col.gas if col else None,
-                           T_k=col.T_k if col else None,
-                           P_torr=col.P_torr if col else None)
-    gas = dict(model=(L.GAS_SDS if sds_on else L.GAS_HS) if col else
-               L.GAS_VACUUM,
-               T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
-               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
-               flow=tuple(col.flow_mm_us) if col else (0.0, 0.0, 0.0))
-    if sds_on:
-        from . import sds as sds_mod
-        gas["sds_stats"] = (sds_stats if sds_stats is not None
-                            else sds_mod.default_statistics())
     integ = spec.integration
+    dt_ns = spec.dt_ns()
     rec_every = max(int(integ.rec_every), 1)
     # rows a flight can write: the birth row, one per rec_every steps and
     # the terminal row (the reference's own estimate of the need,
     # build_stl3d.py:1165, plus one), at most max_records
-    need = int(integ.t_max_us * 1e3 / (integ.dt_ns * rec_every)) + 3
+    need = int(integ.t_max_us * 1e3 / (dt_ns * rec_every)) + 3
     n_rows = max(2, min(int(integ.max_records), need))
     # the flight clocks of the two 3-D kernels (tracer3d.py:765, 1117)
     if sds_on:
-        dt, t_max = integ.dt_ns * 1e-3, float(integ.t_max_us)
+        dt, t_max = dt_ns * 1e-3, float(integ.t_max_us)
     else:
-        dt, t_max = integ.dt_ns * 1e-9, integ.t_max_us * 1e-6
+        dt, t_max = dt_ns * 1e-9, integ.t_max_us * 1e-6
     planes = kernel_planes(spec)
-    flight = Flight(dt, t_max, rec_every=rec_every, rec_rows=n_rows, gas=gas,
+    flight = Flight(dt, t_max, rec_every=rec_every, rec_rows=n_rows,
+                    gas=gas.kernel,
                     planes=planes, transporter=spec.transporter)
     births = np.asarray(births, np.float64)
-    seed_base = spec.source.seed if spec.source.seed is not None else 0
+    precision = spec.engine_precision()
+    seed_base = run_seed_base(spec, verbose=verbose)
     metal = np.asarray(pack["ele"]).astype(bool)
     h = float(pack["h_mm"])
     shape = metal.shape
@@ -116,31 +103,8 @@ def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
         if "fp" not in state:
             eng = shared_engine(device)
             state["eng"] = eng
-            state["fp"] = eng.upload_xyz(pack, spec.precision)
+            state["fp"] = eng.upload_xyz(pack, precision)
         return state["eng"], state["fp"]
-
-    def _ion_arrays(idx):
-        n = len(idx)
-        b = births[idx]
-        env = [per_ion(spec.source.mz_list, spec.source.n_ions, i, seed_base)
-               for i in idx]
-        m = np.array([e.mz for e in env])
-        ions = dict(x=b[:, 0], y=b[:, 1], z=b[:, 2], vx=b[:, 3], vy=b[:, 4],
-                    vz=b[:, 5], tob=b[:, 6], m_amu=m,
-                    acc=q_e * E_CHG / (m * KG_AMU),
-                    sigma_m2=np.full(n, col.sigma_m2 if col else 0.0),
-                    seed=np.array([e.seed for e in env], np.int64))
-        if sds_on:
-            from . import sds as sds_mod
-            rows = []
-            for mi in m:
-                p = sds_mod.transport_params(mi, q_e, g["m_gas"], col.T_k,
-                                             col.P_torr, col.K0_cm2_per_Vs,
-                                             stats=gas["sds_stats"])
-                rows.append((p["damping"], p["mfp_mm"], p["V_mm_us"],
-                             p["log_mr_ratio"]))
-            ions["sds"] = np.array(rows)
-        return ions, env
 
     def _traj(rec, nrec, mz, ncol):
         """The reference's trajectory columns from the kernel rows (t, x,
@@ -209,9 +173,8 @@ def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
             err = xyz_birth_refusal(metal, h, b[0], b[1], b[2], i)
             if err is not None:
                 out[k] = birth_splat(err, col_names=col_names, birth=b,
-                                     mz=per_ion(spec.source.mz_list,
-                                                spec.source.n_ions, i,
-                                                seed_base).mz,
+                                     mz=mz_of(spec.source.mz_list,
+                                              spec.source.n_ions, i),
                                      label=f"ion {i}")
             else:
                 fly_idx.append(k)
@@ -222,7 +185,7 @@ def make_3d_fly_fn(pack, births, spec, *, device=0, sds_stats=None):
         chunk = max(1, min(len(fly_idx), RECORD_BUDGET_BYTES // per_ion_bytes))
         for c0 in range(0, len(fly_idx), chunk):
             ks = fly_idx[c0:c0 + chunk]
-            ions, env = _ion_arrays(idx[ks])
+            ions, env = ion_arrays(spec, gas, births, idx[ks], seed_base)
             res = eng.fly_batch(fp, flight, ions)
             if np.any(res["fate"] == -3):
                 j = int(np.nonzero(res["fate"] == -3)[0][0])

@@ -14,13 +14,11 @@ import math
 
 import numpy as np
 
-from .. import _lib as L
 from ..engine import BASE_CHANNELS, PLANAR_CHANNELS, Flight
 from .build_rz import RECORD_BUDGET_BYTES, shared_engine
-from .collision3d import gas_kernel_scalars
-from .constants import E_CHG, KG_AMU
 from .drift_extension import edge_field_max, extend_ballistic
-from .ion_envelope import (birth_splat, make_summary, per_ion,
+from .flight_inputs import DeckGas, ion_arrays
+from .ion_envelope import (birth_splat, make_summary, mz_of, run_seed_base,
                            planar_birth_refusal)
 from .stations import station_planes
 
@@ -40,7 +38,7 @@ def planar_edge_fields(pack):
 
 
 def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
-                       anchor_mm=(0.0, 0.0)):
+                       anchor_mm=(0.0, 0.0), verbose=False):
     """(fly_fn, col_names) for a planar pack (ExA, EyA, ExK, EyK V/m, ele,
     mm, ch_*), births (N, 7) in the spec's frame and a FlightSpec."""
     bad = [c for c in spec.integration.record_channels
@@ -57,26 +55,9 @@ def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
     if q_e == 0:
         raise ValueError("source.charge = 0: an uncharged ion has no "
                          "electric acceleration; state the charge state.")
-    col = spec.collisions if (spec.collisions is not None
-                              and spec.collisions.enabled) else None
-    model_name = str(col.model).lower() if col is not None else ""
-    if col is not None and model_name not in ("hs", "sds"):
-        raise ValueError(f"collisions.model {model_name!r} is not a known "
-                         f"model; the flight routes implement 'hs' and 'sds'")
-    sds_on = model_name == "sds"
-    g = gas_kernel_scalars(col.gas if col else None,
-                           T_k=col.T_k if col else None,
-                           P_torr=col.P_torr if col else None)
-    gas = dict(model=(L.GAS_SDS if sds_on else L.GAS_HS) if col else
-               L.GAS_VACUUM,
-               T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
-               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
-               flow=tuple(col.flow_mm_us) if col else (0.0, 0.0, 0.0))
-    if sds_on:
-        from . import sds as sds_mod
-        gas["sds_stats"] = (sds_stats if sds_stats is not None
-                            else sds_mod.default_statistics())
-    dt = spec.integration.dt_ns * 1e-3
+    gas = DeckGas(spec, sds_stats)
+    sds_on = gas.sds_on
+    dt = spec.dt_ns() * 1e-3
     rec_every = max(1, int(spec.integration.rec_every))
     n_steps = int(spec.integration.t_max_us / dt) + 2
     n_rows = n_steps // rec_every + 8
@@ -100,10 +81,11 @@ def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
              win[3] - off[t2])
         planes.append((a, float(v) - off[a], sgn, w, kind))
     flight = Flight(dt, spec.integration.t_max_us, rec_every=rec_every,
-                    rec_rows=n_rows, channels=chans, gas=gas,
+                    rec_rows=n_rows, channels=chans, gas=gas.kernel,
                     bounds=(tuple(flags), tuple(kvals)), planes=planes)
     births = np.asarray(births, np.float64)
-    seed_base = spec.source.seed if spec.source.seed is not None else 0
+    precision = spec.engine_precision()
+    seed_base = run_seed_base(spec, verbose=verbose)
     ele = np.asarray(pack["ele"])
     h = float(pack["mm"])
     nx, ny = ele.shape
@@ -115,31 +97,8 @@ def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
         if "fp" not in state:
             eng = shared_engine(device)
             state["eng"] = eng
-            state["fp"] = eng.upload_planar(pack, spec.precision)
+            state["fp"] = eng.upload_planar(pack, precision)
         return state["eng"], state["fp"]
-
-    def _ion_arrays(idx):
-        n = len(idx)
-        b = births[idx]
-        env = [per_ion(spec.source.mz_list, spec.source.n_ions, i, seed_base)
-               for i in idx]
-        m = np.array([e.mz for e in env])
-        ions = dict(x=b[:, 0] - anx, y=b[:, 1] - any_, z=b[:, 2], vx=b[:, 3],
-                    vy=b[:, 4], vz=b[:, 5], tob=b[:, 6], m_amu=m,
-                    acc=q_e * E_CHG / (m * KG_AMU),
-                    sigma_m2=np.full(n, col.sigma_m2 if col else 0.0),
-                    seed=np.array([e.seed for e in env], np.int64))
-        if sds_on:
-            from . import sds as sds_mod
-            rows = []
-            for mi in m:
-                p = sds_mod.transport_params(mi, q_e, g["m_gas"], col.T_k,
-                                             col.P_torr, col.K0_cm2_per_Vs,
-                                             stats=gas["sds_stats"])
-                rows.append((p["damping"], p["mfp_mm"], p["V_mm_us"],
-                             p["log_mr_ratio"]))
-            ions["sds"] = np.array(rows)
-        return ions, env
 
     def _finish(traj, kind, mz, ncol, i):
         if sds_on and "n_col" in col_names:
@@ -185,9 +144,8 @@ def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
             err = planar_birth_refusal(ele, h, b[0] - anx, b[1] - any_, i)
             if err is not None:
                 out[k] = birth_splat(err, col_names=col_names, birth=b,
-                                     mz=per_ion(spec.source.mz_list,
-                                                spec.source.n_ions, i,
-                                                seed_base).mz,
+                                     mz=mz_of(spec.source.mz_list,
+                                              spec.source.n_ions, i),
                                      label=f"ion {i}")
             else:
                 fly_idx.append(k)
@@ -198,7 +156,8 @@ def make_planar_fly_fn(pack, births, spec, *, device=0, sds_stats=None,
         chunk = max(1, min(len(fly_idx), RECORD_BUDGET_BYTES // per_ion_bytes))
         for c0 in range(0, len(fly_idx), chunk):
             ks = fly_idx[c0:c0 + chunk]
-            ions, env = _ion_arrays(idx[ks])
+            ions, env = ion_arrays(spec, gas, births, idx[ks], seed_base,
+                                   offset=(anx, any_))
             res = eng.fly_batch(fp, flight, ions)
             if np.any(res["fate"] == -3):
                 j = int(np.nonzero(res["fate"] == -3)[0][0])

@@ -13,13 +13,11 @@ import threading
 
 import numpy as np
 
-from .. import _lib as L
 from ..engine import BASE_CHANNELS, RZ_CHANNELS, Engine, Flight
-from .collision3d import gas_kernel_scalars
-from .constants import E_CHG, KG_AMU
 from .drift_extension import rz_edge_fields, rz_extend
-from .ion_envelope import (birth_splat, make_summary, per_ion,
-                           rz_birth_refusal)
+from .flight_inputs import DeckGas, ion_arrays
+from .ion_envelope import (birth_splat, make_summary, mz_of,
+                           run_seed_base, rz_birth_refusal)
 from .stations import all_planes
 
 _ENGINES = {}
@@ -39,9 +37,14 @@ def shared_engine(device=0):
 RECORD_BUDGET_BYTES = 1 << 30
 
 
-def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
+def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None,
+                   verbose=False):
     """(fly_fn, col_names) for an RZModel, births (N, 7) and a FlightSpec.
-    Refusals match the reference: an unknown record channel, charge 0."""
+    Refusals match the reference: an unknown record channel, charge 0, an
+    unknown collision or sigma model.  The step is the deck's (or the one
+    derived from its drives), the field storage its channel_dtype (or the
+    spec's engine precision), the cross section of each ion its sigma
+    model's."""
     chans = [c for c in RZ_CHANNELS if c in spec.integration.record_channels]
     bad = [c for c in spec.integration.record_channels
            if c not in RZ_CHANNELS]
@@ -53,34 +56,20 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
     if q_e == 0:
         raise ValueError("source.charge = 0: an uncharged ion has no "
                          "electric acceleration; state the charge state.")
-    col = spec.collisions if (spec.collisions is not None
-                              and spec.collisions.enabled) else None
-    sds_on = col is not None and col.model.lower() == "sds"
-    if col is not None and col.model.lower() not in ("hs", "sds"):
-        raise ValueError(f"collisions.model {col.model!r} is not a known "
-                         f"model; the flight routes implement 'hs' and 'sds'")
-    g = gas_kernel_scalars(col.gas if col else None,
-                           T_k=col.T_k if col else None,
-                           P_torr=col.P_torr if col else None)
-    gas = dict(model=(L.GAS_SDS if sds_on else L.GAS_HS) if col else
-               L.GAS_VACUUM,
-               T_k=g["T_k"], P_pa=g["P_pa"], c_star=g["c_star"],
-               c_bar=g["c_bar"], sig1d=g["sig1d"], m_gas=g["m_gas"],
-               flow=tuple(col.flow_mm_us) if col else (0.0, 0.0, 0.0))
-    if sds_on:
-        from . import sds as sds_mod
-        gas["sds_stats"] = (sds_stats if sds_stats is not None
-                            else sds_mod.default_statistics())
-    dt = spec.integration.dt_ns * 1e-3
+    gas = DeckGas(spec, sds_stats)
+    sds_on = gas.sds_on
+    dt = spec.dt_ns() * 1e-3
     t_max = spec.integration.t_max_us
     rec_every = max(int(spec.integration.rec_every), 1)
     n_rows = int(t_max / dt / rec_every) + 8
     n_rows = min(max(n_rows, 16), 4_000_000)
     flight = Flight(dt, t_max, rec_every=rec_every, rec_rows=n_rows,
-                    channels=chans, gas=gas,
+                    channels=chans, gas=gas.kernel,
                     bounds=spec.bounds.as_tuple(), planes=all_planes(spec))
     births = np.asarray(births, np.float64)
-    seed_base = spec.source.seed if spec.source.seed is not None else 0
+    precision = spec.engine_precision()
+    # the run's seed policy, resolved once per build (build_rz.py:670-673)
+    seed_base = run_seed_base(spec, verbose=verbose)
     state = {}
     # residual |E| on the open faces, the drift extension's guard input
     # (measured once per model, build_rz.py:650-669)
@@ -90,31 +79,8 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
         if "fp" not in state:
             eng = shared_engine(device)
             state["eng"] = eng
-            state["fp"] = eng.upload_rz(model.pack(), spec.precision)
+            state["fp"] = eng.upload_rz(model.pack(), precision)
         return state["eng"], state["fp"]
-
-    def _ion_arrays(idx):
-        n = len(idx)
-        b = births[idx]
-        env = [per_ion(spec.source.mz_list, spec.source.n_ions, i, seed_base)
-               for i in idx]
-        m = np.array([e.mz for e in env])
-        ions = dict(x=b[:, 0], y=b[:, 1], z=b[:, 2], vx=b[:, 3], vy=b[:, 4],
-                    vz=b[:, 5], tob=b[:, 6], m_amu=m,
-                    acc=q_e * E_CHG / (m * KG_AMU),
-                    sigma_m2=np.full(n, col.sigma_m2 if col else 0.0),
-                    seed=np.array([e.seed for e in env], np.int64))
-        if sds_on:
-            from . import sds as sds_mod
-            rows = []
-            for mi in m:
-                p = sds_mod.transport_params(mi, q_e, g["m_gas"], col.T_k,
-                                             col.P_torr, col.K0_cm2_per_Vs,
-                                             stats=gas["sds_stats"])
-                rows.append((p["damping"], p["mfp_mm"], p["V_mm_us"],
-                             p["log_mr_ratio"]))
-            ions["sds"] = np.array(rows)
-        return ions, env
 
     def batch(indices):
         """[(traj, summary)] for the given ion indices, flown together."""
@@ -127,9 +93,8 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
             err = rz_birth_refusal(model.ele, model.mm_per_gu, b[0], r, i)
             if err is not None:
                 out[k] = birth_splat(err, col_names=col_names, birth=b,
-                                     mz=per_ion(spec.source.mz_list,
-                                                spec.source.n_ions, i,
-                                                seed_base).mz,
+                                     mz=mz_of(spec.source.mz_list,
+                                              spec.source.n_ions, i),
                                      label=f"ion {i}")
             else:
                 fly_idx.append(k)
@@ -140,7 +105,7 @@ def make_rz_fly_fn(model, births, spec, *, device=0, sds_stats=None):
         chunk = max(1, min(len(fly_idx), RECORD_BUDGET_BYTES // per_ion_bytes))
         for c0 in range(0, len(fly_idx), chunk):
             ks = fly_idx[c0:c0 + chunk]
-            ions, env = _ion_arrays(idx[ks])
+            ions, env = ion_arrays(spec, gas, births, idx[ks], seed_base)
             res = eng.fly_batch(fp, flight, ions)
             if np.any(res["fate"] == -3):
                 j = int(np.nonzero(res["fate"] == -3)[0][0])

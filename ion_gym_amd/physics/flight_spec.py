@@ -1,13 +1,19 @@
 """The deck fields the flight path reads, as plain dataclasses.
 
-A subset of ion_gym's SimSpec (ion_gym/io/sim_spec.py: SourceSpec :1484,
-CollisionSpec :1767, IntegrationSpec :2041, BoundsSpec :2230-2262) with the
-same names, defaults where the reference has one, and units: enough to
-describe a flight of an already-built field model.
+A subset of ion_gym's SimSpec (ion_gym/io/sim_spec.py: RFGroupSpec
+:591-692, GeometrySpec.field_method/channel_dtype :1339-1340, SourceSpec
+:1484, CollisionSpec :1767, IntegrationSpec :2041, BoundsSpec :2230-2262)
+with the same names, defaults and units: enough to describe a flight of an
+already-built field model.  Two derived values follow the reference's
+rules: an unset step (``dt_ns = None``) is the fastest active drive
+period / 250, 1 ns without drives (sim_spec.py:99, 2622-2631), and the
+field storage precision follows ``geometry.channel_dtype``.
 """
 import math
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
+
+from .ccs import LEGACY_FIXED_SIGMA_M2, SIGMA_MODEL_DEFAULT
 
 FWHM_PER_SIGMA = 2.0 * math.sqrt(2.0 * math.log(2.0))
 SOURCE_DISTRIBUTIONS = ("point", "disc", "line", "grid", "box", "gaussian",
@@ -113,16 +119,107 @@ class SourceSpec:
                 f"nowhere to point and is IGNORED: ions are born at rest")
 
 
+# the stock step per drive period of a derived dt (sim_spec.py:99)
+RF_RECOMMENDED_STEPS_PER_PERIOD = 250
+RF_WAVEFORMS = ("sin", "square", "table")
+
+
+@dataclass
+class RFGroupSpec:
+    """A named drive group (sim_spec.py:591-692): every member electrode
+    carries V(t) = amplitude_v * w(t) + offset_v on top of its dc, with w
+    sin(2 pi f t + phase), the duty square (sign(sin) at duty 0.5) or a
+    table of dimensionless multipliers (hold or linear, clamped at its
+    ends), on the lab clock.  pe_mode concerns only the reference's
+    potential-surface display and is carried for decks that state it."""
+    name: str
+    frequency_hz: float = 5e5
+    amplitude_v: float = 50.0
+    phase_deg: float = 0.0
+    duty: float = 0.5
+    offset_v: float = 0.0
+    waveform: str = "sin"
+    table_t_us: list = field(default_factory=list)
+    table_v: list = field(default_factory=list)
+    interp: str = "hold"
+    pe_mode: Optional[str] = None
+
+    def validate(self):
+        """Raise ValueError for a group the flight cannot mean (the
+        reference's checks: duty in (0, 1), a known waveform, a table with
+        matching non-empty strictly increasing breakpoints and a known
+        interpolation)."""
+        if not 0.0 < float(self.duty) < 1.0:
+            raise ValueError(f"RF group {self.name!r}: duty {self.duty} "
+                             f"outside (0, 1)")
+        if self.waveform not in RF_WAVEFORMS:
+            raise ValueError(f"RF group {self.name!r}: unknown waveform "
+                             f"{self.waveform!r}")
+        if self.waveform != "table":
+            return
+        t, v = list(self.table_t_us), list(self.table_v)
+        if not t or len(t) != len(v):
+            raise ValueError(f"RF group {self.name!r}: table needs matching, "
+                             f"non-empty table_t_us/table_v")
+        if any(b <= a for a, b in zip(t, t[1:])):
+            raise ValueError(f"RF group {self.name!r}: table_t_us must be "
+                             f"strictly increasing")
+        if self.interp not in ("hold", "linear"):
+            raise ValueError(f"RF group {self.name!r}: interp must be "
+                             f"'hold' or 'linear'")
+
+    def resolved_pe_mode(self):
+        if self.pe_mode in ("pseudo", "instant"):
+            return self.pe_mode
+        return "pseudo" if self.waveform == "sin" else "instant"
+
+
+FIELD_METHODS = ("electrode_aware", "plain_gradient")
+CHANNEL_DTYPES = ("float64", "float32")
+
+
+@dataclass
+class GeometrySpec:
+    """The geometry fields the flight reads (sim_spec.py:1330-1345):
+    field_method (the 3-D field build: one-sided differences at metal
+    nodes, or np.gradient everywhere), channel_dtype (storage of the
+    channel stacks: float64, or float32 read back as float64) and the
+    drive groups."""
+    field_method: str = "electrode_aware"
+    channel_dtype: str = "float64"
+    rf_groups: List[RFGroupSpec] = field(default_factory=list)
+
+    def validate(self):
+        if self.field_method not in FIELD_METHODS:
+            raise ValueError(f"geometry.field_method {self.field_method!r}: "
+                             f"choose one of {FIELD_METHODS}")
+        if self.channel_dtype not in CHANNEL_DTYPES:
+            raise ValueError(f"geometry.channel_dtype "
+                             f"{self.channel_dtype!r}: choose one of "
+                             f"{CHANNEL_DTYPES}")
+        for g in self.rf_groups:
+            g.validate()
+
+
 @dataclass
 class CollisionSpec:
-    enabled: bool = False
+    """Buffer gas (sim_spec.py:1767-1830).  ``enabled`` False is vacuum.
+    Pressure is Torr (P_pa derived).  Hard sphere: the per-ion cross
+    section comes from ``sigma_model`` (ion_gym_amd.physics.ccs: 'ccs_fit'
+    by default, 'fixed' flies sigma_m2).  SDS: ``gas_diam_nm`` and
+    ``sds_species`` (declared K0 or diameter per ion mass; other masses
+    are estimated from mass, ion_gym_amd.physics.sds)."""
+    enabled: bool = True
     model: str = "hs"            # "hs" | "sds"
     gas: str = "N2"
     T_k: float = 273.0
     P_torr: float = 1.0
-    sigma_m2: float = 1.0e-18
+    sigma_m2: float = LEGACY_FIXED_SIGMA_M2
+    gas_diam_nm: float = 0.366
     flow_mm_us: Tuple[float, float, float] = (0.0, 0.0, 0.0)
-    K0_cm2_per_Vs: Optional[float] = None   # SDS: declared mobility
+    sigma_model: str = SIGMA_MODEL_DEFAULT
+    ccs_fit_class: str = "all"
+    sds_species: List[dict] = field(default_factory=list)
 
     @property
     def P_pa(self):
@@ -131,11 +228,15 @@ class CollisionSpec:
 
 @dataclass
 class IntegrationSpec:
-    dt_ns: float = 1.0
-    t_max_us: float = 50.0
-    rec_every: int = 10
-    record_channels: List[str] = field(default_factory=list)
-    max_records: int = 200000    # 3-D record buffer rows per ion
+    """The step and the recording (sim_spec.py:2041-2110).  ``dt_ns =
+    None`` asks for the step derived from the drives
+    (FlightSpec.derive_dt_ns)."""
+    dt_ns: Optional[float] = 1.0
+    t_max_us: float = 400.0
+    rec_every: int = 80
+    max_records: int = 100000    # 3-D record buffer rows per ion
+    record_channels: List[str] = field(
+        default_factory=lambda: ["speed", "ke_ev", "e_field"])
 
 
 @dataclass
@@ -160,19 +261,67 @@ class BoundsSpec:
                  self.z_min, self.z_max))
 
 
+# engine field storage per channel_dtype: the reference's float32 channels
+# are stored in FP32 and read back as FP64 (IG_PREC_F32_STORE)
+_STORAGE = {"float64": "f64", "float32": "f32_store"}
+PRECISIONS = ("f64", "f32_store", "f32")
+
+
 @dataclass
 class FlightSpec:
     source: SourceSpec = field(default_factory=SourceSpec)
     collisions: Optional[CollisionSpec] = None
     integration: IntegrationSpec = field(default_factory=IntegrationSpec)
     bounds: BoundsSpec = field(default_factory=BoundsSpec)
+    geometry: GeometrySpec = field(default_factory=GeometrySpec)
     # station planes: (axis 0/1/2, value_mm, sign, (a_lo, a_hi, b_lo, b_hi),
     # kind 5 impact plate / 6 detector)
     planes: List[tuple] = field(default_factory=list)
-    precision: str = "f64"       # engine field storage: f64 | f32_store | f32
+    # engine precision override: None follows geometry.channel_dtype;
+    # "f32" selects the FP32-basis engine (FP32 fields and interpolation,
+    # FP64 state)
+    precision: Optional[str] = None
     # the 3-D periodic transporter (sim_spec transporter; tracer3d.py:
     # 722-735): dict(axis, accept_mm, emit_mm, direction, max_passes)
     transporter: Optional[dict] = None
     # declared stations (ion_gym_amd.physics.stations.StationSpec): the
     # kernel-reaching ones become planes before ``planes``
     stations: List = field(default_factory=list)
+
+    def engine_precision(self):
+        """The engine's field precision for this deck."""
+        if self.precision is not None:
+            if self.precision not in PRECISIONS:
+                raise ValueError(f"precision {self.precision!r}: choose one "
+                                 f"of {PRECISIONS}")
+            return self.precision
+        self.geometry.validate()
+        return _STORAGE[self.geometry.channel_dtype]
+
+    def derive_dt_ns(self):
+        """The step a deck without one flies: the fastest drive period
+        with a nonzero frequency and amplitude / 250, or 1 ns for a
+        drive-less (DC) deck (sim_spec.py:2622-2631)."""
+        fs = [float(g.frequency_hz) for g in self.geometry.rf_groups
+              if g.frequency_hz and g.amplitude_v]
+        if not fs:
+            return 1.0
+        return 1e9 / (max(fs) * RF_RECOMMENDED_STEPS_PER_PERIOD)
+
+    def dt_ns(self):
+        """The step flown, ns: the stated one, or the derived one for
+        ``integration.dt_ns = None`` (refused with a table drive, whose
+        breakpoints no period describes, sim_spec.py:3140-3146)."""
+        dt = self.integration.dt_ns
+        if dt is not None:
+            dt = float(dt)
+            if not (math.isfinite(dt) and dt > 0.0):
+                raise ValueError(f"integration.dt_ns = {dt!r}: the step "
+                                 f"must be a finite number > 0")
+            return dt
+        if any(g.waveform == "table" for g in self.geometry.rf_groups):
+            raise ValueError(
+                "integration: dt_ns is unset with a table-waveform drive; a "
+                "table's breakpoints set no period to derive a step from -- "
+                "pin dt_ns explicitly")
+        return self.derive_dt_ns()

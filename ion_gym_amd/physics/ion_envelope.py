@@ -1,8 +1,8 @@
-"""The per-ion envelope: mass and seed per ion, the summary contract and
-the birth-splat outcome.  Restates ion_gym/physics/ion_envelope.py:26-223
-and the birth refusals of ion_gym/physics/raster2d.py:615-646 and
-ion_gym/physics/birth_domain.py:20-38 (same names, same messages' facts,
-same fates)."""
+"""The per-ion envelope: mass and seed per ion, the run's seed policy, the
+summary contract and the birth-splat outcome.  Restates
+ion_gym/physics/ion_envelope.py:26-223 and the birth refusals of
+ion_gym/physics/raster2d.py:615-646 and ion_gym/physics/birth_domain.py:20-38
+(same names, same facts in the messages, same fates)."""
 from collections import namedtuple
 
 import numpy as np
@@ -15,6 +15,7 @@ FATE_NAME = {0: "impact / exit", 1: "boundary exit", 2: "timeout",
              5: "station impact plane", 6: "station detect",
              UNKNOWN_FATE: "unknown (no fate recorded)"}
 DOMAIN_EDGE_TOL = 1e-9
+BASE_CHANNELS = ("t", "x", "y", "z", "vx", "vy", "vz")
 
 
 class BirthInMetal(ValueError):
@@ -27,6 +28,29 @@ class BirthOutsideDomain(BirthInMetal):
 
 def fate_name(kind):
     return FATE_NAME.get(int(kind), f"unknown fate {int(kind)}")
+
+
+def resolve_run_seed(spec):
+    """The run's seed (ion_envelope.py:56-79): source.seed when stated;
+    for source.seed = None one 31-bit draw of OS entropy, made once per
+    source, kept on it and printed, so births and every per-ion stream of
+    the run share it and the run can be repeated."""
+    from ..births import resolve_run_seed as _source_seed
+    return _source_seed(spec.source)
+
+
+def run_seed_base(spec, verbose=True):
+    """The base of the per-ion seeds (ion_envelope.py:26-50): a stated
+    source.seed is the fixed (common random numbers) policy, announced when
+    ``verbose``; None is the random policy of resolve_run_seed."""
+    raw = getattr(spec.source, "seed", 0)
+    if raw is None:
+        return resolve_run_seed(spec)
+    if verbose:
+        print(f"[seed] policy FIXED (CRN): source.seed = {int(raw)}; reruns "
+              f"are bit-identical -- set source.seed: null for fresh entropy "
+              f"per run")
+    return int(raw)
 
 
 def mz_of(mz_list, n_ions, i):
@@ -53,24 +77,26 @@ def make_summary(*, kind, tof, mz, x_end, y_end, z_end, **extras):
 
 
 def birth_splat(err, *, col_names, birth, mz, label):
-    """One-row record at birth and a fate-0 (in metal) or fate-1 (outside
-    the domain) summary (ion_envelope.py:184-223)."""
-    x, y, z, vx, vy, vz, tob = (float(v) for v in birth[:7])
+    """The outcome of a refused birth (ion_envelope.py:184-223): the other
+    ions fly on, this one ends where and when it was born.  In metal it
+    splats (fate 0, ``born_in_metal``); outside the solved domain it has
+    left it (fate 1, ``born_outside_domain``).  The record is one row with
+    the birth state in the base channels and NaN in every optional one;
+    n_col is None (the ion never flew).  A warning naming the ion and the
+    reason is printed."""
+    state = dict(zip(BASE_CHANNELS[1:], (float(v) for v in birth[:6])))
+    state["t"] = float(birth[6])
     outside = isinstance(err, BirthOutsideDomain)
-    print(f"[flight] WARNING: {label} was born "
-          + ("outside the solved domain and leaves it at birth"
-             if outside else "inside metal and splats at birth")
-          + f" -- {err}", flush=True)
-    traj = np.full((1, len(col_names)), np.nan)
-    for name, v in zip(("t", "x", "y", "z", "vx", "vy", "vz"),
-                       (tob, x, y, z, vx, vy, vz)):
-        if name in col_names:
-            traj[0, list(col_names).index(name)] = v
-    flag = (dict(born_outside_domain=True) if outside
-            else dict(born_in_metal=True))
-    summary = make_summary(kind=1 if outside else 0, tof=tob, mz=mz,
-                           x_end=x, y_end=y, z_end=z, n_col=None,
-                           note=str(err), **flag)
+    what = ("outside the solved domain and leaves it at birth" if outside
+            else "inside metal and splats at birth")
+    print(f"[flight] WARNING: {label} was born {what} -- {err}", flush=True)
+    row = [state.get(c, np.nan) for c in col_names]
+    traj = np.asarray([row], np.float64)
+    flag = "born_outside_domain" if outside else "born_in_metal"
+    summary = make_summary(kind=1 if outside else 0, tof=state["t"], mz=mz,
+                           x_end=state["x"], y_end=state["y"],
+                           z_end=state["z"], n_col=None, note=str(err),
+                           **{flag: True})
     return traj, summary
 
 

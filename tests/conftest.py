@@ -7,6 +7,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# SDS flights in the tests read ion_gym's own jump table (a fixture under
+# tests/golden), as a deployment inside ion_gym does
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+os.environ.setdefault("ION_GYM_SDS_TABLE",
+                      os.path.join(GOLDEN, "sds_jump_icdf.dat"))
+
 
 def pytest_configure(config):
     config.addinivalue_line(

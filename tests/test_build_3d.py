@@ -92,5 +92,5 @@ def test_refusals_before_flight():
     with pytest.raises(ValueError, match="wrap_passes"):
         B3.make_3d_fly_fn(pack, births, spec)
     spec = FlightSpec(collisions=CollisionSpec(enabled=True, model="ems"))
-    with pytest.raises(ValueError, match="unknown collisions.model"):
+    with pytest.raises(ValueError, match="is not a known model"):
         B3.make_3d_fly_fn(pack, births, spec)

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from ion_gym_amd.physics.sds import declared_K0
 from ion_gym_amd.physics.flight_spec import (BoundsSpec, CollisionSpec,
                                              FlightSpec, IntegrationSpec,
                                              SourceSpec)
@@ -72,7 +73,8 @@ def test_sds_cell_fly_fn_matches_oracle():
     spec = FlightSpec(
         source=SourceSpec(n_ions=n, mz_list=[500.0], seed=6000),
         collisions=CollisionSpec(enabled=True, model="sds", gas="N2",
-                                 T_k=300.0, P_torr=4.0, K0_cm2_per_Vs=1.5),
+                                 T_k=300.0, P_torr=4.0,
+                                 sds_species=[declared_K0(500.0, "N2", 1.5)]),
         integration=IntegrationSpec(dt_ns=c["dt"] * 1e3, t_max_us=300.0,
                                     rec_every=200,
                                     record_channels=["e_z", "n_col"]),

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from ion_gym_amd.physics.sds import declared_K0
 from ion_gym_amd.physics.flight_spec import (BoundsSpec, CollisionSpec,
                                              FlightSpec, IntegrationSpec,
                                              SourceSpec)
@@ -91,7 +92,8 @@ def test_planar_fly_fn_sds_refusal_and_drift_extension():
     ions = c["ions"]
     spec = _spec(c, ions, collisions=CollisionSpec(
         enabled=True, model="sds", gas="He", T_k=300.0, P_torr=0.05,
-        K0_cm2_per_Vs=1.5))
+        sds_species=[declared_K0(m, "He", 1.5)
+                     for m in sorted(set(ions["m_amu"]))]))
     fly_fn, cols = make_planar_fly_fn(c["pack"], _births(ions), spec)
     for tr, s in fly_fn.batch(range(40)):
         assert s["n_col"] is None

@@ -100,8 +100,10 @@ def test_planar_sds_matches_oracle_philox(engine, quad):
     gas = _gas(2)
     gas["sds_stats"] = sds_mod.default_statistics()
     ions = {k: v[:200] for k, v in c["ions"].items()}
-    p = sds_mod.transport_params(500.0, 1, gas["m_gas"], 300.0, 0.05, 1.5,
-                                 stats=gas["sds_stats"])
+    p = sds_mod.transport_params(
+        500.0, 1, gas["m_gas"], 0.366, 300.0, 0.05, gas_name="N2",
+        species=sds_mod.declared_K0(500.0, "N2", 1.5),
+        stats=gas["sds_stats"])
     ions["sds"] = np.tile([p["damping"], p["mfp_mm"], p["V_mm_us"],
                            p["log_mr_ratio"]], (200, 1))
     fp = engine.upload_planar(c["pack"], "f64")

@@ -244,7 +244,10 @@ def test_hard_sphere_thermalises_to_three_halves_kT():
 def test_sds_mean_square_jump_is_6Ddt():
     from ion_gym_amd.physics import sds
     st = sds.default_statistics()
-    p = sds.transport_params(500.0, 1, 28.0134, 300.0, 4.0, 1.5, stats=st)
+    p = sds.transport_params(500.0, 1, 28.0134, 0.366, 300.0, 4.0,
+                             gas_name="N2",
+                             species=sds.declared_K0(500.0, "N2", 1.5),
+                             stats=st)
     dt = 1e-3
     # draw jumps with the oracle's kernel formula: E[r^2] = (V/mfp dt / N)
     # mfp^2 <d^2> = 6 D dt by the choice of mfp (sds.py:401)
