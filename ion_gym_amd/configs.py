@@ -136,6 +136,46 @@ def funnel_rz(n_ions=1_000_000, seed=3, coarse=1, dt_ns=0.25):
                 dt=dt_ns * 1e-3, t_max=2000.0, gas=gas, charge=q)
 
 
+def drift_winner_rz(n_ions=500, seed=60601, coarse=1, dt_ns=0.25):
+    """ion_gym's certified drift-tube example (notebooks/data/
+    20_drift_tube_matrix/drift_winner_spec.json): 40 rings 0.5 mm wide at
+    2.1 mm pitch from x = 2 mm, inner radius 10 mm, outer 15 mm, DC ladder
+    16.38 V -> 0 V in 0.42 V steps (0.2 V/mm, 6.9 Td), no RF; C60 (m/z
+    720, z = +1) in He at 1 Torr and 298 K, hard sphere with a fixed
+    1.25e-18 m^2; births on a 0.3 mm disc at x = 6 mm, KE uniform
+    0.1-1.9 eV along +x, tob uniform over 2 us, seed 60601; 1600 us of
+    flight at dt 0.25 ns (the deck's certified step, run_winner.py:49),
+    records every 1 us.  The deck's 86.4 x 15.5 mm domain at 0.125 mm is
+    691.2 cells long; the grid here is 691 cells (86.375 mm), the nearest
+    conforming extent, which ends 1.9 mm past the last ring.
+
+    Returns dict(model, spec (FlightSpec), source (SourceSpec), h)."""
+    from .physics.flight_spec import (CollisionSpec, FlightSpec,
+                                      IntegrationSpec, SourceSpec)
+    h = 0.125 * coarse
+    nz, nr = 691 // coarse + 1, 124 // coarse + 1
+    elec = [(k + 1, 2.0 + 2.1 * k, 2.5 + 2.1 * k, 10.0, 15.0)
+            for k in range(40)]
+    lab = _grid_labels(nz, nr, h, elec)
+    dc = {k + 1: round(16.38 - 0.42 * k, 10) for k in range(40)}
+    (A,) = solve_rz(lab, [dc], h)
+    model = _model_from_potentials(lab, h, A, [], [], [], [], [])
+    src = SourceSpec(n_ions=n_ions, mz_list=[720.0], charge=1, seed=seed,
+                     distribution="disc", x0_mm=6.0, r_mm=0.3, axis="x",
+                     direction=[1.0, 0.0, 0.0], ke_lo=0.1, ke_hi=1.9,
+                     temperature_k=0.0, tob_span_us=2.0)
+    spec = FlightSpec(
+        source=src,
+        collisions=CollisionSpec(gas="He", T_k=298.0,
+                                 P_torr=0.9999972366148532, model="hs",
+                                 sigma_model="fixed", sigma_m2=1.25e-18),
+        integration=IntegrationSpec(dt_ns=dt_ns, t_max_us=1600.0,
+                                    rec_every=max(1, int(round(1000 / dt_ns))),
+                                    record_channels=["speed", "ke_ev"]))
+    return dict(name="drift_winner_rz", model=model, spec=spec, source=src,
+                h=h)
+
+
 def funnel_electrodes(coarse=2):
     """The funnel's electrodes as the reference's per-electrode cache holds
     them (build_rz.py:377-386): each ring's potential solved at 1e4 V with
