@@ -392,6 +392,21 @@ int ig_fly_graph(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
  * recorded around the launches), in milliseconds. */
 int ig_last_fly_ms(ig_engine *h, double *out);
 
+/* ig_fly without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate while they launch (code under HIP
+ * graph capture, serving loops and harnesses that forbid hipMalloc).
+ * Results equal ig_fly's bit for bit.  The live-ion list, its cell order
+ * and the SDS jump table of the window live in one fixed region of device
+ * memory per GPU (a static device array of about 14 MB, held until the
+ * process ends), set up when an engine is created on that device; the
+ * call allocates nothing (the ensemble and the field pack are the
+ * caller's, created beforehand).  At most IG_NOALLOC_MAX_IONS ions per
+ * ensemble (IG_E_ARG beyond).  Every call on a device shares its region:
+ * calls that overlap in time on one device race. */
+#define IG_NOALLOC_MAX_IONS (1 << 20)
+int ig_fly_noalloc(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
+                   ig_ensemble_id e);
+
 /* ---- births on the device ------------------------------------------- */
 
 #define IG_BIRTH_POINT 0

@@ -22,6 +22,7 @@ FACE_BIRTH = -2
 FATE_EMPTY = -4
 MAX_CHANNELS = 16
 MAX_PLANES = 16
+NOALLOC_MAX_IONS = 1 << 20       # IG_NOALLOC_MAX_IONS
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -100,6 +101,7 @@ class Results(C.Structure):
 
 BIRTH_POINT, BIRTH_DISC, BIRTH_LINE, BIRTH_GRID, BIRTH_BOX, \
     BIRTH_GAUSSIAN = range(6)
+E_ARG = -1                      # IG_E_ARG
 E_BIRTH_TRIES = -6
 _u64p = C.POINTER(C.c_uint64)
 _d3 = C.c_double * 3
@@ -192,6 +194,8 @@ SIGNATURES = {
                                C.POINTER(Ions), C.POINTER(Results)]),
     "ig_fly_graph": (C.c_int, [_VP, C.c_int64, C.POINTER(FlightParams),
                                C.c_int64, C.c_int32]),
+    "ig_fly_noalloc": (C.c_int, [_VP, C.c_int64, C.POINTER(FlightParams),
+                                 C.c_int64]),
     "ig_bases_upload": (C.c_int, [_VP, C.POINTER(BasesDesc),
                                   C.POINTER(C.c_int64)]),
     "ig_bases_release": (C.c_int, [_VP, C.c_int64]),

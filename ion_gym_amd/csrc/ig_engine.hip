@@ -168,6 +168,60 @@ struct Ensemble {
   }
 };
 
+// ---- scratch of the no-allocation flight (ig_fly_noalloc) -------------
+// One fixed region of device memory per GPU, part of the loaded code
+// object: the live-ion list, its cell order, the counting-sort histogram
+// and the SDS jump table of ig_fly_noalloc.  In host code `scratch` names a
+// host-side stand-in, never device memory: each device's address comes from
+// hipGetSymbolAddress, taken once per device (with that device current) when
+// an engine is created on it, and only that address is passed to kernels
+// and copies.  Calls of ig_fly_noalloc on one device share the region, so
+// calls that overlap in time on one device race; each call rebuilds the
+// list and histogram it reads (the counter and histogram are cleared by
+// kernels on the call's stream) and rewrites the jump table when it
+// differs from the last one copied.
+constexpr int64_t kNoAllocMaxIons = IG_NOALLOC_MAX_IONS;
+constexpr size_t kNaAlign = 256;
+constexpr size_t na_round(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaHistWords = (1u << 18) + 256; // cell buckets + scan tile sums
+constexpr size_t kNaSdsDoubles = 5 * 1002;
+constexpr size_t kNaIdx = kNaAlign;                // [0, 4): the live count
+constexpr size_t kNaKey = kNaIdx + na_round((size_t)kNoAllocMaxIons * 4);
+constexpr size_t kNaSorted = kNaKey + na_round((size_t)kNoAllocMaxIons * 4);
+constexpr size_t kNaHist = kNaSorted + na_round((size_t)kNoAllocMaxIons * 4);
+constexpr size_t kNaSds = kNaHist + na_round(kNaHistWords * 4);
+constexpr size_t kNaBytes = kNaSds + na_round(kNaSdsDoubles * 8);
+
+// what one device's region holds, as the host knows it
+struct NoAllocDevice {
+  uint8_t *ptr = nullptr;       // hipGetSymbolAddress of scratch
+  int status = IG_E_STATE;      // its return code, as an IG_E_* code
+  std::string err;
+  std::vector<double> sds_host; // the jump table last copied in
+};
+std::mutex g_na_mu;
+std::map<int, NoAllocDevice> g_na; // by device ordinal; nodes never move
+
+} // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+// the device's region, looked up once per device with the device current
+void noalloc_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  NoAllocDevice &d = g_na[device];
+  if (d.ptr) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && p) {
+    d.ptr = (uint8_t *)p;
+    d.status = 0;
+  } else {
+    d.status = IG_E_HIP;
+    d.err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+  }
+}
 } // namespace
 
 struct ig_engine {
@@ -246,6 +300,7 @@ int ig_engine_create(int device, ig_engine **out) {
   std::unique_ptr<ig_engine> h(new ig_engine());
   h->device = device;
   HIPX(hipSetDevice(device));
+  noalloc_setup(device); // an error is kept and reported by ig_fly_noalloc
   HIPX(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&h->ev0));
   HIPX(hipEventCreate(&h->ev1));
@@ -1071,36 +1126,54 @@ static int compact_mode(const ig_flight_params *p, const FieldsK &f) {
   return f.route == IG_ROUTE_XYZ ? 2 : 1;
 }
 
+// where a window's live list is built: the ensemble's own buffers
+// (ig_fly, ig_fly_graph) or a device's no-allocation scratch
+struct LiveBufs {
+  int32_t *idx = nullptr, *sorted = nullptr;
+  uint32_t *cnt = nullptr, *key = nullptr, *hist = nullptr;
+};
+
+static_assert((size_t)(kMaxBlocks + kScanTiles) <= kNaHistWords,
+              "the no-allocation scratch holds the cell histogram");
+
 static int window_ens(ig_engine *h, Ensemble *E, const FieldsK &f, const ig_flight_params *p,
-                      EnsK &ek) {
+                      EnsK &ek, const LiveBufs *fixed = nullptr) {
   ek = E->k;
   const int mode = compact_mode(p, f);
   if (!mode || E->n == 0) return 0;
   const bool ordered = mode == 2;
-  if (int rc = live_bufs(E, ordered)) return rc;
+  LiveBufs b;
+  if (fixed) {
+    b = *fixed;
+  } else {
+    if (int rc = live_bufs(E, ordered)) return rc;
+    b.idx = (int32_t *)E->live_idx.p;
+    b.cnt = (uint32_t *)E->live_cnt.p;
+    b.key = (uint32_t *)E->live_key.p;
+    b.sorted = (int32_t *)E->live_sorted.p;
+    b.hist = (uint32_t *)E->live_hist.p;
+  }
   const unsigned grid = (unsigned)((E->n + 255) / 256);
-  uint32_t *cnt = (uint32_t *)E->live_cnt.p;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, cnt);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, b.cnt);
   hipLaunchKernelGGL(compact_live_kernel, dim3(grid), dim3(256), 0, h->stream, E->k.fate, E->n,
-                     (int32_t *)E->live_idx.p, cnt);
-  ek.idx = (const int32_t *)E->live_idx.p;
-  ek.n_live = (const uint32_t *)cnt;
+                     b.idx, b.cnt);
+  ek.idx = (const int32_t *)b.idx;
+  ek.n_live = (const uint32_t *)b.cnt;
   if (ordered) {
     const BlockGrid g = block_grid(f);
     const int64_t nb = kMaxBlocks;
-    uint32_t *hist = (uint32_t *)E->live_hist.p, *key = (uint32_t *)E->live_key.p;
     hipLaunchKernelGGL(zero_blocks_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
-                       h->stream, hist, nb);
+                       h->stream, b.hist, nb);
     hipLaunchKernelGGL(block_hist_kernel, dim3(grid), dim3(256), 0, h->stream, g, E->k,
-                       (const int32_t *)E->live_idx.p, (const uint32_t *)cnt, key, hist);
+                       (const int32_t *)b.idx, (const uint32_t *)b.cnt, b.key, b.hist);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(kScanTiles), dim3(256), 0, h->stream,
-                       (const uint32_t *)hist, hist + kMaxBlocks);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(kScanTiles), dim3(256), 0, h->stream, hist,
-                       (const uint32_t *)(hist + kMaxBlocks));
+                       (const uint32_t *)b.hist, b.hist + kMaxBlocks);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(kScanTiles), dim3(256), 0, h->stream, b.hist,
+                       (const uint32_t *)(b.hist + kMaxBlocks));
     hipLaunchKernelGGL(block_scatter_kernel, dim3(grid), dim3(256), 0, h->stream,
-                       (const int32_t *)E->live_idx.p, (const uint32_t *)cnt,
-                       (const uint32_t *)key, hist, (int32_t *)E->live_sorted.p);
-    ek.idx = (const int32_t *)E->live_sorted.p;
+                       (const int32_t *)b.idx, (const uint32_t *)b.cnt,
+                       (const uint32_t *)b.key, b.hist, b.sorted);
+    ek.idx = (const int32_t *)b.sorted;
   }
   return 0;
 }
@@ -1109,7 +1182,7 @@ static int window_ens(ig_engine *h, Ensemble *E, const FieldsK &f, const ig_flig
 // the kernel arguments for one window of (fid, p, eid).
 static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params *p,
                           ig_ensemble_id eid, const char *who, const FieldsK *&fk,
-                          FlightK &k, Ensemble *&E) {
+                          FlightK &k, Ensemble *&E, NoAllocDevice *na = nullptr) {
   const std::string w(who);
   if (!h || !p) return fail(IG_E_ARG, w + ": null argument");
   auto fit = h->fields.find(fid);
@@ -1122,20 +1195,25 @@ static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params
   k.rec_cols = E->rec_cols;
   if (int rc = set_dev(h)) return rc;
   if (k.gas == IG_GAS_SDS) {
-    // the (5, 1002) jump table stays on the device until it changes
-    const size_t nt = 5 * 1002;
-    if (h->sds_host.size() != nt ||
-        std::memcmp(h->sds_host.data(), p->sds_stats, nt * sizeof(double)) != 0) {
+    // the (5, 1002) jump table stays on the device until it changes: in
+    // the engine's own buffer, or without allocation in the device's
+    // scratch
+    const size_t nt = kNaSdsDoubles;
+    std::vector<double> &host = na ? na->sds_host : h->sds_host;
+    if (host.size() != nt ||
+        std::memcmp(host.data(), p->sds_stats, nt * sizeof(double)) != 0) {
       HIPX(hipStreamSynchronize(h->stream));
-      h->sds_host.assign(p->sds_stats, p->sds_stats + nt);
-      if (!h->sds.p) {
+      host.assign(p->sds_stats, p->sds_stats + nt);
+      void *dst = na ? (void *)(na->ptr + kNaSds) : h->sds.p;
+      if (!dst) {
         if (int rc = dev_alloc(h->sds, nt * sizeof(double))) return rc;
+        dst = h->sds.p;
       }
-      HIPX(hipMemcpyAsync(h->sds.p, h->sds_host.data(), nt * sizeof(double),
-                          hipMemcpyHostToDevice, h->stream));
+      HIPX(hipMemcpyAsync(dst, host.data(), nt * sizeof(double), hipMemcpyHostToDevice,
+                          h->stream));
       HIPX(hipStreamSynchronize(h->stream));
     }
-    k.sds_stats = (const double *)h->sds.p;
+    k.sds_stats = na ? (const double *)(na->ptr + kNaSds) : (const double *)h->sds.p;
   }
   fk = &fit->second->k;
   const int32_t clk = (fk->route == IG_ROUTE_XYZ && k.gas != IG_GAS_SDS) ? 1 : 0;
@@ -1174,6 +1252,43 @@ int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensembl
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
   if (rc) return launch_error(rc, "ig_fly");
+  return 0;
+}
+
+int ig_fly_noalloc(ig_engine *h, ig_fields_id fid, const ig_flight_params *p,
+                   ig_ensemble_id eid) {
+  if (!h || !p) return fail(IG_E_ARG, "ig_fly_noalloc: null argument");
+  NoAllocDevice *na = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    auto it = g_na.find(h->device);
+    if (it == g_na.end()) return fail(IG_E_STATE, "ig_fly_noalloc: no scratch set up for device");
+    if (it->second.status) return fail(it->second.status, "ig_fly_noalloc: " + it->second.err);
+    na = &it->second;
+  }
+  Ensemble *E0 = find_ens(h, eid);
+  if (E0 && E0->n > kNoAllocMaxIons)
+    return fail(IG_E_ARG, "ig_fly_noalloc: " + std::to_string(E0->n) +
+                              " ions exceed the no-allocation limit of " +
+                              std::to_string(kNoAllocMaxIons) +
+                              " (IG_NOALLOC_MAX_IONS); fly them with ig_fly");
+  const FieldsK *f = nullptr;
+  FlightK k;
+  Ensemble *E = nullptr;
+  if (int rc = prepare_flight(h, fid, p, eid, "ig_fly_noalloc", f, k, E, na)) return rc;
+  LiveBufs b;
+  b.cnt = (uint32_t *)na->ptr;
+  b.idx = (int32_t *)(na->ptr + kNaIdx);
+  b.key = (uint32_t *)(na->ptr + kNaKey);
+  b.sorted = (int32_t *)(na->ptr + kNaSorted);
+  b.hist = (uint32_t *)(na->ptr + kNaHist);
+  HIPX(hipEventRecord(h->ev0, h->stream));
+  EnsK ek;
+  if (int rc = window_ens(h, E, *f, p, ek, &b)) return rc;
+  int rc = launch_route(*f, k, ek, E->n, p->block > 0 ? p->block : h->block, h->stream);
+  HIPX(hipEventRecord(h->ev1, h->stream));
+  h->timed = true;
+  if (rc) return launch_error(rc, "ig_fly_noalloc");
   return 0;
 }
 
@@ -1418,7 +1533,7 @@ int ig_fields_read_set(ig_engine *h, ig_fields_id fid, int32_t set, int32_t comp
 }
 
 // a scratch buffer of at least `bytes` (grown, never shrunk)
-static int scratch(DevBuf &b, size_t bytes) {
+static int grow_buf(DevBuf &b, size_t bytes) {
   if (b.bytes >= bytes && b.p) return 0;
   b.reset();
   return dev_alloc(b, bytes);
@@ -1487,7 +1602,7 @@ static int births_run(ig_engine *h, const ig_births_desc *d, const char *who,
   b.tob_span = d->tob_span_us;
   // tables: 256 thresholds, 256 widths, 256 densities, the masses
   const size_t tab_bytes = 256 * 8 * 3 + (size_t)d->n_mz * 8;
-  if (int rc = scratch(h->btab, tab_bytes)) return rc;
+  if (int rc = grow_buf(h->btab, tab_bytes)) return rc;
   uint8_t *tb = (uint8_t *)h->btab.p;
   HIPX(hipMemcpyAsync(tb, d->zig_k, 256 * 8, hipMemcpyHostToDevice, h->stream));
   HIPX(hipMemcpyAsync(tb + 2048, d->zig_w, 256 * 8, hipMemcpyHostToDevice, h->stream));
@@ -1500,7 +1615,7 @@ static int births_run(ig_engine *h, const ig_births_desc *d, const char *who,
   b.mz = (const double *)(tb + 6144);
   int32_t levels = 1;
   while (levels < 62 && ((n - 1) >> levels) > 0) ++levels;
-  if (int rc = scratch(h->bflag, 16)) return rc;
+  if (int rc = grow_buf(h->bflag, 16)) return rc;
   const BirthsOut out = dest(n);
   for (int c = 0; c < 7; ++c)
     if (!out.col[c]) return fail(IG_E_NOMEM, w + ": no device memory for the births");
@@ -1511,8 +1626,8 @@ static int births_run(ig_engine *h, const ig_births_desc *d, const char *who,
     if (P + 2 > (int64_t)INT32_MAX / 2)
       return fail(IG_E_ARG, w + ": " + std::to_string(n) + " births need more stream positions "
                                 "than one call indexes; split the source");
-    if (int rc = scratch(h->bw, (size_t)nW * 8)) return rc;
-    if (int rc = scratch(h->bj, (size_t)levels * (size_t)(P + 2) * 4)) return rc;
+    if (int rc = grow_buf(h->bw, (size_t)nW * 8)) return rc;
+    if (int rc = grow_buf(h->bj, (size_t)levels * (size_t)(P + 2) * 4)) return rc;
     if (births_launch(b, d->pcg_state[0], d->pcg_state[1], d->pcg_inc[0], d->pcg_inc[1],
                       (uint64_t *)h->bw.p, nW, (int32_t *)h->bj.p, P, levels, out,
                       (int64_t *)h->bflag.p, h->stream))
@@ -1537,7 +1652,7 @@ int ig_births(ig_engine *h, const ig_births_desc *d, double *out) {
   int rc = births_run(h, d, "ig_births",
                       [&](int64_t nn) {
                         BirthsOut o{};
-                        if (scratch(h->bout, (size_t)std::max<int64_t>(nn, 1) * 7 * 8) == 0)
+                        if (grow_buf(h->bout, (size_t)std::max<int64_t>(nn, 1) * 7 * 8) == 0)
                           for (int c = 0; c < 7; ++c) o.col[c] = (double *)h->bout.p + c;
                         o.stride = 7;
                         return o;
@@ -1628,7 +1743,7 @@ int ig_ensemble_refuse_births(ig_engine *h, ig_ensemble_id id, ig_fields_id fid,
     return fail(IG_E_ARG, "ig_ensemble_refuse_births: slots [" + std::to_string(offset) + ", " +
                               std::to_string(offset + n) + ") are not all loaded");
   if (int rc = set_dev(h)) return rc;
-  if (int rc = scratch(h->bflag, 16)) return rc;
+  if (int rc = grow_buf(h->bflag, 16)) return rc;
   unsigned long long *cnt = (unsigned long long *)h->bflag.p;
   HIPX(hipMemsetAsync(cnt, 0, 8, h->stream));
   if (n > 0)

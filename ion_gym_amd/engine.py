@@ -474,6 +474,15 @@ class Engine:
         L.check(L.lib().ig_fly(self.h, fields.id, C.byref(p), ens.id),
                 "ig_fly")
 
+    def fly_noalloc(self, fields, flight, ens):
+        """fly() without device allocation (ig_fly_noalloc): the window's
+        scratch is the device's fixed region; at most NOALLOC_MAX_IONS ions
+        per ensemble, and calls overlapping on one device race."""
+        keep = _Keep()
+        p = flight.struct(keep)
+        L.check(L.lib().ig_fly_noalloc(self.h, fields.id, C.byref(p),
+                                       ens.id), "ig_fly_noalloc")
+
     # ----------------------------------------------- field composition
     def upload_bases_rz(self, phi, ele, Z_mm, U_mm, base_volts=1e4):
         """Per-electrode r-z potentials for composition on the device:
