@@ -9,33 +9,61 @@ benchmark is one window of S integrator steps for every live ion (one
 ig_fly launch), with the ion state and the field resident in HBM.
 
 Prints ONE JSON line.  ``value`` is ion-steps/s (live ion-steps, whole
-job).  Beside it: the plain FP64 kernel's ion-steps/s on the same
-workload, the roofline of the flight kernel (algorithmic gather bytes per
-launch / HIP-event launch time, against 8 TB/s HBM), the CPU baseline
-(the oracle's C FP64 port on the host cores, timed on a bounded sample)
-and the serving leg: flight requests on the 3-D SLIM traveling-wave board
-(201 x 81 x 801 @ 0.05 mm, 11 FP32 field sets, N2 4 Torr hard sphere,
-1e3-5e5 ions per request with their own TW amplitude, speed and m/z)
-arriving as a Poisson stream at a set fraction of the engine's solo
-throughput, served by continuous batching; p50/p99 request latency and
-the sustained ion-steps/s.  The precision object states the FP32-basis
-error against FP64 on the full-grid 3-D quadrupole (100,000 ions, 2,000
-RF cycles): transmission of both and per-ion position / TOF differences.
+job).  Beside it:
+
+* the plain FP64 kernel's ion-steps/s on the same workload;
+* the roofline of the flight kernel.  Its 16 MB field pack is served from
+  L1/L2/MALL, not HBM, so the gather is measured against the gather rate
+  the memory hierarchy reaches for the same access pattern: a probe kernel
+  (csrc/probe_gather.hip) walks every ion's own sample-point sequence
+  through an identical FP32 record array and does nothing else.  HBM
+  traffic (FETCH_SIZE, WRITE_SIZE) and vector-issue counters
+  (SQ_ACTIVE_INST_VALU, SQ_INSTS_VALU, SQ_WAVES) are read by two
+  rocprofv3 counter runs of this script's own probe mode, each calibrated
+  against a plain copy of known size in the same run; ``binding`` names
+  the resource the kernel is closest to saturating;
+* the CPU baseline (the oracle's C FP64 port of the reference kernels on
+  the host cores, timed on a bounded sample, with the CPU model and the
+  cost per ion-step per core beside the reference's own 87-101 ns);
+* ``precision_funnel``: the FP32 basis against FP64 on this workload (the
+  same births, one 4000-step window): distributions and means of the
+  observables;
+* the serving leg: flight requests on the 3-D SLIM traveling-wave board
+  (201 x 81 x 801 @ 0.05 mm, 11 FP32 field sets, N2 4 Torr hard sphere,
+  1e3-5e5 ions per request with their own TW amplitude, speed and m/z)
+  arriving as Poisson streams at three offered loads (fractions of the
+  engine's solo throughput), served by continuous batching; p50/p99
+  request latency and the sustained ion-steps/s per load;
+* ``precision``: the FP32-basis error against FP64 on the full-grid 3-D
+  quadrupole (100,000 ions, 2,000 RF cycles).
 """
 import argparse
+import csv
+import ctypes
+import glob
 import json
 import math
 import os
-import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+# the SDS legs fly ion_gym's own jump table (kept with the tests' fixtures)
+os.environ.setdefault("ION_GYM_SDS_TABLE", os.path.join(
+    HERE, "tests", "golden", "sds_jump_icdf.dat"))
 
 HBM_PEAK_GBS = 8000.0
+SIMDS = 1024                # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9            # MI355X peak engine clock
+# the reference's own cost of an r-z ion-step with gas, one core
+# (ion_gym/io/sim_spec.py:112-116)
+REF_NS_PER_ION_STEP = (87.0, 101.0)
 
 
 def gather_bytes_per_ion_step(n_sets, value_bytes):
@@ -78,7 +106,8 @@ def run_gpu(cfg, precision, warmup, steps, window, engine, compact=3):
     live = ens.live()
     n_sets = 1 + len(model.ch_kind)
     out = dict(ion_steps=s1 - s0, wall_s=wall, kernel_ms=kernel_ms,
-               live_end=live, n=n, n_sets=n_sets, field_bytes=fp.nbytes)
+               live_end=live, n=n, n_sets=n_sets, field_bytes=fp.nbytes,
+               state=ens.read(records=False))
     ens.release()
     fp.release()
     return out
@@ -98,15 +127,222 @@ def cpu_baseline(cfg, window, n_sample, threads):
     return int(r["steps"].sum()), el
 
 
-def run_serving(eng, n_req, load, flight_us, window, capacity, seed,
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _probe_lib():
+    path = os.path.join(HERE, "ion_gym_amd", "libig_probe.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    L.igp_gather_rz.argtypes = [ctypes.c_void_p, ctypes.c_int32,
+                                ctypes.c_int32, ctypes.c_int32,
+                                ctypes.c_void_p, ctypes.c_int64,
+                                ctypes.c_int32, ctypes.c_int32,
+                                ctypes.POINTER(ctypes.c_double)]
+    L.igp_copy.argtypes = [ctypes.c_int64, ctypes.c_int32,
+                           ctypes.POINTER(ctypes.c_double)]
+    return L
+
+
+def fp32_records(pack):
+    """The engine's FP32 r-z record array of a pack: node-major records
+    of the sets' (Ez, Eu) pairs, padded to whole 16-byte units (the layout
+    ig_fields_upload builds, unit set scales)."""
+    sets = [(pack["EzA"], pack["EuA"])]
+    sets += [(pack["EzK"][k], pack["EuK"][k])
+             for k in range(len(pack["ch_kind"]))]
+    nx, nu = np.asarray(pack["EzA"]).shape
+    nv = 2 * len(sets)
+    stride = (nv + 3) // 4 * 4
+    rec = np.zeros((nx, nu, stride), np.float32)
+    for s, (a, b) in enumerate(sets):
+        rec[:, :, 2 * s] = a
+        rec[:, :, 2 * s + 1] = b
+    return rec, stride
+
+
+def probe_gather(pack, state, dt, n_iter, reps=3):
+    """The gather rate of the flight's own access pattern: each ion of
+    ``state`` (slot order, as the r-z flight lays out its lanes) walks
+    n_iter sample points from its position with its velocity times dt,
+    reading its bilinear cell's four records.  Returns (gathers per
+    second, bytes loaded per gather, ms per launch)."""
+    rec, stride = fp32_records(pack)
+    nx, nu, _ = rec.shape
+    h = float(pack["mm"])
+    u0 = float(pack["u0"])
+    x, y, z = state["x"], state["y"], state["z"]
+    r = np.hypot(y, z)
+    vr = np.where(r > 0, (y * state["vy"] + z * state["vz"])
+                  / np.maximum(r, 1e-300), 0.0)
+    pts = np.stack([x / h, (r - u0) / h, state["vx"] * dt / h,
+                    vr * dt / h]).astype(np.float32)
+    pts = np.ascontiguousarray(pts)
+    n = pts.shape[1]
+    ms = ctypes.c_double()
+    rc = _probe_lib().igp_gather_rz(rec.ctypes.data, nx, nu, stride,
+                                    pts.ctypes.data, n, n_iter, reps,
+                                    ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"igp_gather_rz failed ({rc})")
+    return n * n_iter / (ms.value * 1e-3), 4 * stride * 4, ms.value
+
+
+COPY_FLOATS = 1 << 28          # 1 GiB of floats: the counters' calibration
+
+
+def counter_probe_main(args):
+    """This script's probe mode, the program a rocprofv3 counter run
+    profiles: the throughput workload's first two FP32 windows (the second
+    is the measured dispatch), then two plain copies of 1 GiB."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Engine, Flight
+    cfg = configs.funnel_rz(n_ions=args.ions, seed=3)
+    eng = Engine(0)
+    fp = eng.upload_rz(cfg["model"].pack(), "f32")
+    ens = eng.ensemble(len(cfg["ions"]["x"]))
+    ens.load(cfg["ions"])
+    fl = Flight(cfg["dt"], cfg["t_max"], gas=cfg["gas"],
+                max_steps=args.window, compact=int(args.compact))
+    for _ in range(2):
+        eng.fly(fp, fl, ens)
+    eng.sync()
+    ens.release()
+    fp.release()
+    eng.close()
+    ms = ctypes.c_double()
+    rc = _probe_lib().igp_copy(COPY_FLOATS, 2, ctypes.byref(ms))
+    print(f"[probe] two windows and two 1 GiB copies done (copy rc {rc}, "
+          f"{ms.value:.3f} ms)", flush=True)
+
+
+def _counter_rows(outdir):
+    rows = []
+    for f in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"),
+                       recursive=True):
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _per_dispatch(rows, match):
+    """{dispatch id: {counter: value, '_ns': duration}} of the dispatches
+    whose kernel name contains ``match``."""
+    out = {}
+    for r in rows:
+        name = r.get("Kernel_Name", "")
+        if match not in name:
+            continue
+        d = out.setdefault(int(r.get("Dispatch_Id", 0)), {})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + \
+            float(r["Counter_Value"])
+        try:
+            d["_ns"] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+        except (KeyError, TypeError, ValueError):
+            pass
+    return out
+
+
+def run_counters(args):
+    """Two rocprofv3 counter runs of the probe mode (FETCH_SIZE; then
+    WRITE_SIZE with the vector-issue counters), each its own process under
+    a time limit.  Returns the flight dispatch's counters per launch and
+    the copy calibration, or {'error': ...}."""
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return {"error": "rocprofv3 not found"}
+    passes = (["FETCH_SIZE"],
+              ["WRITE_SIZE", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU",
+               "SQ_WAVES", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE"])
+    env = dict(os.environ, TMPDIR="/tmp")
+    flight, copy = {}, {}
+    cmds = []
+    for k, counters in enumerate(passes):
+        out = tempfile.mkdtemp(prefix="ig_counters_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", *counters,
+               "--output-format", "csv", "-d", out, "-o", "run", "--",
+               sys.executable, os.path.join(HERE, "bench.py"),
+               "--counter-probe", "--ions", str(args.ions), "--window",
+               str(args.window), "--compact", str(args.compact)]
+        cmds.append(" ".join(cmd[4:]))
+        print(f"[bench] counters pass {k + 1}: {' '.join(counters)}",
+              file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True,
+                           text=True)
+        rows = _counter_rows(out)
+        shutil.rmtree(out, ignore_errors=True)
+        fl = _per_dispatch(rows, "rz_fly_kernel")
+        cp = _per_dispatch(rows, "copy_probe_kernel")
+        if not fl or not cp:
+            return {"error": f"pass {k + 1} ({' '.join(counters)}): rc "
+                             f"{r.returncode}, no counter rows for the "
+                             f"flight or the copy; "
+                             f"{(r.stderr or '')[-300:]}"}
+        flight.update(fl[max(fl)])           # the second (timed) window
+        for d in cp.values():
+            for c, v in d.items():
+                copy.setdefault(c, []).append(v)
+    copy = {c: float(np.mean(v)) for c, v in copy.items()}
+    return {"flight": flight, "copy": copy, "commands": cmds}
+
+
+def funnel_precision(eng, cfg, steps=4000):
+    """FP32 basis against FP64 on the throughput workload itself: the same
+    births, one window of ``steps`` steps in each precision; two-sample KS
+    p-values and mean ratios of the axial displacement, final radius,
+    kinetic energy and collision count, and the fate fractions."""
+    from scipy import stats
+    from ion_gym_amd.engine import Flight
+    res = {}
+    for prec in ("f32", "f64"):
+        fp = eng.upload_rz(cfg["model"].pack(), prec)
+        n = len(cfg["ions"]["x"])
+        ens = eng.ensemble(n)
+        ens.load(cfg["ions"])
+        eng.fly(fp, Flight(cfg["dt"], cfg["t_max"], gas=cfg["gas"],
+                           max_steps=steps), ens)
+        g = ens.read(records=False)
+        ens.release()
+        fp.release()
+        fly = g["fate"] == -2
+        m = cfg["ions"]["m_amu"] * 1.66053906660e-27
+        ke = 0.5 * m * (g["vx"] ** 2 + g["vy"] ** 2 + g["vz"] ** 2) * 1e6 \
+            / 1.602176634e-19
+        res[prec] = dict(dx=(g["x"] - cfg["ions"]["x"])[fly],
+                         rad=np.hypot(g["y"], g["z"])[fly], ke=ke[fly],
+                         ncol=g["ncol"][fly].astype(float),
+                         flying=float(fly.mean()),
+                         splat=float((g["fate"] == 0).mean()))
+    a, b = res["f32"], res["f64"]
+    out = {"workload": "funnel_rz, the throughput workload's births",
+           "ions": int(len(cfg["ions"]["x"])), "window_steps": steps,
+           "flying_f32": a["flying"], "flying_f64": b["flying"],
+           "splat_f32": a["splat"], "splat_f64": b["splat"]}
+    for k in ("dx", "rad", "ke", "ncol"):
+        out[f"ks_p_{k}"] = float(stats.ks_2samp(a[k], b[k]).pvalue)
+        out[f"mean_ratio_{k}"] = float(a[k].mean() / b[k].mean())
+    return out
+
+def run_serving(eng, n_req, loads, flight_us, window, capacity, seed,
                 compact=3):
     """The serving leg: requests on the SLIM traveling-wave board (11 FP32
     field sets, N2 4 Torr hard sphere) arrive as a Poisson stream and are
     served by one FlightServer (continuous batching over one resident
-    slab).  The arrival rate is set to ``load`` times the engine's measured
-    solo throughput divided by the mean request work.  Returns p50/p99
-    request latency (submit -> result on the host) and the sustained
-    ion-steps/s over the run."""
+    slab).  For each offered load the arrival rate is that fraction of the
+    engine's measured solo throughput divided by the mean request work;
+    the same n_req requests are replayed at every load.  Returns, per
+    load, p50/p99 request latency (submit -> result on the host) and the
+    sustained ion-steps/s over the run."""
     from ion_gym_amd import configs
     from ion_gym_amd.engine import Flight
     from ion_gym_amd.serve import (FlightRequest, FlightServer, latency_stats,
@@ -118,12 +354,12 @@ def run_serving(eng, n_req, load, flight_us, window, capacity, seed,
     fp = eng.upload_xyz(c["pack"], "f32")
     build_s = time.perf_counter() - t0
     rng = np.random.default_rng(seed)
-    reqs, work = [], []
+    specs, work = [], []
     for k in range(n_req):
         ions, tune = configs.slim_request(c, rng, t_flight_us=flight_us,
                                           rng_key=k + 1)
         t = make_tune(c["pack"], **tune)
-        reqs.append(FlightRequest(ions=ions, tune=t))
+        specs.append((ions, t))
         work.append(len(ions["x"]) * math.ceil(t["t_max"] / t["dt"]))
     # solo throughput: one 200,000-ion request, two windows
     cal_ions, cal_tune = configs.slim_request(c, np.random.default_rng(seed + 1),
@@ -144,27 +380,39 @@ def run_serving(eng, n_req, load, flight_us, window, capacity, seed,
     solo = (ens.steps() - s0) / (time.perf_counter() - t1)
     ens.release()
     mean_work = float(np.mean(work))
-    lam = load * solo / mean_work
-    gaps = np.random.default_rng(seed + 2).exponential(1.0 / lam, n_req)
-    arrivals = list(zip(np.cumsum(gaps) - gaps[0], reqs))
-    srv = FlightServer(eng, fp, Flight(cal_tune["dt"], cal_tune["t_max"],
-                                       gas=c["gas"], compact=compact),
-                       capacity=capacity, window=window)
-    t2 = time.perf_counter()
-    done = srv.run_arrivals(arrivals)
-    wall = time.perf_counter() - t2
-    st = latency_stats(done)
+    per_load = []
+    for li, load in enumerate(loads):
+        lam = load * solo / mean_work
+        gaps = np.random.default_rng(seed + 2 + li).exponential(1.0 / lam,
+                                                                n_req)
+        arrivals = [(float(t), FlightRequest(ions=ions, tune=tune))
+                    for t, (ions, tune) in zip(np.cumsum(gaps) - gaps[0],
+                                               specs)]
+        srv = FlightServer(eng, fp, Flight(cal_tune["dt"], cal_tune["t_max"],
+                                           gas=c["gas"], compact=compact),
+                           capacity=capacity, window=window)
+
+        def progress(n_done, load=load):
+            print(f"[bench] serving at load {load}: {n_done}/{n_req} "
+                  f"requests done", file=sys.stderr, flush=True)
+
+        t2 = time.perf_counter()
+        done = srv.run_arrivals(arrivals, progress=progress)
+        wall = time.perf_counter() - t2
+        st = latency_stats(done)
+        per_load.append({"offered_load": load, "arrival_rate_per_s": lam,
+                         "requests": len(done), "p50_s": st["p50_s"],
+                         "p99_s": st["p99_s"], "mean_s": st["mean_s"],
+                         "sustained_ion_steps_per_s": srv.ion_steps / wall,
+                         "wall_s": wall})
+        srv.release()
     out = {"workload": "slim_3d (SLIM TW board 201x81x801 @ 0.05 mm, 11 "
                        "FP32 sets, N2 4 Torr HS)",
-           "requests": len(done), "ions_per_request": "log-U(1e3, 5e5)",
-           "flight_us": flight_us, "offered_load": load,
-           "arrival_rate_per_s": lam, "window_steps": window,
+           "requests_per_load": n_req, "ions_per_request": "log-U(1e3, 5e5)",
+           "flight_us": flight_us, "window_steps": window,
            "slab_capacity": capacity, "compact": compact,
-           "p50_s": st["p50_s"], "p99_s": st["p99_s"], "mean_s": st["mean_s"],
-           "sustained_ion_steps_per_s": srv.ion_steps / wall,
-           "solo_ion_steps_per_s": solo, "wall_s": wall,
+           "solo_ion_steps_per_s": solo, "loads": per_load,
            "field_bytes": fp.nbytes, "build_s": build_s}
-    srv.release()
     fp.release()
     return out
 
@@ -438,8 +686,10 @@ def main():
     ap.add_argument("--precision-ions", type=int, default=100_000)
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving leg (SLIM board, Poisson arrivals)")
-    ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-load", type=float, default=0.6)
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests per offered load")
+    ap.add_argument("--serve-loads", default="0.3,0.6,0.9",
+                    help="offered loads (fractions of the solo throughput)")
     ap.add_argument("--serve-flight-us", type=float, default=2.0)
     ap.add_argument("--serve-window", type=int, default=500)
     ap.add_argument("--serve-capacity", type=int, default=2_000_000)
@@ -448,7 +698,17 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the einzel, batched drift-tube, planar and "
                          "3-D SDS legs")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter runs (HBM traffic, "
+                         "vector issue)")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the gather-rate probe")
+    ap.add_argument("--counter-probe", action="store_true",
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.counter_probe:
+        counter_probe_main(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
               f"is not supported (ions shard across GPUs with a replicated "
@@ -474,8 +734,7 @@ def main():
         r64 = run_gpu(cfg, "f64", 1, max(2, args.steps // 4), args.window,
                       eng, int(args.compact))
         fp64 = r64["ion_steps"] / r64["wall_s"]
-    # roofline of the flight kernel: algorithmic bytes per launch / mean
-    # HIP-event launch time on the engine's stream
+    # roofline of the flight kernel
     vb = 8 if args.precision == "f64" else 4
     per_step = gather_bytes_per_ion_step(main_run["n_sets"], vb)
     launches = args.steps
@@ -483,17 +742,56 @@ def main():
     alg_bytes = ion_steps_per_launch * per_step + main_run["n"] * \
         state_bytes_per_ion()
     mean_ms = float(np.mean(main_run["kernel_ms"]))
-    achieved = alg_bytes / (mean_ms * 1e-3) / 1e9
+    kernel_rate = ion_steps_per_launch / (mean_ms * 1e-3)  # gathers/s
+    roof = {"kernel": "rz_fly_kernel", "kernel_ms": mean_ms,
+            "alg_bytes_per_launch": alg_bytes,
+            "alg_bytes_per_ion_step": per_step,
+            "alg_gbs": alg_bytes / (mean_ms * 1e-3) / 1e9}
+    if not args.no_probe and args.precision == "f32":
+        print("[bench] gather-rate probe: the ions' own sample points "
+              "through the FP32 records", file=sys.stderr, flush=True)
+        p_rate, p_bytes, p_ms = probe_gather(cfg["model"].pack(),
+                                             main_run["state"], cfg["dt"],
+                                             args.window)
+        roof.update(bound="l2_mall", unit="GB/s",
+                    achieved=kernel_rate * p_bytes / 1e9,
+                    peak=p_rate * p_bytes / 1e9,
+                    frac=kernel_rate / p_rate,
+                    bytes_per_gather=p_bytes, probe_ms=p_ms,
+                    peak_source="probe_gather.hip: every ion's own "
+                                "sample-point walk, one gather per step, "
+                                "nothing else")
+    counters = None
+    if not args.no_counters:
+        counters = run_counters(args)
     traffic = None
-    # HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE counter runs
-    # (scaled by a plain copy's readings) kept beside the profiles
-    tpath = os.path.join(HERE, "profiles",
-                         f"traffic_funnel_rz_{args.precision}.md")
-    if os.path.exists(tpath) and args.window == 2000 and \
-            args.ions == 1_000_000:
-        with open(tpath) as fh:
-            m = re.search(r"^bytes_per_launch = (\d+)", fh.read(), re.M)
-        traffic = int(m.group(1)) if m else None
+    if counters and "flight" in counters:
+        fl, cp = counters["flight"], counters["copy"]
+        kib = COPY_FLOATS * 4 / 1024.0
+        f_fac = kib / cp["FETCH_SIZE"]
+        w_fac = kib / cp["WRITE_SIZE"]
+        traffic = int(round((fl["FETCH_SIZE"] * f_fac
+                             + fl["WRITE_SIZE"] * w_fac) * 1024))
+        busy_cycles = fl["SQ_ACTIVE_INST_VALU"] * 4.0 / SIMDS
+        kern_s = fl.get("_ns", mean_ms * 1e6) * 1e-9
+        waves = fl["SQ_WAVES"]
+        roof.update(
+            traffic=traffic,
+            hbm={"achieved": traffic / (mean_ms * 1e-3) / 1e9,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": traffic / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            valu_busy=busy_cycles / (CLOCK_HZ * kern_s),
+            valu_insts_per_wave_step=fl["SQ_INSTS_VALU"] / waves
+            / args.window,
+            counter_calibration={"fetch_factor": f_fac,
+                                 "write_factor": w_fac,
+                                 "copy_kib": kib},
+            counter_kernel_ms=kern_s * 1e3)
+        roof["binding"] = ("valu_issue" if roof["valu_busy"] >
+                           roof.get("frac", 0.0) else "l2_mall")
+    elif counters:
+        roof["traffic"] = None
+        roof["counters_error"] = counters["error"]
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
@@ -501,11 +799,25 @@ def main():
               f"{args.cpu_window} steps on {threads} threads",
               file=sys.stderr, flush=True)
         st, el = cpu_baseline(cfg, args.cpu_window, args.cpu_sample, threads)
+        ns = threads / (st / el) * 1e9
         cpu = {"value": st / el, "unit": "ion-steps/s", "cores": threads,
                "kind": "port",
                "sample": f"{args.cpu_sample} ions of the same workload x "
                          f"{args.cpu_window} steps (FP64, HS, Philox), "
-                         f"{el:.1f} s"}
+                         f"{el:.1f} s",
+               "cpu_model": cpu_model(),
+               "ns_per_ion_step_per_core": ns,
+               "reference_ns_per_ion_step_per_core": list(REF_NS_PER_ION_STEP),
+               "ratio_to_reference": [ns / r for r in REF_NS_PER_ION_STEP],
+               "gpu_over_cpu": value / (st / el),
+               "gpu_over_reference_cores": [
+                   value / (threads / (r * 1e-9))
+                   for r in REF_NS_PER_ION_STEP]}
+    precision_funnel = None
+    if not args.no_precision:
+        print("[bench] precision on the funnel: FP32 basis vs FP64, "
+              "4000-step window", file=sys.stderr, flush=True)
+        precision_funnel = funnel_precision(eng, cfg)
     precision = None
     if not args.no_precision:
         print(f"[bench] precision: quad_3d full grid, "
@@ -514,9 +826,10 @@ def main():
         precision = run_precision(eng, args.precision_ions, 2000)
     serving = None
     if not args.no_serve:
+        loads = [float(v) for v in args.serve_loads.split(",") if v]
         print(f"[bench] serving: {args.serve_requests} SLIM requests at "
-              f"offered load {args.serve_load}", file=sys.stderr, flush=True)
-        serving = run_serving(eng, args.serve_requests, args.serve_load,
+              f"each offered load {loads}", file=sys.stderr, flush=True)
+        serving = run_serving(eng, args.serve_requests, loads,
                               args.serve_flight_us, args.serve_window,
                               args.serve_capacity, seed=11,
                               compact=int(args.compact))
@@ -556,13 +869,9 @@ def main():
                    "field_bytes": main_run["field_bytes"]},
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,
-        "roofline": {"bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rz_fly_kernel", "kernel_ms": mean_ms,
-                     "alg_bytes_per_launch": alg_bytes,
-                     "bytes_per_ion_step": per_step},
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "precision_funnel": precision_funnel,
         "serving": serving,
         "precision": precision,
         "solver": solver,

@@ -2,7 +2,9 @@
 
 Each HIP source is compiled by hipcc into an object in ``_build/`` and the
 objects are linked in-tree into ``ion_gym_amd/libion_gym_amd.so``, which the
-ctypes binding (``ion_gym_amd._lib``) loads.
+ctypes binding (``ion_gym_amd._lib``) loads.  The benchmark's gather-rate
+probe (csrc/probe_gather.hip, measurement only) is linked on its own into
+``ion_gym_amd/libig_probe.so``.
 """
 import concurrent.futures as cf
 import os
@@ -17,6 +19,8 @@ SOURCES = ("ig_engine.hip", "ig_rz_f64.hip", "ig_rz_f32s.hip", "ig_rz_f32.hip",
            "ig_xyz_f64.hip", "ig_xyz_f32s.hip", "ig_xyz_f32.hip",
            "ig_planar_f64.hip", "ig_planar_f32s.hip", "ig_planar_f32.hip",
            "ig_births.hip", "ig_solve.hip")
+PROBE_SOURCES = ("probe_gather.hip",)
+PROBE_OUT = os.path.join(HERE, "libig_probe.so")
 ARCH = os.environ.get("ION_GYM_AMD_ARCH", "gfx950")
 
 
@@ -60,14 +64,17 @@ def build(verbose=False, jobs=None, incremental=False):
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with cf.ThreadPoolExecutor(max_workers=jobs or min(8, len(srcs))) as ex:
-        objs = list(ex.map(one, srcs))
-    tmp = OUT + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
-    os.replace(tmp, OUT)
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(8, len(srcs) + 1)) as ex:
+        objs = list(ex.map(one, srcs + list(PROBE_SOURCES)))
+    n_probe = len(PROBE_SOURCES)
+    for out, group in ((OUT, objs[:-n_probe]), (PROBE_OUT, objs[-n_probe:])):
+        tmp = out + ".tmp"
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+               *group]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
+        os.replace(tmp, out)
     return OUT
 
 

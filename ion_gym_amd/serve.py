@@ -276,14 +276,21 @@ class FlightServer:
             self.step()
         raise RuntimeError("FlightServer.drain: window budget exhausted")
 
-    def run_arrivals(self, arrivals, max_windows=1_000_000):
+    def run_arrivals(self, arrivals, max_windows=1_000_000, progress=None,
+                     every=50):
         """Replay (arrival_time_s, request) pairs against the wall clock:
         each request is submitted once its arrival time has passed, while
-        the server keeps flying.  Returns the finished requests."""
+        the server keeps flying.  ``progress(n_done)`` is called each time
+        another ``every`` requests have finished.  Returns the finished
+        requests."""
         arrivals = sorted(arrivals, key=lambda a: a[0])
         t0 = time.perf_counter()
         k = 0
+        reported = 0
         for _ in range(max_windows):
+            if progress is not None and len(self.done) >= reported + every:
+                reported = len(self.done) - len(self.done) % every
+                progress(len(self.done))
             now = time.perf_counter() - t0
             while k < len(arrivals) and arrivals[k][0] <= now:
                 req = arrivals[k][1]
