@@ -446,7 +446,8 @@ __device__ __forceinline__ void azimuth(float u, float &sn, float &cs) {
 // * Scattering: the reference's impact angle asin(sqrt(U)), uniform
 //   azimuth and line-of-centres attenuation, in FP32 unit vectors.
 // Logarithms, reciprocals and square roots use the hardware's FP32
-// instructions; the angles use sin_pi_range.  A lane performs one partner
+// instructions (v_log_f32, v_rcp_f32, v_rsq_f32, v_sqrt_f32, each within
+// about one ulp); the angles use sin_pi_range.  A lane performs one partner
 // try per pass of its kernel loop while the other lanes keep stepping.
 struct HsTry {
   float wx, wy, wz; // ion velocity in the gas frame
@@ -454,10 +455,11 @@ struct HsTry {
   int tries;
 };
 
-// per-ion constant of the collision exponent: dt P sigma / (1000 kB T)
+// per-ion constant of the collision exponent, dt P sigma c_bar / (1000 kB
+// T): the step's hazard is this times c_rel(sp) / c_bar
 __device__ __forceinline__ float hs_rate_const(const FlightK &p, double dt, double dt_to_us,
                                                double sigma) {
-  return (float)(dt * dt_to_us * p.P_pa * sigma / (1000.0 * IG_KB * p.T_k));
+  return (float)(dt * dt_to_us * p.P_pa * sigma / (1000.0 * IG_KB * p.T_k) * p.c_bar);
 }
 
 // an Exp(1) variate from a word
@@ -499,9 +501,9 @@ __device__ __forceinline__ bool hs_hazard_step(Rng &rng, float &hz, double vx, d
   }
   const float wx = (float)(vx - p.vg[0]), wy = (float)(vy - p.vg[1]),
               wz = (float)(vz - p.vg[2]);
-  float sp = __builtin_sqrtf(wx * wx + wy * wy + wz * wz);
+  float sp = __builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz);
   if (sp < 1e-7f) sp = 1e-7f;
-  hz -= kconst * (float)p.c_bar * c_rel_ratio_f(sp * inv_c_star);
+  hz -= kconst * c_rel_ratio_f(sp * inv_c_star);
   if (hz > 0.0f) return false;
   st.wx = wx;
   st.wy = wy;
@@ -529,29 +531,29 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
   float gx, gy, gz, gl;
   if (u24_low(w[1], w[2], w[3]) * (st.sp + gmean) < st.sp) {
     // Maxwell: three normals from two Box-Muller pairs
-    const float r1 = sig * __builtin_sqrtf(-2.0f * __logf(u24p(w[1])));
-    const float r2 = sig * __builtin_sqrtf(-2.0f * __logf(u24p(w[3])));
+    const float r1 = sig * __builtin_amdgcn_sqrtf(-2.0f * __logf(u24p(w[1])));
+    const float r2 = sig * __builtin_amdgcn_sqrtf(-2.0f * __logf(u24p(w[3])));
     float s1, c1, s2, c2;
     azimuth(u24(w[2]), s1, c1);
     azimuth(u24(w[4]), s2, c2);
     gx = r1 * c1;
     gy = r1 * s1;
     gz = r2 * c2;
-    gl = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
+    gl = __builtin_amdgcn_sqrtf(gx * gx + gy * gy + gz * gz);
   } else {
     // speed-weighted Maxwell: |g| = sig1d sqrt(chi^2_4), isotropic
-    gl = sig * __builtin_sqrtf(-2.0f * __logf(u24p(w[1]) * u24p(w[2])));
+    gl = sig * __builtin_amdgcn_sqrtf(-2.0f * __logf(u24p(w[1]) * u24p(w[2])));
     const float c = __builtin_fmaf(2.0f, u24(w[3]), -1.0f);
     float sn, cs;
     azimuth(u24(w[4]), sn, cs);
     const float one_m = 1.0f - c * c;
-    const float sr = gl * __builtin_sqrtf(one_m > 0.0f ? one_m : 0.0f);
+    const float sr = gl * __builtin_amdgcn_sqrtf(one_m > 0.0f ? one_m : 0.0f);
     gx = sr * cs;
     gy = sr * sn;
     gz = gl * c;
   }
   const float rx = st.wx - gx, ry = st.wy - gy, rz = st.wz - gz;
-  const float rlen = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
+  const float rlen = __builtin_amdgcn_sqrtf(rx * rx + ry * ry + rz * rz);
   if (!(u24(w[0]) * (st.sp + gl) < rlen)) {
     return ++st.tries >= 10000 ? -1 : 0;
   }
@@ -559,7 +561,7 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
   if (rlen < 1e-12f) return 1;
   // sin(asin(sqrt(U))) = sqrt(U), cos = sqrt(1 - U), U < 1
   const float au = u24(w[6]);
-  const float sa = __builtin_sqrtf(au), ca = __builtin_sqrtf(1.0f - au);
+  const float sa = __builtin_amdgcn_sqrtf(au), ca = __builtin_amdgcn_sqrtf(1.0f - au);
   float sth, cth;
   azimuth(u24(w[7]), sth, cth);
   const float inv = __builtin_amdgcn_rcpf(rlen);

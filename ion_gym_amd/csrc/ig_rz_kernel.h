@@ -79,20 +79,20 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
   using L = RecLayout<T, NS, 2>;
   const int64_t nx = f.n0, nu = f.n1;
   // the pre-clamp keeps the conversion defined and gives int(g) clamped as
-  // the reference clamps it
-  int64_t i = (int64_t)fmin(fmax(gx, -1.0), (double)nx);
-  int64_t j = (int64_t)fmin(fmax(gu, -1.0), (double)nu);
+  // the reference clamps it; within [-1, n] a 32-bit index is exact
+  int32_t i = (int32_t)fmin(fmax(gx, -1.0), (double)nx);
+  int32_t j = (int32_t)fmin(fmax(gu, -1.0), (double)nu);
   if (i < 0) i = 0;
   if (j < 0) j = 0;
-  if (i > nx - 2) i = nx - 2;
-  if (j > nu - 2) j = nu - 2;
+  if (i > nx - 2) i = (int32_t)(nx - 2);
+  if (j > nu - 2) j = (int32_t)(nu - 2);
   double fx = gx - (double)i, fu = gu - (double)j;
   if (fx < 0.0) fx = 0.0;
   if (fx > 1.0) fx = 1.0;
   if (fu < 0.0) fu = 0.0;
   if (fu > 1.0) fu = 1.0;
   const T *g = reinterpret_cast<const T *>(f.grid);
-  const T *p0 = g + (i * nu + j) * (int64_t)L::kStride; // (i, j), (i, j+1)
+  const T *p0 = g + ((int64_t)i * nu + j) * (int64_t)L::kStride; // (i, j), (i, j+1)
   const T *p1 = p0 + nu * (int64_t)L::kStride;         // (i+1, j), (i+1, j+1)
   T a00[L::kStride], a01[L::kStride], a10[L::kStride], a11[L::kStride];
   load_vals(p0, a00);
@@ -277,13 +277,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // Node coordinates, the radius and the radial direction: the FP64 modes
   // divide and take FP64 square roots in the reference's order; the
   // FP32-basis mode multiplies by the reciprocal pitch and takes the radius
-  // and its reciprocal in FP32 (relative error ~1e-7, the size of the FP32
-  // field values they weight)
+  // and its reciprocal in FP32 with the hardware square root and reciprocal
+  // (relative error ~1e-7, the size of the FP32 field values they weight)
   const double inv_mm = 1.0 / mm;
   auto node = [&](double v) { return PREC == IG_PREC_F32 ? v * inv_mm : v / mm; };
   auto radius = [&](double a, double b) {
     if constexpr (PREC == IG_PREC_F32)
-      return (double)sqrtf((float)(a * a + b * b));
+      return (double)__builtin_amdgcn_sqrtf((float)(a * a + b * b));
     else
       return sqrt(a * a + b * b);
   };
@@ -372,6 +372,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // drift, the SDS jump and the terminal interpolations move the ion)
   bool rvalid = false;
   double rcur = 0.0;
+  // the radial kick components of the last field evaluation; kfresh while
+  // the ion has not moved since (only the step's drift, the SDS jump and
+  // the terminal interpolations move it), so that a step starting at the
+  // same lab time reuses the field and the kick
+  bool kfresh = false;
+  double ky_last = 0.0, kz_last = 0.0;
+  const double half_dt = 0.5 * dt; // the FP32-basis kicks: one FMA each
   // the step's start point, for the impact bisection and the crossings
   double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
   for (;;) {
@@ -390,34 +397,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         vzo = vz;
         to = t;
         double r = rvalid ? rcur : radius(y, z);
-        efield(x, r, tob + t, ez, er);
-        double rr = r > 1e-9 ? r : 1e-9;
         double ky, kz;
-        radial(er, rr, ky, kz);
+        if (kfresh && tob + t == wtt) {
+          // the point and lab time of the previous step's second field
+          // evaluation: the same inputs, so the same field and kick
+          ky = ky_last;
+          kz = kz_last;
+        } else {
+          efield(x, r, tob + t, ez, er);
+          double rr = r > 1e-9 ? r : 1e-9;
+          radial(er, rr, ky, kz);
+        }
         if constexpr (GAS == IG_GAS_SDS) {
           vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
           vy += 0.5 * sds_fac * (ky - (vy - p.vg[1]) * sds_damp) * dt;
           vz += 0.5 * sds_fac * (kz - (vz - p.vg[2]) * sds_damp) * dt;
+        } else if constexpr (PREC == IG_PREC_F32) {
+          vx = __builtin_fma(ez, half_dt, vx);
+          vy = __builtin_fma(ky, half_dt, vy);
+          vz = __builtin_fma(kz, half_dt, vz);
         } else {
           vx += 0.5 * ez * dt;
           vy += 0.5 * ky * dt;
           vz += 0.5 * kz * dt;
         }
-        x += vx * dt;
-        y += vy * dt;
-        z += vz * dt;
+        if constexpr (PREC == IG_PREC_F32) {
+          x = __builtin_fma(vx, dt, x);
+          y = __builtin_fma(vy, dt, y);
+          z = __builtin_fma(vz, dt, z);
+        } else {
+          x += vx * dt;
+          y += vy * dt;
+          z += vz * dt;
+        }
         if constexpr (REC)
           path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
         r = radius(y, z);
         rcur = r;
         rvalid = true;
         efield(x, r, tob + t + dt, ez, er);
-        rr = r > 1e-9 ? r : 1e-9;
-        radial(er, rr, ky, kz);
+        {
+          const double rr = r > 1e-9 ? r : 1e-9;
+          radial(er, rr, ky, kz);
+        }
+        ky_last = ky;
+        kz_last = kz;
+        kfresh = true;
         if constexpr (GAS == IG_GAS_SDS) {
           vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
           vy += 0.5 * sds_fac * (ky - (vy - p.vg[1]) * sds_damp) * dt;
           vz += 0.5 * sds_fac * (kz - (vz - p.vg[2]) * sds_damp) * dt;
+        } else if constexpr (PREC == IG_PREC_F32) {
+          vx = __builtin_fma(ez, half_dt, vx);
+          vy = __builtin_fma(ky, half_dt, vy);
+          vz = __builtin_fma(kz, half_dt, vz);
         } else {
           vx += 0.5 * ez * dt;
           vy += 0.5 * ky * dt;
@@ -446,6 +479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
             y += jy;
             z += jz;
             rvalid = false;
+            kfresh = false;
             if constexpr (REC)
               path += sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo)) - pre;
           }
