@@ -27,9 +27,10 @@ EDGE_FIELD_MAX_V_PER_MM = 1e-3
 
 _BASE = ("t", "x", "y", "z", "vx", "vy", "vz")
 
-# field-free extension rule per recorded channel: "const" (unchanged by a
-# zero-field drift), "zero" (the drift declares zero field), or
-# (previous value, state, dt_us) -> value
+# field-free extension rule per recorded channel (drift_extension.py:73):
+# "const" (unchanged by a zero-field drift), "zero" (the drift declares
+# zero field) or "integral" (path length and the KE integral grow with the
+# drift time)
 _CHANNEL_RULES = {
     "speed": "const", "ke_ev": "const",
     "ke_x": "const", "ke_y": "const", "ke_z": "const",
@@ -37,33 +38,31 @@ _CHANNEL_RULES = {
     "e_field": "zero", "e_axial": "zero", "e_radial": "zero",
     "e_x": "zero", "e_y": "zero", "e_z": "zero",
     "e_axial_tint": "const",
-    "path_mm": lambda prev, s, dt: prev + s["sp"] * dt,
-    "ke_tint": lambda prev, s, dt: prev + s["ke_ev"] * dt,
+    "path_mm": "integral", "ke_tint": "integral",
 }
+
+
+def _face(arr, axis, side, inset):
+    """The node layer of ``arr`` on one face (``inset`` layers inside)."""
+    a = np.asarray(arr, float)
+    k = int(inset) if side == "lo" else a.shape[axis] - 1 - int(inset)
+    return np.take(a, k, axis=axis)
 
 
 def edge_field_max(components, axis, side, inset=0):
     """Worst-case |E| over one boundary face: each component a list of
-    (array, weight) pairs bounded by sum |w a|, the components combined
-    as a vector norm, the max over the face's nodes (``inset`` node rows
-    inside the face)."""
-    acc = None
+    (array, weight) pairs bounded by sum |w a| node by node, the
+    components combined as a vector norm, the largest over the face's
+    nodes (drift_extension.py:88-117)."""
+    bounds = []
     for comp in components:
-        mag = None
-        for arr, w in comp:
-            if arr is None:
-                continue
-            a = np.asarray(arr, float)
-            sl = [slice(None)] * a.ndim
-            sl[axis] = int(inset) if side == "lo" else -1 - int(inset)
-            face = np.abs(float(w) * a[tuple(sl)])
-            mag = face if mag is None else mag + face
-        if mag is None:
-            continue
-        acc = mag ** 2 if acc is None else acc + mag ** 2
-    if acc is None:
+        terms = [np.abs(float(w) * _face(a, axis, side, inset))
+                 for a, w in comp if a is not None]
+        if terms:
+            bounds.append(np.sum(terms, axis=0))
+    if not bounds:
         raise ValueError("edge_field_max: no field components supplied")
-    return float(np.sqrt(acc).max())
+    return float(np.linalg.norm(np.stack(bounds), axis=0).max())
 
 
 def rz_edge_fields(model):
@@ -134,44 +133,19 @@ def extend_ballistic(traj, col_names, kind, spec, *, edge_e_vpermm,
     if budget <= 1e-12:
         return traj, kind, None
     bflags, bvals = spec.bounds.as_tuple()
-    own_bound = bool(bflags[2 * face_axis + (1 if face_side == "hi" else 0)])
-    own_station = any(
-        int(p[0]) == face_axis
-        and ((face_side == "hi" and float(p[1]) >= pos[face_axis])
-             or (face_side == "lo" and float(p[1]) <= pos[face_axis]))
-        for p in all_planes(spec))
-    if not own_bound and not own_station:
+    planes = all_planes(spec)
+    face_slot = 2 * face_axis + (1 if face_side == "hi" else 0)
+    ahead = (lambda v: v >= pos[face_axis]) if face_side == "hi" else \
+        (lambda v: v <= pos[face_axis])
+    if not (bflags[face_slot] or any(int(q[0]) == face_axis
+                                     and ahead(float(q[1]))
+                                     for q in planes)):
         return traj, kind, None
-    tb, bface = math.inf, None
-    for k in range(6):
-        if not bflags[k]:
-            continue
-        a, hi = k // 2, (k % 2 == 1)
-        v, p, val = vel[a], pos[a], float(bvals[k])
-        if v == 0.0:
-            continue
-        dt = (val - p) / v
-        if dt <= 1e-12:
-            continue
-        crosses = (hi and v > 0.0) or ((not hi) and v < 0.0)
-        if crosses and dt < tb:
-            tb, bface = dt, k
-    st_dts = []
-    ts, ts_kind = math.inf, None
-    for pl in all_planes(spec):
-        a = int(pl[0])
-        if vel[a] == 0.0:
-            continue
-        dt = (float(pl[1]) - pos[a]) / vel[a]
-        if dt > 1e-12:
-            st_dts.append(dt)
-            fate = plane_terminates(pl, pos + vel * dt)
-            if fate is not None and dt < ts:
-                ts, ts_kind = dt, fate
-    bound_reachable = tb <= budget
-    term_reachable = ts <= min(tb, budget)
-    st_reachable = [d for d in st_dts if d <= min(tb, budget)]
-    if not bound_reachable and not st_reachable:
+    tb, bface = _first_bound(pos, vel, bflags, bvals)
+    ts, ts_kind, st_dts = _station_crossings(pos, vel, planes)
+    horizon = min(tb, budget)
+    reach = st_dts[st_dts <= horizon]
+    if tb > budget and reach.size == 0:
         return traj, kind, None
     col = getattr(spec, "collisions", None)
     if col is not None and getattr(col, "enabled", False):
@@ -187,42 +161,93 @@ def extend_ballistic(traj, col_names, kind, spec, *, edge_e_vpermm,
             f"{float(edge_e_vpermm):.3g} V/mm, above the field-free ceiling "
             f"{EDGE_FIELD_MAX_V_PER_MM:g} V/mm; pad the domain until the "
             f"fringe decays")
-    if term_reachable:
+    # the nearest stopping target: a terminating station before any bound,
+    # else the bound, else the last passing station within reach
+    if ts <= horizon:
         dt_end, kind_out, bface = ts, ts_kind, None
-    elif bound_reachable:
+    elif tb <= budget:
         dt_end, kind_out = tb, 3
     else:
-        dt_end, kind_out = max(st_reachable), kind
+        dt_end, kind_out = float(reach.max()), kind
     end = pos + vel * dt_end
-    state = {"sp": float(np.linalg.norm(vel)),
-             "ke_ev": float(last[idx["ke_ev"]]) if "ke_ev" in idx else None}
-    row = np.array(last, float, copy=True)
-    row[idx["t"]] = t0 + dt_end
-    row[idx["x"]], row[idx["y"]], row[idx["z"]] = end
-    derived = derived or {}
-    for c in col_names[7:]:
-        if c in derived:
-            row[idx[c]] = float(derived[c](float(last[idx[c]]), row, idx,
-                                           dt_end))
-            continue
-        rule = _CHANNEL_RULES.get(c)
-        if rule == "const":
-            continue
-        if rule == "zero":
-            row[idx[c]] = 0.0
-            continue
-        if callable(rule):
-            if c == "ke_tint" and state["ke_ev"] is None:
-                raise ValueError("drift extension: ke_tint is recorded "
-                                 "without ke_ev; the extension cannot "
-                                 "derive the integrand")
-            row[idx[c]] = rule(float(last[idx[c]]), state, dt_end)
-            continue
-        raise ValueError(f"drift extension ({label}): recorded channel "
-                         f"{c!r} has no field-free extension rule")
+    row = _drift_row(traj[-1], col_names, idx, t0 + dt_end, end, vel,
+                     dt_end, derived or {}, label)
     return np.vstack([traj, row[None, :]]), kind_out, {
         "dt_us": float(dt_end), "bnd_face": bface,
         "end_mm": tuple(float(v) for v in end)}
+
+
+_BOUND_AXIS = np.array([0, 0, 1, 1, 2, 2])
+_BOUND_HI = np.array([False, True, False, True, False, True])
+
+
+def _first_bound(pos, vel, flags, vals):
+    """(time, bound index) of the first enabled bound the straight line
+    from pos with vel crosses outward, or (inf, None)."""
+    v = vel[_BOUND_AXIS]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dt = (np.asarray(vals, float) - pos[_BOUND_AXIS]) / v
+    outward = np.where(_BOUND_HI, v > 0.0, v < 0.0)
+    ok = np.asarray(flags, bool) & (v != 0.0) & (dt > 1e-12) & outward
+    if not ok.any():
+        return math.inf, None
+    k = int(np.flatnonzero(ok)[np.argmin(dt[ok])])
+    return float(dt[k]), k
+
+
+def _station_crossings(pos, vel, planes):
+    """(time, fate) of the first station crossing that stops the ion (or
+    (inf, None)) and the times of every forward crossing."""
+    times, first, fate = [], math.inf, None
+    for pl in planes:
+        a = int(pl[0])
+        if vel[a] == 0.0:
+            continue
+        dt = (float(pl[1]) - pos[a]) / vel[a]
+        if dt <= 1e-12:
+            continue
+        times.append(dt)
+        stop = plane_terminates(pl, pos + vel * dt)
+        if stop is not None and dt < first:
+            first, fate = dt, stop
+    return first, fate, np.asarray(times, float)
+
+
+def _drift_row(last, col_names, idx, t_end, end, vel, dt, derived, label):
+    """The appended record: the base channels at the end of the straight
+    drift, each optional channel by its field-free rule (a caller's
+    ``derived`` rule first)."""
+    speed = float(np.linalg.norm(vel))
+    ke = float(last[idx["ke_ev"]]) if "ke_ev" in idx else None
+    base = dict(zip(_BASE, (t_end, *end, *vel)))
+    out = np.empty(len(col_names))
+    for j, c in enumerate(col_names):
+        if c in base:
+            out[j] = base[c]
+            continue
+        prev = float(last[j])
+        rule = _CHANNEL_RULES.get(c)
+        if c in derived:
+            row_so_far = np.array(last, float, copy=True)
+            for b, v in base.items():
+                row_so_far[idx[b]] = v
+            out[j] = float(derived[c](prev, row_so_far, idx, dt))
+        elif rule == "const":
+            out[j] = prev
+        elif rule == "zero":
+            out[j] = 0.0
+        elif c == "path_mm":
+            out[j] = prev + speed * dt
+        elif c == "ke_tint":
+            if ke is None:
+                raise ValueError("drift extension: ke_tint is recorded "
+                                 "without ke_ev; the extension cannot "
+                                 "derive the integrand")
+            out[j] = prev + ke * dt
+        else:
+            raise ValueError(f"drift extension ({label}): recorded channel "
+                             f"{c!r} has no field-free extension rule")
+    return out
 
 
 def rz_extend(traj, col_names, kind, spec, model, edge_fields, label=""):

@@ -16,8 +16,10 @@ is flown ion by ion, on one thread or a pool of daemon threads, as the
 reference does.
 """
 import queue
+import sys
 import threading
 import time
+import traceback
 from dataclasses import dataclass, field
 from typing import Callable, Optional
 
@@ -63,29 +65,43 @@ class EnsembleProgress:
 
 
 def _decimate(traj, stride):
-    """Every stride-th row plus the last (ensemble_driver.py:90-98)."""
-    if traj is None or stride <= 1 or traj.shape[0] <= 2:
-        return None if traj is None else traj.copy()
-    idx = np.arange(0, traj.shape[0], stride)
-    if idx[-1] != traj.shape[0] - 1:
-        idx = np.append(idx, traj.shape[0] - 1)
-    return traj[idx].copy()
+    """The record thinned to every stride-th row, the terminal row always
+    kept (it holds the fate's state); a copy, never a view of the
+    flight's buffer (ensemble_driver.py:90-98)."""
+    if traj is None:
+        return None
+    n = traj.shape[0]
+    if stride <= 1 or n <= 2:
+        return traj.copy()
+    keep = np.zeros(n, bool)
+    keep[::stride] = True
+    keep[n - 1] = True
+    return traj[keep]
 
 
 class StopToken:
-    """Cooperative, thread-safe stop flag."""
+    """A cooperative stop shared between threads: ``stop(reason)`` from any
+    thread, ``stopped`` and ``reason`` polled by the driver between
+    batches (ensemble_driver.py:101-116).  The latest stop's reason is
+    the one reported."""
+
+    __slots__ = ("_lock", "_why")
 
     def __init__(self):
-        self._e = threading.Event()
-        self.reason = ""
+        self._lock = threading.Lock()
+        self._why = None
 
     def stop(self, reason: str = "user requested"):
-        self.reason = reason
-        self._e.set()
+        with self._lock:
+            self._why = str(reason)
 
     @property
     def stopped(self) -> bool:
-        return self._e.is_set()
+        return self._why is not None
+
+    @property
+    def reason(self) -> str:
+        return self._why or ""
 
 
 def default_workers() -> int:
@@ -363,62 +379,81 @@ def fly_all(n_ions: int, fly_fn, *, n_workers: int, keep_traj=True,
 def run_threaded(n_ions: int, fly_fn, *, check_every=25, decimate=1,
                  keep_full=False, keep_traj=True, viability=None,
                  n_workers=1, engine_batch=DEFAULT_ENGINE_BATCH) -> RunHandle:
-    """The same generator on a daemon thread; returns a RunHandle."""
+    """run() on a daemon thread; returns a RunHandle whose .latest follows
+    the progress, .stop() asks the run to stop, and .final (with .error
+    and .error_tb when the run raised: the traceback is also written to
+    stderr, and the progress so far is kept) marks it done
+    (ensemble_driver.py:495-533)."""
     handle = RunHandle()
 
-    def _worker():
-        last = None
-        try:
-            for prog in _iter_batches(
-                    n_ions, fly_fn, check_every=check_every,
-                    decimate=decimate, keep_full=keep_full,
-                    keep_traj=keep_traj, n_workers=n_workers,
-                    viability=viability, stop_token=handle._stop,
-                    engine_batch=engine_batch):
-                handle.latest = last = prog
-            handle.final = last
-        except Exception as e:  # noqa: BLE001 -- recorded on the handle
-            import traceback
-            handle.error = e
-            handle.error_tb = traceback.format_exc()
-            handle.final = last
-            print(handle.error_tb, flush=True)
+    def track(prog):
+        handle.latest = prog
 
-    t = threading.Thread(target=_worker, daemon=True)
-    handle._thread = t
-    t.start()
+    def body():
+        try:
+            handle.final = run(n_ions, fly_fn, check_every=check_every,
+                               decimate=decimate, keep_full=keep_full,
+                               keep_traj=keep_traj, viability=viability,
+                               stop_token=handle._stop, on_progress=track,
+                               n_workers=n_workers,
+                               engine_batch=engine_batch)
+        except Exception as exc:  # noqa: BLE001 -- kept on the handle
+            tb = "".join(traceback.format_exception(type(exc), exc,
+                                                    exc.__traceback__))
+            handle.error, handle.error_tb = exc, tb
+            handle.final = handle.latest
+            sys.stderr.write(tb)
+            sys.stderr.flush()
+
+    handle._thread = threading.Thread(target=body, daemon=True,
+                                      name="ensemble-run")
+    handle._thread.start()
     return handle
 
 
-def viability_transmission(min_fraction: float, after: int = 50):
-    """Abort when the impact (kind 0) fraction after `after` ions is below
-    min_fraction."""
-    def _v(results, prog):
-        if prog.done < after:
+# ------------------------------------------------------------ viability
+# Predicates a run consults after each batch: each returns the reason to
+# stop (a string) or None (ensemble_driver.py:535-571).
+
+class viability_transmission:
+    """Stop once, after at least ``after`` ions, the fraction that ended in
+    an impact (kind 0) is below ``min_fraction``."""
+
+    def __init__(self, min_fraction: float, after: int = 50):
+        self.min_fraction = float(min_fraction)
+        self.after = int(after)
+
+    def __call__(self, results, prog):
+        if prog.done < self.after:
             return None
         frac = prog.n_terminated / prog.done
-        if frac < min_fraction:
-            return (f"transmission {frac:.0%} < {min_fraction:.0%} "
-                    f"after {prog.done} ions")
-        return None
-    return _v
+        if frac >= self.min_fraction:
+            return None
+        return (f"transmission {frac:.0%} < {self.min_fraction:.0%} "
+                f"after {prog.done} ions")
 
 
-def viability_beam_shrinks(radius_of, after: int = 50,
-                           min_ratio: float = 0.9):
-    """Abort when the mean final radius of the last third of ions grew
-    past 1/min_ratio of the first third's."""
-    def _v(results, prog):
-        if prog.done < after:
+class viability_beam_shrinks:
+    """Stop when the beam is not confined: over the ions so far with a
+    finite final radius (``radius_of(summary)``), the mean of the last
+    third exceeds 1 / min_ratio times the mean of the first third."""
+
+    def __init__(self, radius_of, after: int = 50, min_ratio: float = 0.9):
+        self.radius_of = radius_of
+        self.after = int(after)
+        self.limit = 1.0 / float(min_ratio)
+
+    def __call__(self, results, prog):
+        if prog.done < self.after:
             return None
-        r = np.array([radius_of(x.summary) for x in results])
-        r = r[np.isfinite(r)]
-        if len(r) < after:
+        radii = np.fromiter((self.radius_of(r.summary) for r in results),
+                            np.float64, len(results))
+        radii = radii[np.isfinite(radii)]
+        if radii.size < self.after:
             return None
-        k = len(r) // 3
-        early, late = r[:k].mean(), r[-k:].mean()
-        if early > 0 and late / early > 1.0 / min_ratio:
-            return (f"beam not confined: final radius grew "
-                    f"{late / early:.2f}x across the run")
-        return None
-    return _v
+        third = radii.size // 3
+        first, last = radii[:third].mean(), radii[radii.size - third:].mean()
+        if first <= 0 or last / first <= self.limit:
+            return None
+        return (f"beam not confined: final radius grew "
+                f"{last / first:.2f}x across the run")

@@ -34,23 +34,27 @@ class StationSpec:
     window: Dict[str, List[float]] = field(default_factory=dict)
 
 
+# (kind, on_hit) -> whether the station reaches the kernels
+# (stations.py:33-57): impact plates always do, a detector only when it
+# absorbs what it sees
+_ROUTING = {("impact_plane", None): True, ("detect", "splat"): True,
+            ("detect", "pass"): False}
+
+
 def reaches_kernel(st):
     """True for the stations that become kernel planes, False for pass
     detectors; anything else is refused by name (stations.py:33-57)."""
     kind = getattr(st, "kind", "")
     name = getattr(st, "name", "<unnamed>")
-    if kind == "impact_plane":
-        return True
-    if kind == "detect":
-        on_hit = getattr(st, "on_hit", None)
-        if on_hit == "splat":
-            return True
-        if on_hit == "pass":
-            return False
+    if kind not in ("impact_plane", "detect"):
+        raise ValueError(f"station {name!r}: kind {kind!r} is not a station "
+                         f"kind (impact_plane, detect)")
+    on_hit = getattr(st, "on_hit", None) if kind == "detect" else None
+    routed = _ROUTING.get((kind, on_hit))
+    if routed is None:
         raise ValueError(f"station {name!r}: kind='detect' needs on_hit "
                          f"'pass' or 'splat', got {on_hit!r}")
-    raise ValueError(f"station {name!r}: kind {kind!r} is not a station kind "
-                     f"(impact_plane, detect)")
+    return routed
 
 
 def station_planes(spec, world_off_mm=(0.0, 0.0, 0.0)):
@@ -101,18 +105,31 @@ def all_planes(spec, world_off_mm=(0.0, 0.0, 0.0)):
                                                  (spec.planes or [])]
 
 
+def _in_window(window, coords):
+    """Boolean mask: every windowed transverse axis of ``coords`` (axis
+    name -> value array) inside its [lo, hi]; all True for no window."""
+    n = len(next(iter(coords.values())))
+    ok = np.ones(n, bool)
+    for ax, (lo, hi) in (window or {}).items():
+        v = np.asarray(coords[ax], float)
+        ok &= (v >= float(lo)) & (v <= float(hi))
+    return ok
+
+
 def station_terminates(st, point):
     """The fate a crossing of station ``st`` at ``point`` (x, y, z)
-    assigns, or None when the ion passes (stations.py:133-151)."""
+    assigns, or None when the ion passes (stations.py:133-151): an impact
+    plate stops what lands outside its window (everything, with no
+    window), an absorbing detector what lands inside it (everything, with
+    no window)."""
     if not reaches_kernel(st):
         return None
     wd = st.window or {}
-    p = dict(zip("xyz", (float(v) for v in point)))
-    inside = bool(wd) and all(float(w[0]) <= p[a] <= float(w[1])
-                              for a, w in wd.items())
+    hit = bool(_in_window(wd, {a: [float(v)] for a, v in
+                               zip("xyz", point)})[0])
     if st.kind == "impact_plane":
-        return None if inside else FATE_IMPACT_PLANE
-    return FATE_DETECT if (inside or not wd) else None
+        return FATE_IMPACT_PLANE if not (wd and hit) else None
+    return FATE_DETECT if hit else None
 
 
 def kernel_stations(spec):
@@ -126,70 +143,62 @@ _STATE = ("t", "x", "y", "z", "vx", "vy", "vz")
 
 def station_hits(traj, cols, station, *, t_min_us=1.0e-3):
     """Every crossing of one station plane by a recorded trajectory, in
-    time order (stations.py:238-275): a crossing between rows j and j+1
-    when row j is strictly on one side and row j+1 on the plane or past
-    it; the state is interpolated linearly at the crossing; k counts all
-    crossings (in or out of the window); crossings at t <= t_min_us are
-    dropped (a birth on the plane); ``in_window`` applies the station's
-    windows."""
+    time order (stations.py:238-275).  Rows j, j+1 bracket a crossing when
+    row j lies strictly on one side and row j+1 on the plane or beyond;
+    the state (t, x, y, z, vx, vy, vz) at the crossing is the linear
+    interpolation between the two rows.  Crossings at t_j <= t_min_us (a
+    birth on the plane) are dropped; ``k`` numbers the rest whether or not
+    they fall in the window, and ``in_window`` applies the station's
+    window."""
     traj = np.asarray(traj, float)
     ix = {c: i for i, c in enumerate(cols)}
-    t = traj[:, ix["t"]]
-    q = traj[:, ix[station.axis]] - float(station.pos_mm)
-    a, b = q[:-1], q[1:]
-    crossing = ((a < 0) & (b >= 0)) | ((a > 0) & (b <= 0))
-    hits = []
-    k = 0
-    for j in np.nonzero(crossing)[0]:
-        if t[j] <= t_min_us:
-            continue
-        f = q[j] / (q[j] - q[j + 1])
-        st = {c: float(traj[j, ix[c]] + f * (traj[j + 1, ix[c]]
-                                             - traj[j, ix[c]]))
-              for c in _STATE}
-        st["t_us"] = st.pop("t")
-        st["k"] = k
-        k += 1
-        inside = True
-        for wax, (lo, hi) in (station.window or {}).items():
-            if not (float(lo) <= st[wax] <= float(hi)):
-                inside = False
-                break
-        st["in_window"] = inside
-        hits.append(st)
-    return hits
+    side = traj[:, ix[station.axis]] - float(station.pos_mm)
+    lo, hi = side[:-1], side[1:]
+    brackets = ((lo < 0) & (hi >= 0)) | ((lo > 0) & (hi <= 0))
+    j = np.flatnonzero(brackets)
+    j = j[traj[j, ix["t"]] > t_min_us]
+    if j.size == 0:
+        return []
+    frac = (lo[j] / (lo[j] - hi[j]))[:, None]
+    sel = [ix[c] for c in _STATE]
+    at = traj[j][:, sel] + frac * (traj[j + 1][:, sel] - traj[j][:, sel])
+    inside = _in_window(station.window,
+                        {c: at[:, n] for n, c in enumerate(_STATE)
+                         if c in "xyz"})
+    names = ("t_us",) + _STATE[1:]
+    return [dict(zip(names, map(float, row)), k=k, in_window=bool(w))
+            for k, (row, w) in enumerate(zip(at, inside))]
+
+
+def _matching_station(spec, name, kind, who):
+    """The one declared station with this name / kind (either may be
+    None), refusing none or several by name."""
+    found = [st for st in (getattr(spec, "stations", None) or [])
+             if (kind is None or st.kind == kind)
+             and (name is None or st.name == name)]
+    if len(found) == 1:
+        return found[0]
+    what = f"'{kind}' " if kind is not None else ""
+    if not found:
+        raise ValueError(f"{who}: the spec declares no matching {what}"
+                         f"station (name={name!r})")
+    raise ValueError(f"{who}: {len(found)} {what}stations match "
+                     f"name={name!r}; pass the station name: "
+                     f"{[st.name for st in found]}")
 
 
 def first_station_crossing(traj, cols, spec, *, name=None, kind=None,
                            t_min_us=1.0e-3):
     """The first in-window crossing of the one declared station matching
     ``name`` / ``kind``, or None (stations.py:278-305)."""
-    sts = [st for st in (spec.stations or [])
-           if (kind is None or st.kind == kind)
-           and (name is None or st.name == name)]
-    want = f"'{kind}' " if kind is not None else ""
-    if not sts:
-        raise ValueError(f"first_station_crossing: the spec declares no "
-                         f"matching {want}station (name={name!r})")
-    if len(sts) > 1:
-        raise ValueError(f"first_station_crossing: {len(sts)} {want}stations "
-                         f"match name={name!r}; pass the station name: "
-                         f"{[d.name for d in sts]}")
-    for h in station_hits(traj, cols, sts[0], t_min_us=t_min_us):
-        if h["in_window"]:
-            return h
-    return None
+    st = _matching_station(spec, name, kind, "first_station_crossing")
+    return next((h for h in station_hits(traj, cols, st, t_min_us=t_min_us)
+                 if h["in_window"]), None)
 
 
 def first_detection(traj, cols, spec, *, name=None, t_min_us=1.0e-3):
     """First in-window crossing of the named (or only) detect station
     (stations.py:308-327)."""
-    try:
-        return first_station_crossing(traj, cols, spec, name=name,
-                                      kind="detect", t_min_us=t_min_us)
-    except ValueError as e:
-        if "declares no matching" in str(e):
-            raise ValueError(
-                "first_detection: the spec declares no matching 'detect' "
-                f"station (name={name!r})") from e
-        raise
+    st = _matching_station(spec, name, "detect", "first_detection")
+    return next((h for h in station_hits(traj, cols, st, t_min_us=t_min_us)
+                 if h["in_window"]), None)

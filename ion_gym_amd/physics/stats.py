@@ -1,17 +1,19 @@
 """Ensemble statistics of a flight (ion_gym/physics/stats.py:45-447).
 
 ``compute_stats(results, transmitted_fate=...)`` reduces an ensemble's
-summaries: fate counts and transmission (the success fate is a required
-argument, ``auto_transmitted_fate(spec)`` answers it for transmission
-devices), per-mass time-of-flight mean, sigma, FWHM and resolving power
-of the transmitted ions, end-point spreads, mean collision count, and
-where ions landed -- grouped by declared plane when the plane set is
-given (``enabled_planes(spec)``), with the plane's normal known, else one
-group per fate with the normal taken as the axis of smallest spread.
-Standard deviations are population (ddof = 0) throughout, as the
-reference states.
+summaries to fate counts and transmission (the success fate is a required
+argument; ``auto_transmitted_fate(spec)`` answers it for transmission
+devices), per-mass time-of-flight mean, sigma, FWHM and resolving power of
+the transmitted ions, end-point spreads, the mean collision count, and
+where ions landed: grouped by declared plane when the plane set is given
+(``enabled_planes(spec)``, whose normal is then known), else one group per
+fate with the axis of least spread as its normal.  Standard deviations are
+population ones (ddof = 0), as the reference states.
+
+The summaries are first laid out as columns (one float array per key,
+NaN where a summary lacks the key or holds None), so every reduction below
+is an array operation over a boolean row mask.
 """
-import math
 from dataclasses import dataclass
 from typing import Optional
 
@@ -20,58 +22,45 @@ import numpy as np
 from .ion_envelope import FATE_NAME
 
 FWHM_PER_SIGMA = 2.35482004503
-
-
-def _summaries(results):
-    return [r.summary if hasattr(r, "summary") else r for r in results]
-
-
-def _mean_std(xs):
-    xs = [float(x) for x in xs if x is not None]
-    n = len(xs)
-    if n == 0:
-        return None, None, 0
-    mu = sum(xs) / n
-    if n == 1:
-        return mu, 0.0, 1
-    var = sum((x - mu) ** 2 for x in xs) / n
-    return mu, math.sqrt(var), n
+_AXES = "xyz"
+_KEYS = ("kind", "tof", "mz", "x_end", "y_end", "z_end", "r_end", "n_col",
+         "ke_end", "vx_end", "vy_end", "vz_end")
 
 
 def auto_transmitted_fate(spec):
-    """3 when any bounding plane is enabled, else 1 (stats.py:69-89)."""
+    """The fate that counts as transmitted for a transmission device: 3
+    (a bounding plane) when the deck enables any bound, else 1 (left the
+    domain) (stats.py:69-89)."""
     b = getattr(spec, "bounds", None)
-    if b is not None:
-        for ax in ("x", "y", "z"):
-            for side in ("min", "max"):
-                if getattr(b, f"{ax}_{side}_on", False):
-                    return 3
+    if b is not None and any(getattr(b, f"{a}_{side}_on", False)
+                             for a in _AXES for side in ("min", "max")):
+        return 3
     return 1
 
 
 def mz_of_results(spec, n):
-    """Per-ion mass for n results, contiguous blocks of n_ions
-    (stats.py:92-108)."""
-    lst = list(spec.source.mz_list or [])
-    if not lst:
+    """Per-ion mass of n results: the source's masses in contiguous blocks
+    of n_ions (stats.py:92-108)."""
+    masses = list(spec.source.mz_list or [])
+    if not masses:
         return [None] * n
-    n_per = max(1, int(getattr(spec.source, "n_ions", 1)))
-    return [lst[(i // n_per) % len(lst)] for i in range(n)]
+    block = max(1, int(getattr(spec.source, "n_ions", 1)))
+    return [masses[(i // block) % len(masses)] for i in range(n)]
 
 
 def enabled_planes(spec):
-    """The enabled bounding planes as (label, axis, coord_mm)
+    """(label, axis, coord_mm) of each enabled bounding plane
     (stats.py:186-201)."""
     b = getattr(spec, "bounds", None) if spec is not None else None
     if b is None:
         return []
-    out = []
-    for axis in ("x", "y", "z"):
+    planes = []
+    for a in _AXES:
         for side in ("min", "max"):
-            if getattr(b, f"{axis}_{side}_on", False):
-                coord = float(getattr(b, f"{axis}_{side}"))
-                out.append((f"{axis}={coord:+g}", axis, coord))
-    return out
+            if getattr(b, f"{a}_{side}_on", False):
+                c = float(getattr(b, f"{a}_{side}"))
+                planes.append((f"{a}={c:+g}", a, c))
+    return planes
 
 
 @dataclass
@@ -133,75 +122,143 @@ class Stats:
     landing: dict = None
 
 
-def _tof_block(rows):
-    """Arrival-time statistics of one landing group (stats.py:204-223): a
-    stated blank for mixed masses or no recorded tof; no width below two
-    ions."""
-    mzs = {s.get("mz") for s in rows if s.get("mz") is not None}
-    tofs = [s.get("tof") for s in rows if s.get("tof") is not None]
-    if not tofs:
+class _Columns:
+    """The summaries as float columns (NaN for a missing or None value)
+    plus the raw fate of each row (fates are compared as given)."""
+
+    def __init__(self, summaries, mz):
+        self.n = len(summaries)
+        self.kind = [s.get("kind") for s in summaries]
+        self.col = {}
+        for key in _KEYS[1:]:
+            self.col[key] = np.array(
+                [np.nan if s.get(key) is None else float(s.get(key))
+                 for s in summaries], np.float64)
+        # the per-mass grouping key (the caller's masses; None keeps
+        # unknown masses apart from numbers)
+        self.mz_key = (list(mz) if mz is not None else [None] * self.n)
+
+    def rows(self, mask):
+        return np.flatnonzero(mask)
+
+    def fate_is(self, code):
+        return np.array([k == code for k in self.kind], bool)
+
+
+def _moments(v):
+    """(mean, population sigma, count) of the finite entries, or
+    (None, None, 0) when there are none."""
+    v = v[np.isfinite(v)]
+    if v.size == 0:
+        return None, None, 0
+    return float(v.mean()), float(v.std()) if v.size > 1 else 0.0, int(v.size)
+
+
+def _tof_width(tof, masses):
+    """Arrival-time block of a landing group (stats.py:204-223): blank
+    with a stated reason for mixed masses or no recorded tof, and no width
+    below two ions (or with none)."""
+    t = tof[np.isfinite(tof)]
+    if t.size == 0:
         return dict(tof_n=0, tof_note="route records no tof")
-    if len(mzs) > 1:
+    if len({m for m in masses if m is not None}) > 1:
         return dict(tof_n=0, tof_note="mixed masses — see the per-mass tabs")
-    mu, sd, cnt = _mean_std(tofs)
+    mu, sd, cnt = _moments(t)
     if cnt < 2 or not sd:
         return dict(tof_n=cnt, tof_mean=mu, tof_note="n < 2 — no width")
-    fwhm = FWHM_PER_SIGMA * sd
-    return dict(tof_n=cnt, tof_mean=mu, tof_fwhm=fwhm, tof_R=mu / (2 * fwhm))
+    w = FWHM_PER_SIGMA * sd
+    return dict(tof_n=cnt, tof_mean=mu, tof_fwhm=w, tof_R=mu / (2 * w))
 
 
-def _plane_stats(rows, fate, normal=None, plane_label=""):
-    """Landing statistics of one group (stats.py:226-297)."""
-    pts = {a: [s.get(f"{a}_end") for s in rows] for a in "xyz"}
-    if not rows or any(v is None for v in pts["x"]):
+def _landing(cols, idx, fate, normal=None, label=""):
+    """Where the rows idx of one fate landed (stats.py:226-297): spread
+    about the centroid in the plane across ``normal`` (the axis of least
+    spread when not given), energy, velocity split and divergence, and the
+    arrival-time block.  None for an empty group or one missing an end
+    point."""
+    if idx.size == 0:
         return None
-    stat = {}
-    for a in "xyz":
-        mu, sd, cnt = _mean_std(pts[a])
-        if cnt == 0:
-            return None
-        stat[a] = (mu, sd or 0.0)
-    if normal not in ("x", "y", "z"):
-        normal = min("xyz", key=lambda a: stat[a][1])
-    ip = tuple(a for a in "xyz" if a != normal)
-    cx, cy = stat[ip[0]][0], stat[ip[1]][0]
-    rs = sorted(math.hypot(px - cx, py - cy)
-                for px, py in zip(pts[ip[0]], pts[ip[1]]))
-    n = len(rs)
-    r_mean = sum(rs) / n
-    r_rms = math.sqrt(sum(r * r for r in rs) / n)
-    r_p95 = rs[min(n - 1, int(0.95 * n))]
-    kes = [r.get("ke_end") for r in rows if r.get("ke_end") is not None]
-    ke_mu, ke_sd, _ = _mean_std(kes) if kes else (0.0, 0.0, 0)
-    vel = [(r.get("vx_end"), r.get("vy_end"), r.get("vz_end")) for r in rows]
-    vel = [v for v in vel if all(c is not None for c in v)]
-    vn_mu = vn_sd = vp_mu = vp_sd = vp_rms = d_rms = d_p95 = 0.0
-    if vel:
-        V = np.asarray(vel, float)
-        k = "xyz".index(normal)
-        vN = V[:, k]
-        vP = np.hypot(*[V[:, j] for j in range(3) if j != k])
-        vn_mu = float(vN.mean())
-        vn_sd = float(vN.std() if len(vN) > 1 else 0.0)
-        vp_mu = float(vP.mean())
-        vp_sd = float(vP.std() if len(vP) > 1 else 0.0)
-        vp_rms = float(np.sqrt((vP ** 2).mean()))
+    end = np.stack([cols.col[f"{a}_end"][idx] for a in _AXES], axis=1)
+    if not np.all(np.isfinite(end[:, 0])):
+        return None
+    mean = end.mean(axis=0)
+    sig = end.std(axis=0) if idx.size > 1 else np.zeros(3)
+    if normal is None or normal not in _AXES:
+        normal = _AXES[int(np.argmin(sig))]
+    k = _AXES.index(normal)
+    across = [q for q in range(3) if q != k]
+    c0, c1 = float(mean[across[0]]), float(mean[across[1]])
+    r = np.sort(np.hypot(end[:, across[0]] - c0, end[:, across[1]] - c1))
+    n = r.size
+    ke = cols.col["ke_end"][idx]
+    ke = ke[np.isfinite(ke)]
+    ke_mu, ke_sd, _ = _moments(ke)
+    v = np.stack([cols.col[f"v{a}_end"][idx] for a in _AXES], axis=1)
+    v = v[np.all(np.isfinite(v), axis=1)]
+    vel = dict(v_norm_mean=0.0, v_norm_sigma=0.0, v_perp_mean=0.0,
+               v_perp_sigma=0.0, v_perp_rms=0.0, div_mrad_rms=0.0,
+               div_mrad_p95=0.0)
+    if v.shape[0]:
+        vn = v[:, k]
+        vp = np.hypot(v[:, across[0]], v[:, across[1]])
+        several = v.shape[0] > 1
+        vel.update(v_norm_mean=float(vn.mean()),
+                   v_norm_sigma=float(vn.std()) if several else 0.0,
+                   v_perp_mean=float(vp.mean()),
+                   v_perp_sigma=float(vp.std()) if several else 0.0,
+                   v_perp_rms=float(np.sqrt(np.mean(vp * vp))))
         with np.errstate(invalid="ignore", divide="ignore"):
-            ang = np.abs(np.arctan2(vP, np.abs(vN))) * 1e3
-        ang = ang[np.isfinite(ang)]
-        if ang.size:
-            d_rms = float(np.sqrt((ang ** 2).mean()))
-            d_p95 = float(np.percentile(ang, 95.0))
+            mrad = 1e3 * np.abs(np.arctan2(vp, np.abs(vn)))
+        mrad = mrad[np.isfinite(mrad)]
+        if mrad.size:
+            vel.update(div_mrad_rms=float(np.sqrt(np.mean(mrad * mrad))),
+                       div_mrad_p95=float(np.percentile(mrad, 95.0)))
+    # mixed masses are told by the summaries' own mz
+    own = cols.col["mz"][idx]
+    masses = list(own[np.isfinite(own)])
     return PlaneStats(
-        fate=fate, n=n, normal=normal, normal_mean=stat[normal][0],
-        normal_sigma=stat[normal][1], in_plane=ip, centroid=(cx, cy),
-        sigma=(stat[ip[0]][1], stat[ip[1]][1]), r_mean=r_mean, r_rms=r_rms,
-        r_p95=r_p95, plane_label=plane_label, ke_n=len(kes),
-        ke_mean=ke_mu or 0.0, ke_sigma=ke_sd or 0.0,
-        ke_min=min(kes) if kes else 0.0, ke_max=max(kes) if kes else 0.0,
-        v_norm_mean=vn_mu, v_norm_sigma=vn_sd, v_perp_mean=vp_mu,
-        v_perp_sigma=vp_sd, v_perp_rms=vp_rms, div_mrad_rms=d_rms,
-        div_mrad_p95=d_p95, **_tof_block(rows))
+        fate=fate, n=n, normal=normal, normal_mean=float(mean[k]),
+        normal_sigma=float(sig[k]), in_plane=tuple(_AXES[q] for q in across),
+        centroid=(c0, c1), sigma=(float(sig[across[0]]),
+                                  float(sig[across[1]])),
+        r_mean=float(r.mean()), r_rms=float(np.sqrt(np.mean(r * r))),
+        r_p95=float(r[min(n - 1, int(0.95 * n))]), plane_label=label,
+        ke_n=int(ke.size), ke_mean=ke_mu or 0.0, ke_sigma=ke_sd or 0.0,
+        ke_min=float(ke.min()) if ke.size else 0.0,
+        ke_max=float(ke.max()) if ke.size else 0.0, **vel,
+        **_tof_width(cols.col["tof"][idx], masses))
+
+
+def _spread(cols, idx):
+    out = {}
+    for a in _AXES:
+        mu, sd, cnt = _moments(cols.col[f"{a}_end"][idx])
+        if cnt:
+            out[a] = (mu, sd)
+    return out
+
+
+def _plane_tolerance(planes, tol):
+    """The match tolerance: at most half the smallest gap between two
+    declared planes on one axis."""
+    for a in _AXES:
+        cs = np.unique([c for _, ax, c in planes if ax == a])
+        if cs.size > 1:
+            tol = min(tol, 0.5 * float(np.diff(cs).min()))
+    return tol
+
+
+def _assign_planes(cols, idx, planes, tol):
+    """For rows idx, the index of the nearest declared plane within tol
+    (distance along the plane's own axis), or -1."""
+    dist = np.full((idx.size, len(planes)), np.inf)
+    for p, (_, a, c) in enumerate(planes):
+        d = np.abs(cols.col[f"{a}_end"][idx] - c)
+        dist[:, p] = np.where(np.isfinite(d), d, np.inf)
+    best = np.argmin(dist, axis=1) if planes else np.zeros(idx.size, int)
+    ok = dist[np.arange(idx.size), best] <= tol if planes else \
+        np.zeros(idx.size, bool)
+    return np.where(ok, best, -1)
 
 
 def compute_stats(results, *, transmitted_fate=None, mz=None, planes=None,
@@ -220,79 +277,58 @@ def compute_stats(results, *, transmitted_fate=None, mz=None, planes=None,
     if transmitted_fate not in FATE_NAME:
         raise ValueError(f"compute_stats: transmitted_fate "
                          f"{transmitted_fate!r} is not a fate code")
-    S = _summaries(results)
-    n = len(S)
-    mz = list(mz) if mz is not None else [None] * n
+    summaries = [r.summary if hasattr(r, "summary") else r for r in results]
+    cols = _Columns(summaries, mz)
+    n = cols.n
     fate_counts = {}
-    for s in S:
-        k = s.get("kind")
+    for k in cols.kind:
         fate_counts[k] = fate_counts.get(k, 0) + 1
-    ok = [(s, m) for s, m in zip(S, mz) if s.get("kind") == transmitted_fate]
-    impact_rows = [s for s in S if s.get("kind") == 0]
-    transmission = (len(ok) / n) if n else None
-    groups = {}
-    for s, m in ok:
-        groups.setdefault(m, []).append(s.get("tof"))
+    ok = cols.rows(cols.fate_is(transmitted_fate))
+
+    # per mass, over the transmitted ions (masses sorted, None last)
     per_mz = []
-    for m in sorted(groups, key=lambda v: (v is None, v)):
-        mu, sd, cnt = _mean_std(groups[m])
-        fwhm = R = None
-        if mu is not None and sd is not None:
+    ok_masses = [cols.mz_key[i] for i in ok]
+    for m in sorted(set(ok_masses), key=lambda v: (v is None, v)):
+        sel = ok[[q for q, mm in enumerate(ok_masses) if mm == m]]
+        mu, sd, cnt = _moments(cols.col["tof"][sel])
+        fwhm = rp = None
+        if mu is not None:
             fwhm = FWHM_PER_SIGMA * sd
-            R = (mu / (2 * fwhm)) if fwhm > 0 else float("inf")
-        per_mz.append(MzStats(m, cnt, mu, sd, fwhm, R))
+            rp = mu / (2 * fwhm) if fwhm > 0 else float("inf")
+        per_mz.append(MzStats(m, cnt, mu, sd, fwhm, rp))
 
-    def spread(rows):
-        out = {}
-        for key in ("x_end", "y_end", "z_end"):
-            mu, sd, cnt = _mean_std([s.get(key) for s in rows])
-            if cnt:
-                out[key[0]] = (mu, sd)
-        return out
+    r_mu, r_sd, _ = _moments(cols.col["r_end"][ok])
+    nc_mu, _, _ = _moments(cols.col["n_col"])
 
-    ok_rows = [s for s, _ in ok]
-    r_mu, r_sd, _ = _mean_std([s.get("r_end") for s in ok_rows])
-    nc_mu, _, _ = _mean_std([s.get("n_col") for s in S])
     landing = {}
     plist = list(planes) if planes else []
-    tol = float(plane_tol_mm)
-    for ax in "xyz":
-        cs = sorted(c for _, a, c in plist if a == ax)
-        gaps = [b - a for a, b in zip(cs, cs[1:]) if b > a]
-        if gaps:
-            tol = min(tol, 0.5 * min(gaps))
+    tol = _plane_tolerance(plist, float(plane_tol_mm))
     for code in sorted(fate_counts, key=lambda c: (c is None, c)):
-        rows = [s for s in S if s.get("kind") == code]
+        idx = cols.rows(cols.fate_is(code))
         if not plist:
-            ps = _plane_stats(rows, code)
+            ps = _landing(cols, idx, code)
             if ps is not None:
                 landing[code] = ps
             continue
-        by_plane, residual = {}, []
-        for s in rows:
-            best = None
-            for label, axis, coord in plist:
-                v = s.get(f"{axis}_end")
-                if v is None:
-                    continue
-                d = abs(v - coord)
-                if best is None or d < best[0]:
-                    best = (d, label, axis)
-            if best is not None and best[0] <= tol:
-                by_plane.setdefault((best[1], best[2]), []).append(s)
-            else:
-                residual.append(s)
-        for (label, axis), prows in sorted(by_plane.items()):
-            ps = _plane_stats(prows, code, normal=axis, plane_label=label)
+        which = _assign_planes(cols, idx, plist, tol)
+        hit = sorted({int(p) for p in which if p >= 0},
+                     key=lambda p: (plist[p][0], plist[p][1]))
+        for p in hit:
+            label, axis, _ = plist[p]
+            ps = _landing(cols, idx[which == p], code, normal=axis,
+                          label=label)
             if ps is not None:
                 landing[f"{code}:{label}"] = ps
-        if residual:
-            ps = _plane_stats(residual, code)
+        rest = idx[which < 0]
+        if rest.size:
+            ps = _landing(cols, rest, code)
             if ps is not None:
-                landing[f"{code}:·" if by_plane else code] = ps
+                landing[f"{code}:·" if hit else code] = ps
+
     return Stats(n_total=n, fate_counts=fate_counts,
-                 transmitted_fate=transmitted_fate, transmission=transmission,
-                 per_mz=per_mz, end_spread=spread(ok_rows), r_end_mean=r_mu,
+                 transmitted_fate=transmitted_fate,
+                 transmission=(ok.size / n) if n else None, per_mz=per_mz,
+                 end_spread=_spread(cols, ok), r_end_mean=r_mu,
                  r_end_sigma=r_sd, n_col_mean=nc_mu,
-                 impact_spread=spread(impact_rows), n_transmitted=len(ok),
-                 landing=landing)
+                 impact_spread=_spread(cols, cols.rows(cols.fate_is(0))),
+                 n_transmitted=int(ok.size), landing=landing)
