@@ -123,6 +123,37 @@ def test_cascade_matches_oracle_bitwise(eng, thin):
     _same(p, q)
 
 
+def _slab_129x65x129():
+    fixed = np.zeros((129, 65, 129), bool)
+    val = np.zeros(fixed.shape)
+    fixed[32:97, 31:34, 32:97] = True
+    val[fixed] = 1e4
+    fixed[:, 0, :] = True
+    fixed[:, -1, :] = True
+    return fixed, val
+
+
+@pytest.mark.parametrize("route", ["sor", "cascade"])
+def test_two_stage_pair_maximum_matches_oracle_bitwise(eng, route):
+    """129x65x129: 17 x 129 = 2193 workgroups per colour sweep, so a pair's
+    4386 partial maxima exceed the one-workgroup limit of 4096 and the
+    convergence check takes the two-stage reduction."""
+    from ion_gym_amd.physics import solver3d as S
+    fixed, val = _slab_129x65x129()
+    half = -(-65 * 65 // 256)
+    assert 2 * half * 129 > 4096
+    om = S.optimal_omega(fixed.shape)
+    print(f"{route} slab 129x65x129 two-stage maximum", flush=True)
+    if route == "sor":
+        p, sw, d = S.solve3d(fixed, val, tol=1e-2, engine=eng)
+        q, sw_o, d_o, st = O.solve3d(fixed, val, tol=1e-2, omega=om)
+    else:
+        p, sw, d = S.solve3d_mg(fixed, val, tol=1e-2, engine=eng)
+        q, sw_o, d_o, st, _ = O.solve3d_mg(fixed, val, tol=1e-2, omega=om)
+    assert st == 0 and sw == sw_o and d == d_o
+    _same(p, q)
+
+
 @pytest.mark.parametrize("stencil", ["ghost_linear", "sw"])
 def test_cascade_fractions_match_oracle_bitwise(eng, stencil):
     from ion_gym_amd.physics import solver3d as S
