@@ -202,6 +202,54 @@ def funnel_electrodes(coarse=2):
     return lab, h, bases, dc, members, groups
 
 
+def funnel_deck(coarse=2):
+    """The funnel's electrodes as a deck: 100 ring electrodes (rectangles
+    in (z, r), mm), a 100 -> 1 V linear DC ladder over their rung numbers
+    (1 V/mm, the same volts as funnel_electrodes) and the two RF groups on
+    alternate rings, on a cylindrical 120 x 20 mm domain at 0.1 * coarse
+    mm.  Its bases come from rz_deck_labels and a Laplace solve; its
+    voltages from the deck (fields_rz.compose_spec_rz)."""
+    from .physics.flight_spec import (DCGroupSpec, ElectrodeSpec, FlightSpec,
+                                      GeometrySpec, RFGroupSpec)
+    rings = []
+    for i in range(100):
+        z0 = 10.0 + i * 1.0
+        rin = 12.5 + (1.0 - 12.5) * i / 99.0
+        rings.append(ElectrodeSpec(
+            name=f"ring{i + 1:03d}",
+            shapes=[{"type": "rect", "z0_mm": z0, "z1_mm": z0 + 0.5,
+                     "r0_mm": rin, "r1_mm": rin + 3.0}],
+            rf_groups=["rfA" if i % 2 == 0 else "rfB"],
+            dc_group="ladder", dc_index=i))
+    geo = GeometrySpec(
+        width_mm=120.0, height_mm=20.0, mm_per_gu=0.1 * coarse,
+        symmetry="cylindrical", electrodes=rings,
+        dc_groups=[DCGroupSpec(name="ladder", v_in=100.0, v_out=1.0)],
+        rf_groups=[RFGroupSpec("rfA", 1.0e6, 100.0, 0.0),
+                   RFGroupSpec("rfB", 1.0e6, 100.0, 180.0)])
+    return FlightSpec(geometry=geo, name=f"funnel_deck_c{coarse}")
+
+
+def rz_deck_labels(spec):
+    """(nz, nr) int labels of an r-z deck whose electrodes are 'rect'
+    shapes: electrode k (1-based position, or its ``basis``) painted on
+    the node centres inside its rectangles."""
+    g = spec.geometry
+    h = g.mm_per_gu
+    nz = int(round(g.width_mm / h)) + 1
+    nr = int(round(g.height_mm / h)) + 1
+    rects = []
+    for pos, el in enumerate(g.electrodes, start=1):
+        idx = el.basis if el.basis is not None else pos
+        for sh in el.shapes:
+            if sh.get("type") != "rect":
+                raise ValueError(f"electrode {el.name!r}: shape type "
+                                 f"{sh.get('type')!r} is not 'rect'")
+            rects.append((idx, sh["z0_mm"], sh["z1_mm"], sh["r0_mm"],
+                          sh["r1_mm"]))
+    return _grid_labels(nz, nr, h, rects)
+
+
 def drives_for(model):
     """DriveGroup view of a composed model's channels (for reports)."""
     out = []

@@ -198,3 +198,27 @@ def compose_rz(bases, ele, h, dc, members, groups):
                    ch_om=ch_om, ch_ph=ch_ph, ch_duty=ch_duty, ele=ele,
                    u0=float(Ue[0]), mm_per_gu=h, EzG=EzG, EuG=EuG,
                    tau_gate=tau_gate)
+
+
+def spec_terms_rz(spec):
+    """(dc, members, groups) for compose_rz from a deck's geometry: each
+    electrode's basis index (its ``basis``, else its 1-based position)
+    with its effective DC after the ladders are resolved
+    (FlightSpec.resolve_dc_groups, dc_effective) and its drive-group
+    names, and the deck's drive groups (build_rz.py:435-534 reads the same
+    three)."""
+    spec.resolve_dc_groups()
+    dc, members = {}, {}
+    for pos, el in enumerate(spec.geometry.electrodes, start=1):
+        idx = el.basis if el.basis is not None else pos
+        dc[idx] = spec.dc_effective(el)
+        members[idx] = tuple(el.group_names())
+    return dc, members, list(spec.geometry.rf_groups)
+
+
+def compose_spec_rz(spec, bases, ele):
+    """The r-z model of a deck from its solved bases (compose_rz with
+    spec_terms_rz); the grid pitch is geometry.mm_per_gu."""
+    dc, members, groups = spec_terms_rz(spec)
+    return compose_rz(bases, ele, spec.geometry.mm_per_gu, dc, members,
+                      groups)

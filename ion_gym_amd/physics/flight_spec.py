@@ -174,20 +174,101 @@ class RFGroupSpec:
         return "pseudo" if self.waveform == "sin" else "instant"
 
 
+@dataclass
+class ElectrodeSpec:
+    """A named conductor (sim_spec.py:1000-1036): its metal (``shapes``, a
+    list of JSON shape dicts, or an ``stl`` file) and its voltages -- a
+    ``dc`` plus membership in any number of drive groups (``rf_groups``)
+    and at most one DC ladder (``dc_group`` with its rung ``dc_index`` or
+    tap ``dc_weight``).  ``basis`` is the index of its solved basis
+    (default: its 1-based position)."""
+    name: str
+    shapes: List[dict] = field(default_factory=list)
+    stl: Optional[str] = None
+    is_grid: bool = False
+    dc: float = 0.0
+    rf_groups: List[str] = field(default_factory=list)
+    dc_group: Optional[str] = None
+    dc_index: Optional[int] = None
+    dc_weight: Optional[float] = None
+    color: List[int] = field(default_factory=lambda: [200, 60, 60])
+    basis: Optional[int] = None
+
+    def group_names(self):
+        """Drive-group memberships, first occurrence order, no repeats."""
+        return list(dict.fromkeys(g for g in self.rf_groups if g))
+
+
+DC_INTERPS = ("linear", "weights")
+
+
+@dataclass
+class DCGroupSpec:
+    """A DC ladder fed from two ends (sim_spec.py:901-958): v_in on the
+    member with the lowest dc_index, v_out on the highest, the rest
+    equally spaced by index ('linear') or placed by their dc_weight
+    ('weights'); ``uniform`` puts every member at v_in.  mode 'set' writes
+    the member dc, 'add' rides on top of the authored dc."""
+    name: str = "LADDER"
+    v_in: float = 0.0
+    v_out: float = 0.0
+    interp: str = "linear"
+    mode: str = "set"
+    uniform: bool = False
+
+
+def dc_group_member_values(grp, members):
+    """{member name: the group's voltage for it} (sim_spec.py:961-996); a
+    weighted ladder with an unweighted member, or an indexed ladder with
+    an unnumbered one, is refused by name."""
+    v_in, v_out = float(grp.v_in), float(grp.v_out)
+    if grp.uniform:
+        return {e.name: v_in for e in members}
+    if grp.interp == "weights":
+        pos = {e.name: e.dc_weight for e in members}
+        missing = [n for n, w in pos.items() if w is None]
+        if missing:
+            raise ValueError(f"dc_group {grp.name!r} (interp='weights'): "
+                             f"member(s) {missing} have no dc_weight -- a "
+                             f"weighted ladder needs every tap placed")
+        return {n: v_in + (v_out - v_in) * float(w) for n, w in pos.items()}
+    rung = {e.name: e.dc_index for e in members}
+    missing = [n for n, k in rung.items() if k is None]
+    if missing:
+        raise ValueError(f"dc_group {grp.name!r} (interp={grp.interp!r}): "
+                         f"member(s) {missing} have no dc_index -- an "
+                         f"indexed ladder needs every rung numbered")
+    lo = min(rung.values())
+    span = (max(rung.values()) - lo) or 1
+    # (v_out - v_in) * (k - lo) first, then / span: the rungs of an
+    # integer-volt ladder stay exact
+    return {n: v_in + (v_out - v_in) * (k - lo) / span
+            for n, k in rung.items()}
+
+
 FIELD_METHODS = ("electrode_aware", "plain_gradient")
 CHANNEL_DTYPES = ("float64", "float32")
 
 
 @dataclass
 class GeometrySpec:
-    """The geometry fields the flight reads (sim_spec.py:1330-1345):
+    """The geometry fields the flight reads (sim_spec.py:1252-1345):
     field_method (the 3-D field build: one-sided differences at metal
     nodes, or np.gradient everywhere), channel_dtype (storage of the
     channel stacks: float64, or float32 read back as float64) and the
-    drive groups."""
+    drive groups; plus, for a deck whose bases are cached, the solve
+    domain (width/height/depth mm, mm_per_gu, symmetry), the electrodes
+    and the DC ladders."""
     field_method: str = "electrode_aware"
     channel_dtype: str = "float64"
     rf_groups: List[RFGroupSpec] = field(default_factory=list)
+    width_mm: Optional[float] = None
+    height_mm: Optional[float] = None
+    depth_mm: float = 0.0
+    mm_per_gu: float = 0.1
+    symmetry: object = "planar"
+    electrodes: List[ElectrodeSpec] = field(default_factory=list)
+    dc_groups: List[DCGroupSpec] = field(default_factory=list)
 
     def validate(self):
         if self.field_method not in FIELD_METHODS:
@@ -199,6 +280,13 @@ class GeometrySpec:
                              f"{CHANNEL_DTYPES}")
         for g in self.rf_groups:
             g.validate()
+        for d in self.dc_groups:
+            if d.interp not in DC_INTERPS:
+                raise ValueError(f"dc_group {d.name!r}: interp must be one "
+                                 f"of {DC_INTERPS}")
+            if d.mode not in ("set", "add"):
+                raise ValueError(f"dc_group {d.name!r}: mode must be 'set' "
+                                 f"or 'add'")
 
 
 @dataclass
@@ -287,6 +375,39 @@ class FlightSpec:
     # declared stations (ion_gym_amd.physics.stations.StationSpec): the
     # kernel-reaching ones become planes before ``planes``
     stations: List = field(default_factory=list)
+    # the deck's name (provenance of cached bases) and its builder (part of
+    # the basis key; "" dispatches on the geometry)
+    name: str = "simulation"
+    builder: str = ""
+
+    def resolve_dc_groups(self):
+        """Write every 'set'-mode ladder member's dc from its group
+        (sim_spec.py:2633-2663); a uniform group's v_out follows v_in.
+        Returns the spec."""
+        g = self.geometry
+        for grp in g.dc_groups:
+            members = [e for e in g.electrodes if e.dc_group == grp.name]
+            if not members or grp.mode == "add":
+                continue
+            values = dc_group_member_values(grp, members)
+            if grp.uniform:
+                grp.v_out = grp.v_in
+            for e in members:
+                e.dc = values[e.name]
+        return self
+
+    def dc_effective(self, el):
+        """The static volts electrode ``el`` contributes: its dc, plus its
+        'add'-mode group's value for it (sim_spec.py:2665-2686)."""
+        base = float(el.dc or 0.0)
+        grp = next((x for x in self.geometry.dc_groups
+                    if el.dc_group is not None and x.name == el.dc_group),
+                   None)
+        if grp is None or grp.mode != "add":
+            return base
+        members = [e for e in self.geometry.electrodes
+                   if e.dc_group == grp.name]
+        return base + dc_group_member_values(grp, members)[el.name]
 
     def engine_precision(self):
         """The engine's field precision for this deck."""
