@@ -66,3 +66,38 @@ def test_an_ensemble_alone_equals_it_in_the_batch(engine, batch_cfg):
         alone.release()
     together.release()
     fp.release()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_full_size_batch_transport(engine, prec):
+    """The benchmark's batch at full size: the 801 x 101 tube, 64
+    ensembles of 1e3-2e5 ions (3.0e6 in all) in one slab, in both engine
+    precisions, 8000 steps: every ensemble drifts at its K E within 1 %,
+    the pooled longitudinal diffusion is 2 D t within 1 % and each
+    ensemble's within 1 % or four of its standard errors."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.batch import EnsembleBatch
+    from ion_gym_amd.engine import Flight
+    c = configs.drift_batch_rz(coarse=1)
+    ns = np.array([e["n"] for e in c["ensembles"]], float)
+    print(f"drift batch full size {prec}: {int(ns.sum())} ions", flush=True)
+    fp = engine.upload_rz(c["model"].pack(), prec)
+    fl = Flight(1.0, 2000.0, gas=c["gas"], max_steps=1000)
+    b = EnsembleBatch(engine, fp, fl, [(e["ions"], e["tune"])
+                                       for e in c["ensembles"]])
+    b.fly(8, graph=True)
+    ratios, dif = [], []
+    for k, e in enumerate(c["ensembles"]):
+        g = b.read(k)
+        assert np.all(g["steps"] == 8000)
+        T = float(np.mean(g["t_us"]))
+        ratios.append(np.mean(g["x"] - 5.0) / T /
+                      (e["K_mm2_per_V_us"] * e["E_vpmm"]))
+        dif.append(np.var(g["x"]) / (2 * e["D"] * T))
+    b.release()
+    fp.release()
+    ratios, dif = np.array(ratios), np.array(dif)
+    assert np.all(np.abs(ratios - 1.0) < 0.01), ratios
+    assert abs(np.average(dif, weights=ns) - 1.0) < 0.01, dif
+    band = np.maximum(0.01, 4.0 * np.sqrt(2.0 / ns))
+    assert np.all(np.abs(dif - 1.0) < band), (dif, band)
