@@ -198,9 +198,12 @@ typedef struct {
                           before scattering them together (1 = at once;
                           0 = engine default); results do not depend on
                           it */
-  int32_t occupancy;   /* register-budget variant of the hard-sphere
-                          kernel: waves per SIMD 1, 2 or 3 (0 = engine
-                          default); results do not depend on it */
+  int32_t occupancy;   /* register-budget variant of the flight
+                          kernel: at least 1, 2 or 3 waves per SIMD, or 4
+                          for the planar FP32-basis vacuum kernel (others
+                          take 3); 0 = the engine's measured default per
+                          precision, route and gas; results do not depend
+                          on it */
   int32_t block;       /* lanes per workgroup: 64, 128, 192 or 256
                           (0 = engine default 256); results do not depend
                           on it */

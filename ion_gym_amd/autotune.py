@@ -5,9 +5,10 @@ workload is how the lanes are launched:
 
 * ``block``: lanes per workgroup (64, 128, 192, 256).  Smaller blocks let
   the dispatcher fill the 256 CUs more evenly when ions retire unevenly.
-* ``occupancy``: the register-budget variant of the hard-sphere kernels
-  (at least 1, 2 or 3 waves per SIMD; above 1 the compiler keeps the
-  rest in scratch).  Vacuum and SDS kernels have one variant.
+* ``occupancy``: the register-budget variant of the flight kernels (at
+  least 1, 2 or 3 waves per SIMD, and 4 for the planar FP32-basis vacuum
+  step; a budget below the kernel's own need keeps the rest in scratch).
+  The 3-D SDS kernel has one variant.
 * ``coll_batch``: how many parked lanes a 64-lane wave gathers before it
   scatters their hard-sphere collisions together (FP64 and FP32-storage
   modes; the FP32-basis mode tries partners one per loop pass and ignores
@@ -48,10 +49,16 @@ def default_cache_path():
                      "launch.json"))
 
 
-def candidates(precision, gas_model):
-    """The launch settings worth measuring for a precision and gas."""
+def candidates(precision, gas_model, route=None):
+    """The launch settings worth measuring for a precision, gas and
+    route."""
     hs = gas_model == L.GAS_HS
-    occs = OCCUPANCIES if hs else (1,)
+    occs = OCCUPANCIES
+    if route == L.ROUTE_XYZ and gas_model == L.GAS_SDS:
+        occs = (1,)
+    if (route == L.ROUTE_PLANAR and precision == "f32"
+            and gas_model == L.GAS_VACUUM):
+        occs = OCCUPANCIES + (4,)
     cbs = COLL_BATCHES if hs else (0,)
     return [dict(block=b, occupancy=o, coll_batch=c)
             for b, o, c in itertools.product(BLOCKS, occs, cbs)]
@@ -129,7 +136,7 @@ def autotune(engine, fields, flight, ions, *, sample=131072, window=200,
     sub = _sample(ions, sample)
     gas = int(flight.gas.get("model", L.GAS_VACUUM))
     table = []
-    for cand in candidates(fields.precision, gas):
+    for cand in candidates(fields.precision, gas, fields.route):
         t0 = time.perf_counter()
         ms = measure(engine, fields, flight, sub, cand, window=window,
                      reps=reps, load=load)

@@ -60,24 +60,41 @@ namespace {
 thread_local std::string g_err;
 
 // parked hard-sphere lanes a wave scatters together when the caller leaves
-// coll_batch at 0 (the autotuner measures the best value per workload)
-constexpr int kDefaultCollBatch = 16;
+// coll_batch at 0 (the autotuner measures the best value per workload);
+// measured on the funnel at 2 waves/SIMD: 193 ms per window at 16, 184 at
+// 24, 187 at 32, 220 at 48
+constexpr int kDefaultCollBatch = 24;
 // FP32-basis hard sphere: parked lanes a wave tries partners for together
 // (measured on MI355X: the funnel r-z flight runs 17 % faster at 8 than at
 // 1; the 11-set SLIM 3-D flight, whose step is longer, is best at 1)
 constexpr int kDefaultCollBatchFlat = 8;
 constexpr int kDefaultCollBatchFlat3D = 1;
-// default register-budget variant of the flight kernels per precision,
-// route and gas (measured: the FP32 r-z and planar kernels on the funnel are
-// best at 3 waves/SIMD; the FP32 3-D kernel, whose step keeps its cell's
-// corner sums, at 2 on the SLIM board (3 spills inside the step); FP64 and
-// FP32-storage at the compiler's own budget)
+// default register-budget variant (waves per SIMD) of the flight kernels per
+// precision, route and gas, as measured on MI355X (1M-ion windows):
+// * r-z and planar, FP32 basis: 3 (funnel HS 35.7 ms against 48.7 at 4;
+//   vacuum 27.8 against 34.4 at 1; SDS drift tube 14.8 against 17.3 at
+//   1), 4 for the planar vacuum step (27.5 against 30.6 at 3);
+// * r-z and planar, FP64 and FP32 storage: hard sphere 2 (funnel 184 ms
+//   against 308 at 1 and 327 at 3), vacuum 3 (41.0 against 48.3 at 1),
+//   SDS 1 (38.7 against 71.2 at 3);
+// * 3-D: the FP32 kernel, whose step keeps its cell's corner sums, at 2
+//   with gas on the SLIM board (3 spills inside the step) and at the
+//   compiler's own budget in vacuum (222 VGPRs whatever the budget:
+//   quadrupole 6.8e9 at 1 against 5.5e9 at 2); FP64 and FP32 storage
+//   with hard sphere at 2 (quadrupole 67.7 ms against 72.2 at 1 and 97.5
+//   at 3), in vacuum at the compiler's own budget (67.9 against 85.3 at 3).
+// The planar hard-sphere kernels were measured too (FP32 42.4 ms at 3
+// against 52.7 at 1; FP64 386 at 2 against 620 at 1 and 436 at 3); the
+// planar SDS kernels follow the r-z ones.
 inline int default_occupancy(int prec, int route, int gas) {
-  if (prec != IG_PREC_F32) return 1;
-  if (route != IG_ROUTE_XYZ) return 3;
-  // the 3-D vacuum kernel (222 VGPRs whatever the budget) is fastest with
-  // the compiler's own schedule: quadrupole 6.8e9 at 1 against 5.5e9 at 2
-  return gas == IG_GAS_VACUUM ? 1 : 2;
+  if (route == IG_ROUTE_XYZ) {
+    if (prec != IG_PREC_F32) return gas == IG_GAS_HS ? 2 : 1;
+    return gas == IG_GAS_VACUUM ? 1 : 2;
+  }
+  if (prec == IG_PREC_F32)
+    return route == IG_ROUTE_PLANAR && gas == IG_GAS_VACUUM ? 4 : 3;
+  if (gas == IG_GAS_HS) return 2;
+  return gas == IG_GAS_VACUUM ? 3 : 1;
 }
 
 int fail(int code, const std::string &msg) {
@@ -921,8 +938,8 @@ static int build_flight(const ig_flight_params *p, FlightK &k) {
   if (p->coll_batch < 0 || p->coll_batch > 64)
     return fail(IG_E_ARG, "ig_fly: coll_batch must be in [0, 64]");
   k.coll_batch = p->coll_batch; // 0: the precision's default, set per flight
-  if (p->occupancy < 0 || p->occupancy > 3)
-    return fail(IG_E_ARG, "ig_fly: occupancy must be 0 (default), 1, 2 or 3");
+  if (p->occupancy < 0 || p->occupancy > 4)
+    return fail(IG_E_ARG, "ig_fly: occupancy must be 0 (default), 1, 2, 3 or 4");
   k.occupancy = p->occupancy;
   if (p->block != 0 && (p->block < 64 || p->block > 256 || p->block % 64 != 0))
     return fail(IG_E_ARG, "ig_fly: block must be 0 (default), 64, 128, 192 or 256");

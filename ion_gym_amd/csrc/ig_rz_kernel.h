@@ -670,10 +670,10 @@ static void launch_rz_gas(const FieldsK &f, const FlightK &p, const EnsK &e, dim
   if (p.rec_rows > 0 && e.rec) {
     hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, true, 1>), grid, dim3(block), 0, st, f, p,
                        e);
-  } else if (GAS == IG_GAS_HS && p.occupancy == 2) {
+  } else if (p.occupancy == 2) {
     hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, false, 2>), grid, dim3(block), 0, st, f, p,
                        e);
-  } else if (GAS == IG_GAS_HS && p.occupancy == 3) {
+  } else if (p.occupancy >= 3) {
     hipLaunchKernelGGL((rz_fly_kernel<PREC, NS, GAS, false, 3>), grid, dim3(block), 0, st, f, p,
                        e);
   } else {

@@ -979,7 +979,7 @@ static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int6
     if (p.occupancy == 2)
       hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 2>), grid, dim3(block), lds, st, f,
                          p, e);
-    else if (p.occupancy == 3)
+    else if (p.occupancy >= 3)
       hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_HS, 3>), grid, dim3(block), lds, st, f,
                          p, e);
     else
@@ -989,7 +989,7 @@ static int launch_xyz_ns(const FieldsK &f, const FlightK &p, const EnsK &e, int6
     if (p.occupancy == 2)
       hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 2>), grid, dim3(block), lds,
                          st, f, p, e);
-    else if (p.occupancy == 3)
+    else if (p.occupancy >= 3)
       hipLaunchKernelGGL((xyz_fly_kernel<PREC, NS, IG_GAS_VACUUM, 3>), grid, dim3(block), lds,
                          st, f, p, e);
     else

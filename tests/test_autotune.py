@@ -61,8 +61,17 @@ def test_candidates_follow_the_kernel_variants():
     hs32 = AT.candidates("f32", L.GAS_HS)
     assert {c["coll_batch"] for c in hs32} == set(AT.COLL_BATCHES)
     vac = AT.candidates("f32", L.GAS_VACUUM)
-    assert {c["occupancy"] for c in vac} == {1}
+    assert {c["occupancy"] for c in vac} == {1, 2, 3}
+    assert {c["coll_batch"] for c in vac} == {0}
     assert {c["block"] for c in vac} == set(AT.BLOCKS)
+    # the planar FP32-basis vacuum step has a fourth, 128-VGPR variant; the
+    # 3-D SDS kernel has one
+    pl = AT.candidates("f32", L.GAS_VACUUM, L.ROUTE_PLANAR)
+    assert {c["occupancy"] for c in pl} == {1, 2, 3, 4}
+    assert {c["occupancy"] for c in AT.candidates("f64", L.GAS_VACUUM,
+                                                  L.ROUTE_PLANAR)} == {1, 2, 3}
+    assert {c["occupancy"] for c in AT.candidates("f32", L.GAS_SDS,
+                                                  L.ROUTE_XYZ)} == {1}
 
 
 def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
@@ -78,7 +87,7 @@ def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
                     window=50, cache_path=path, device=("stub", 7))
     assert (r["block"], r["occupancy"], r["coll_batch"]) == (128, 2, 16)
     assert r["ms"] == 1.0 and not r["cached"]
-    assert len(r["table"]) == len(AT.candidates("f64", L.GAS_HS))
+    assert len(r["table"]) == len(AT.candidates("f64", L.GAS_HS, 0))
     # every measured window is the requested one, over the sample
     assert {f[3] for f in eng.flights} == {50}
     # the flight passed in is not modified

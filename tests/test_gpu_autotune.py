@@ -71,7 +71,7 @@ def test_autotuner_measures_and_caches(engine, tmp_path):
     path = str(tmp_path / "launch.json")
     r = AT.autotune(engine, fp, fl, c["ions"], sample=20000, window=100,
                     reps=1, cache_path=path)
-    cands = AT.candidates("f32", fl.gas["model"])
+    cands = AT.candidates("f32", fl.gas["model"], fp.route)
     assert {k: r[k] for k in ("block", "occupancy", "coll_batch")} in cands
     assert len(r["table"]) == len(cands)
     assert all(row["ms"] > 0 for row in r["table"])
@@ -81,3 +81,36 @@ def test_autotuner_measures_and_caches(engine, tmp_path):
     g = _fly(engine, fp, AT.tuned(fl, again), c["ions"], 300)
     assert np.all(g["steps"] == 300)
     fp.release()
+
+
+def test_vacuum_sds_and_planar_results_do_not_depend_on_launch_settings(
+        engine):
+    """Every register-budget variant of the vacuum and SDS kernels (r-z
+    einzel lens, r-z SDS drift tube, FP64 and FP32 basis) and the planar
+    guide's FP32 vacuum kernel at 1-4 waves per SIMD."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    cases = []
+    e = configs.einzel_rz(n_ions=3000, coarse=2)
+    cases.append(("einzel", e["model"].pack(), e, None, engine.upload_rz,
+                  (1, 2, 3)))
+    d = configs.drift_rz(n_ions=3000, coarse=2)
+    dpk = dict(d["model"].pack())
+    dpk["EzA"] = dpk["EzA"] * d["scale"]
+    dpk["EuA"] = dpk["EuA"] * d["scale"]
+    cases.append(("drift", dpk, d, d["gas"], engine.upload_rz, (1, 2, 3)))
+    q = configs.planar_quad(n_ions=3000, coarse=2)
+    cases.append(("planar", q["pack"], q, None, engine.upload_planar,
+                  (1, 2, 3, 4)))
+    for name, pk, c, gas, upload, occs in cases:
+        for prec in ("f64", "f32"):
+            print(f"launch invariance {name} {prec}", flush=True)
+            fp = upload(pk, prec)
+            base = Flight(c["dt"], c["t_max"], gas=gas)
+            outs = [_fly(engine, fp, base.launch(occupancy=o, block=b),
+                         c["ions"], 800)
+                    for o, b in zip(occs, (256, 64, 192, 128))]
+            for o in outs[1:]:
+                for k in KEYS:
+                    assert np.array_equal(o[k], outs[0][k]), (name, prec, k)
+            fp.release()
