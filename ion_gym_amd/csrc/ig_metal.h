@@ -15,6 +15,25 @@ __device__ __forceinline__ bool metal_nn(const uint8_t *__restrict__ m, double g
   return m[(int64_t)(int32_t)fi * ny + (int32_t)fj] != 0;
 }
 
+// metal_nn split in two so the load can be issued early: probe() loads
+// the nearest node's byte with no branch (node 0 when the point is
+// outside the grid, so the load stays in bounds); hit() gives metal_nn's
+// answer when the value is used
+struct NodeProbe {
+  uint32_t v;
+  bool inside;
+  __device__ __forceinline__ bool hit() const { return inside && v != 0; }
+};
+__device__ __forceinline__ NodeProbe metal_nn_probe(const uint8_t *__restrict__ m, double gx,
+                                                    double gy, int64_t nx, int64_t ny) {
+  const double fi = floor(gx + 0.5), fj = floor(gy + 0.5);
+  NodeProbe r;
+  r.inside = fi >= 0.0 && fj >= 0.0 && fi <= (double)(nx - 1) && fj <= (double)(ny - 1);
+  const int32_t ci = (int32_t)(r.inside ? fi : 0.0), cj = (int32_t)(r.inside ? fj : 0.0);
+  r.v = m[(int64_t)ci * ny + cj];
+  return r;
+}
+
 // tracer3d._in_metal_3d (tracer3d.py:174-189)
 __device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, double x, double y,
                                             double z, int64_t nx, int64_t ny, int64_t nz) {

@@ -151,11 +151,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       cgy = gy;
     }
     if (!(tt == wtt)) {
+      // an untuned flight reads the pack's waveforms through the kernel
+      // argument, one address for the whole wave: scalar loads, where a
+      // tune's row is a per-lane address and a vector load per parameter
+      auto wave_at = [&](const Waves *Wp, int k) {
+        return PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(Wp, f, k, tt)
+                                   : (CT)wave_eval<false>(Wp, f, k, tt);
+      };
+      if (e.tunes) {
 #pragma unroll
-      for (int k = 0; k < NS - 1; ++k)
-        if (k < K)
-          wk[k] = PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(W, f, k, tt)
-                                      : (CT)wave_eval<false>(W, f, k, tt);
+        for (int k = 0; k < NS - 1; ++k)
+          if (k < K) wk[k] = wave_at(W, k);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NS - 1; ++k)
+          if (k < K) wk[k] = wave_at(f.waves, k);
+      }
       wtt = tt;
     }
     CT ex = cz[0], ey = cr[0];
@@ -194,6 +205,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   const int64_t lim = p.max_steps;
   int stage = kStep;
   double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, to = t;
+  NodeProbe metal_next{0u, false};
   for (;;) {
     if (stage == kStep) {
       if (!(t < t_max)) {
@@ -224,6 +236,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         x += vx * dt;
         y += vy * dt;
         z += vz * dt;
+        // the post-step metal test's node load, issued beside the field
+        // gathers of the same point (without an SDS jump the ion does not
+        // move before that test)
+        if constexpr (GAS != IG_GAS_SDS)
+          metal_next = metal_nn_probe(f.metal, node(x), node(y), nx, ny);
         efield(x, y, tob + t + dt, ex, ey);
         fxr = ex;
         fyr = ey;
@@ -294,7 +311,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     }
     if (stage == kPost) {
       stage = kStep;
-      if (metal_nn(f.metal, node(x), node(y), nx, ny)) {
+      const bool hit =
+          GAS != IG_GAS_SDS ? metal_next.hit() : metal_nn(f.metal, node(x), node(y), nx, ny);
+      if (hit) {
         double f0 = 0.0, f1 = 1.0;
         for (int b = 0; b < 40; ++b) {
           const double fm = 0.5 * (f0 + f1);

@@ -314,11 +314,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       cgu = gu;
     }
     if (!(tt == wtt)) {
+      // an untuned flight reads the pack's waveforms through the kernel
+      // argument, one address for the whole wave: scalar loads, where a
+      // tune's row is a per-lane address and a vector load per parameter
+      auto wave_at = [&](const Waves *Wp, int k) {
+        return PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(Wp, f, k, tt)
+                                   : (CT)wave_eval<false>(Wp, f, k, tt);
+      };
+      if (e.tunes) {
 #pragma unroll
-      for (int k = 0; k < NS - 1; ++k)
-        if (k < K)
-          wk[k] = PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(W, f, k, tt)
-                                      : (CT)wave_eval<false>(W, f, k, tt);
+        for (int k = 0; k < NS - 1; ++k)
+          if (k < K) wk[k] = wave_at(W, k);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NS - 1; ++k)
+          if (k < K) wk[k] = wave_at(f.waves, k);
+      }
       wtt = tt;
     }
     CT zz = cz[0], rz = cr[0];
@@ -378,6 +389,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // same lab time reuses the field and the kick
   bool kfresh = false;
   double ky_last = 0.0, kz_last = 0.0;
+  NodeProbe metal_next{0u, false};
   const double half_dt = 0.5 * dt; // the FP32-basis kicks: one FMA each
   // the step's start point, for the impact bisection and the crossings
   double xo = x, yo = y, zo = z, vxo = vx, vyo = vy, vzo = vz, to = t;
@@ -435,6 +447,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         r = radius(y, z);
         rcur = r;
         rvalid = true;
+        // the post-step metal test reads the nearest node of this same
+        // point; issued beside the field gathers, its load overlaps theirs
+        // (without an SDS jump the ion does not move before that test)
+        if constexpr (GAS != IG_GAS_SDS)
+          metal_next = metal_nn_probe(f.metal, node(x), node(r - u0), nx, nu);
         efield(x, r, tob + t + dt, ez, er);
         {
           const double rr = r > 1e-9 ? r : 1e-9;
@@ -537,7 +554,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         }
       }
       const double r = rvalid ? rcur : radius(y, z);
-      if (metal_nn(f.metal, node(x), node(r - u0), nx, nu)) {
+      const bool hit =
+          GAS != IG_GAS_SDS ? metal_next.hit() : metal_nn(f.metal, node(x), node(r - u0), nx, nu);
+      if (hit) {
         double f0 = 0.0, f1 = 1.0;
         for (int b = 0; b < 40; ++b) {
           double fm = 0.5 * (f0 + f1);
