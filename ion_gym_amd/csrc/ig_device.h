@@ -358,6 +358,15 @@ __device__ __forceinline__ float fast_unit(int kind, double om, double ph, doubl
   return sin_pi_range(r);
 }
 
+// The tune row every active lane of the wave flies (its index read from
+// the first active lane, so the row's address derives from the table's
+// with a wave-uniform offset and loads through it are scalar), or -1 when
+// the lanes fly different rows.
+__device__ __forceinline__ int32_t wave_shared_row(int32_t row) {
+  const int32_t row0 = __builtin_amdgcn_readfirstlane(row);
+  return __all(row == row0) ? row0 : -1;
+}
+
 // FP32-basis waveform: sin, cos and square at duty 0.5 by fast_unit; the
 // rest as in FP64
 template <bool AFFINE>

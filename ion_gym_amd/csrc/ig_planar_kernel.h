@@ -92,6 +92,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     rkey = T->rng_key;
     tscale = T->scale;
   }
+  // drive parameters every lane of the wave shares are read through one
+  // address (scalar loads): the pack's waveforms when the flight is
+  // untuned, the tune row when the wave's lanes all fly the same row
+  // (rows that differ take a vector load per parameter)
+  const int32_t row0 = e.tunes ? wave_shared_row(e.tune[i]) : -1;
+  const Tune *const Tu = row0 >= 0 ? e.tunes + row0 : nullptr;
+  const Waves *const Wu = e.tunes ? (row0 >= 0 ? &e.tunes[row0].w : nullptr) : f.waves;
   const int64_t nx = f.n0, ny = f.n1;
   const int K = f.K;
   int64_t ncol = e.ncol[i];
@@ -141,31 +148,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     if (!(gx == cgx && gy == cgy)) {
       rz_gather<PREC, NS>(f, gx, gy, cz, cr);
       if (tscale) {
+        if (Tu) {
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-          cz[q] = cz[q] * (CT)tscale[q];
-          cr[q] = cr[q] * (CT)tscale[q];
+          for (int q = 0; q < NS; ++q) {
+            cz[q] = cz[q] * (CT)Tu->scale[q];
+            cr[q] = cr[q] * (CT)Tu->scale[q];
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            cz[q] = cz[q] * (CT)tscale[q];
+            cr[q] = cr[q] * (CT)tscale[q];
+          }
         }
       }
       cgx = gx;
       cgy = gy;
     }
     if (!(tt == wtt)) {
-      // an untuned flight reads the pack's waveforms through the kernel
-      // argument, one address for the whole wave: scalar loads, where a
-      // tune's row is a per-lane address and a vector load per parameter
       auto wave_at = [&](const Waves *Wp, int k) {
         return PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(Wp, f, k, tt)
                                    : (CT)wave_eval<false>(Wp, f, k, tt);
       };
-      if (e.tunes) {
+      if (Wu) {
 #pragma unroll
         for (int k = 0; k < NS - 1; ++k)
-          if (k < K) wk[k] = wave_at(W, k);
+          if (k < K) wk[k] = wave_at(Wu, k);
       } else {
 #pragma unroll
         for (int k = 0; k < NS - 1; ++k)
-          if (k < K) wk[k] = wave_at(f.waves, k);
+          if (k < K) wk[k] = wave_at(W, k);
       }
       wtt = tt;
     }
