@@ -56,6 +56,10 @@ struct FieldsK {
   int32_t route;
   int32_t prec;
   int64_t n0, n1, n2;
+  // the node counts and their last two indices as doubles, kernel
+  // arguments so the kernels compare FP64 coordinates with them straight
+  // from scalar registers
+  double dn[3], dnm1[3], dnm2[3];
   double h, inv_h, u0;
   int32_t K;        // drive channels
   int32_t has_gate; // r-z gate set present (set K+1)

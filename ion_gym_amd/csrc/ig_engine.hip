@@ -375,6 +375,14 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   k.n0 = d->n0;
   k.n1 = d->n1;
   k.n2 = d->n2;
+  {
+    const int64_t nn[3] = {k.n0, k.n1, k.n2};
+    for (int a = 0; a < 3; ++a) {
+      k.dn[a] = (double)nn[a];
+      k.dnm1[a] = (double)(nn[a] - 1);
+      k.dnm2[a] = (double)(nn[a] - 2);
+    }
+  }
   k.h = d->h_mm;
   k.inv_h = 1.0 / d->h_mm;
   k.u0 = d->u0;

@@ -24,11 +24,13 @@ struct NodeProbe {
   bool inside;
   __device__ __forceinline__ bool hit() const { return inside && v != 0; }
 };
+// (last_i, last_j: the last node indices nx - 1, ny - 1 as doubles)
 __device__ __forceinline__ NodeProbe metal_nn_probe(const uint8_t *__restrict__ m, double gx,
-                                                    double gy, int64_t nx, int64_t ny) {
+                                                    double gy, double last_i, double last_j,
+                                                    int64_t ny) {
   const double fi = floor(gx + 0.5), fj = floor(gy + 0.5);
   NodeProbe r;
-  r.inside = fi >= 0.0 && fj >= 0.0 && fi <= (double)(nx - 1) && fj <= (double)(ny - 1);
+  r.inside = fi >= 0.0 && fj >= 0.0 && fi <= last_i && fj <= last_j;
   const int32_t ci = (int32_t)(r.inside ? fi : 0.0), cj = (int32_t)(r.inside ? fj : 0.0);
   r.v = m[(int64_t)ci * ny + cj];
   return r;

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         // gathers of the same point (without an SDS jump the ion does not
         // move before that test)
         if constexpr (GAS != IG_GAS_SDS)
-          metal_next = metal_nn_probe(f.metal, node(x), node(y), nx, ny);
+          metal_next = metal_nn_probe(f.metal, node(x), node(y), f.dnm1[0], f.dnm1[1], ny);
         efield(x, y, tob + t + dt, ex, ey);
         fxr = ex;
         fyr = ey;
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         t = to + f1 * dt;
         fate = IG_FATE_IMPACT;
         stage = kDone;
-      } else if (x < 0 || y < 0 || node(x) > (double)(nx - 1) || node(y) > (double)(ny - 1)) {
+      } else if (x < 0 || y < 0 || node(x) > f.dnm1[0] || node(y) > f.dnm1[1]) {
         fate = IG_FATE_BOX;
         stage = kDone;
       } else {

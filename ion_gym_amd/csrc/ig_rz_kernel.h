@@ -80,8 +80,8 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
   const int64_t nx = f.n0, nu = f.n1;
   // the pre-clamp keeps the conversion defined and gives int(g) clamped as
   // the reference clamps it; within [-1, n] a 32-bit index is exact
-  int32_t i = (int32_t)fmin(fmax(gx, -1.0), (double)nx);
-  int32_t j = (int32_t)fmin(fmax(gu, -1.0), (double)nu);
+  int32_t i = (int32_t)fmin(fmax(gx, -1.0), f.dn[0]);
+  int32_t j = (int32_t)fmin(fmax(gu, -1.0), f.dn[1]);
   if (i < 0) i = 0;
   if (j < 0) j = 0;
   if (i > nx - 2) i = (int32_t)(nx - 2);
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         // point; issued beside the field gathers, its load overlaps theirs
         // (without an SDS jump the ion does not move before that test)
         if constexpr (GAS != IG_GAS_SDS)
-          metal_next = metal_nn_probe(f.metal, node(x), node(r - u0), nx, nu);
+          metal_next = metal_nn_probe(f.metal, node(x), node(r - u0), f.dnm1[0], f.dnm1[1], nu);
         efield(x, r, tob + t + dt, ez, er);
         {
           const double rr = r > 1e-9 ? r : 1e-9;
@@ -594,11 +594,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         if (f.label) {
           double re = radius(y, z);
           double li = floor(node(x) + 0.5), lj = floor(node(re - u0) + 0.5);
-          if (li >= 0.0 && lj >= 0.0 && li <= (double)(nx - 1) && lj <= (double)(nu - 1))
+          if (li >= 0.0 && lj >= 0.0 && li <= f.dnm1[0] && lj <= f.dnm1[1])
             label = f.label[(int64_t)li * nu + (int64_t)lj];
         }
         stage = kDone;
-      } else if (x < 0 || node(r) > (double)(nu - 2)) {
+      } else if (x < 0 || node(r) > f.dnm2[1]) {
         fate = IG_FATE_BOX;
         stage = kDone;
       } else {
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
             }
           }
         }
-        if (stage != kDone && node(x) > (double)(nx - 1)) {
+        if (stage != kDone && node(x) > f.dnm1[0]) {
           fate = IG_FATE_BOX;
           stage = kDone;
         }
