@@ -365,10 +365,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         else
           v = twf[q].a * (double)fast_unit(mode, twf[q].om, twf[q].ph, tt) + twf[q].b;
       } else {
-        v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+        // untuned (no tune row): the pack's waveforms through the kernel
+        // argument, one address for the wave (scalar loads)
+        v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(f.waves, f, q - 1, tt) : 0.0);
       }
       v = v * f.scale[q];
-      if (!twf && tscale) v = v * tscale[q];
       o[q] = (float)v;
     }
   };
@@ -414,9 +415,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   // FP64 modes: the waveform weights, kept while the lab time repeats
   auto weights = [&](double tt) {
     if (!(tt == wtt)) {
+      // an untuned flight reads the pack's waveforms through the kernel
+      // argument (scalar loads); a tune row is a per-lane address
+      if (e.tunes) {
 #pragma unroll
-      for (int k = 0; k < NW; ++k)
-        if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
+        for (int k = 0; k < NW; ++k)
+          if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+          if (k < K) w[k] = wave_eval<true>(f.waves, f, k, tt);
+      }
       wtt = tt;
     }
   };
@@ -815,18 +824,31 @@ __global__ __launch_bounds__(256) void xyz_sds_kernel(const FieldsK f, const Fli
   auto efield = [&](double px, double py, double pz, double tt, double &a, double &b,
                     double &c) {
     if (!(tt == wtt)) {
+      // an untuned flight reads the pack's waveforms through the kernel
+      // argument (scalar loads); a tune row is a per-lane address
+      const bool tuned = e.tunes != nullptr;
       if constexpr (PREC == IG_PREC_F32) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-          double v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(W, f, q - 1, tt) : 0.0);
+          double v = q == 0 ? 1.0 : 0.0;
+          if (q > 0 && q <= K) {
+            if (tuned)
+              v = wave_eval_fast<true>(W, f, q - 1, tt);
+            else
+              v = wave_eval_fast<true>(f.waves, f, q - 1, tt);
+          }
           v = v * f.scale[q];
           if (tscale) v = v * tscale[q];
           wt[q] = (float)v;
         }
-      } else {
+      } else if (tuned) {
 #pragma unroll
         for (int k = 0; k < NW; ++k)
           if (k < K) w[k] = wave_eval<true>(W, f, k, tt);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+          if (k < K) w[k] = wave_eval<true>(f.waves, f, k, tt);
       }
       wtt = tt;
     }
