@@ -508,8 +508,18 @@ __device__ __forceinline__ bool hs_hazard_step(Rng &rng, float &hz, double vx, d
     rng_block(rng, w);
     hz = exp1_of(w[0]);
   }
-  const float wx = (float)(vx - p.vg[0]), wy = (float)(vy - p.vg[1]),
-              wz = (float)(vz - p.vg[2]);
+  // in still gas (a uniform test) v - 0 is v: the same values without the
+  // three FP64 subtractions
+  float wx, wy, wz;
+  if (p.vg[0] == 0.0 && p.vg[1] == 0.0 && p.vg[2] == 0.0) {
+    wx = (float)vx;
+    wy = (float)vy;
+    wz = (float)vz;
+  } else {
+    wx = (float)(vx - p.vg[0]);
+    wy = (float)(vy - p.vg[1]);
+    wz = (float)(vz - p.vg[2]);
+  }
   float sp = __builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz);
   if (sp < 1e-7f) sp = 1e-7f;
   hz -= kconst * c_rel_ratio_f(sp * inv_c_star);

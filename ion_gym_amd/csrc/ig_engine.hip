@@ -358,6 +358,9 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   if (d->n0 < 2 || d->n1 < 2 || (two_d ? d->n2 != 1 : d->n2 < 2))
     return fail(IG_E_ARG, "ig_fields_upload: grid shape must be at least 2 nodes per axis "
                           "(n2 == 1 for the r-z and planar routes)");
+  if (d->n0 >= INT32_MAX || d->n1 >= INT32_MAX || d->n2 >= INT32_MAX)
+    return fail(IG_E_ARG, "ig_fields_upload: a grid axis of 2^31 or more nodes (the kernels "
+                          "index nodes in 32 bits)");
   if (d->n_chan < 0 || d->n_chan > IG_MAX_CHANNELS)
     return fail(IG_E_ARG, "ig_fields_upload: n_chan out of [0, " +
                               std::to_string(IG_MAX_CHANNELS) + "]");

@@ -82,10 +82,12 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
   // the reference clamps it; within [-1, n] a 32-bit index is exact
   int32_t i = (int32_t)fmin(fmax(gx, -1.0), f.dn[0]);
   int32_t j = (int32_t)fmin(fmax(gu, -1.0), f.dn[1]);
+  // (node counts are below 2^31, checked at upload: 32-bit compares)
+  const int32_t imax = (int32_t)(nx - 2), jmax = (int32_t)(nu - 2);
   if (i < 0) i = 0;
   if (j < 0) j = 0;
-  if (i > nx - 2) i = (int32_t)(nx - 2);
-  if (j > nu - 2) j = (int32_t)(nu - 2);
+  if (i > imax) i = imax;
+  if (j > jmax) j = jmax;
   double fx = gx - (double)i, fu = gu - (double)j;
   if (fx < 0.0) fx = 0.0;
   if (fx > 1.0) fx = 1.0;
@@ -333,33 +335,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         return PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(Wp, f, k, tt)
                                    : (CT)wave_eval<false>(Wp, f, k, tt);
       };
+      const int nch = f.has_gate ? NS - 2 : NS - 1; // K, per the pack's set count
       if (Wu) {
 #pragma unroll
         for (int k = 0; k < NS - 1; ++k)
-          if (k < K) wk[k] = wave_at(Wu, k);
+          if (k < nch) wk[k] = wave_at(Wu, k);
       } else {
 #pragma unroll
         for (int k = 0; k < NS - 1; ++k)
-          if (k < K) wk[k] = wave_at(W, k);
+          if (k < nch) wk[k] = wave_at(W, k);
       }
       wtt = tt;
     }
     CT zz = cz[0], rz = cr[0];
-#pragma unroll
-    for (int k = 0; k < NS - 1; ++k) {
-      if (k < K) {
-        if constexpr (PREC == IG_PREC_F32) {
-          zz = __builtin_fmaf(wk[k], cz[1 + k], zz);
-          rz = __builtin_fmaf(wk[k], cr[1 + k], rz);
-        } else {
-          zz = zz + wk[k] * cz[1 + k];
-          rz = rz + wk[k] * cr[1 + k];
-        }
+    auto add_channel = [&](int k) {
+      if constexpr (PREC == IG_PREC_F32) {
+        zz = __builtin_fmaf(wk[k], cz[1 + k], zz);
+        rz = __builtin_fmaf(wk[k], cr[1 + k], rz);
+      } else {
+        zz = zz + wk[k] * cz[1 + k];
+        rz = rz + wk[k] * cr[1 + k];
       }
-    }
-    if (gate && tt >= f.tau_gate) {
-      zz = zz + cz[NS - 1];
-      rz = rz + cr[NS - 1];
+    };
+    // the pack's sets are 1 + K + has_gate: without a gate every set after
+    // the static one is a drive channel (a uniform branch, so the channel
+    // count is known in each arm)
+    if (!f.has_gate) {
+#pragma unroll
+      for (int k = 0; k < NS - 1; ++k) add_channel(k);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NS - 2; ++k) add_channel(k);
+      if (gate && tt >= f.tau_gate) {
+        zz = zz + cz[NS - 1];
+        rz = rz + cr[NS - 1];
+      }
     }
     if constexpr (PREC == IG_PREC_F32) {
       ez_o = (double)zz * acc9;
