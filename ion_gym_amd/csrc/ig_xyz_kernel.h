@@ -375,15 +375,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   };
   // the weights of a step's three lab times, each tuned set's record read
   // once for all three (the same arithmetic as three wt_at calls)
-  auto wt_at3 = [&](double t0, double t1, double t2, float(&o0)[NS], float(&o1)[NS],
-                    float(&o2)[NS]) {
-    if (!twf) {
-      wt_at(t0, o0);
-      wt_at(t1, o1);
-      wt_at(t2, o2);
-      return;
-    }
-    const uint4 mw = *reinterpret_cast<const uint4 *>(tun->wmode);
+  // the tune row every lane of the wave flies, if they share one: its
+  // records then come through scalar loads (the row index is read from
+  // the first lane, so the address is wave-uniform)
+  const int32_t row0 = e.tunes ? wave_shared_row(e.tune[i]) : -1;
+  const Tune *const tunU = row0 >= 0 ? e.tunes + row0 : nullptr;
+  auto wt_rows3 = [&](const Tune *T, double t0, double t1, double t2, float(&o0)[NS],
+                      float(&o1)[NS], float(&o2)[NS]) {
+    const WaveF *Twf = T->wf;
+    const uint4 mw = *reinterpret_cast<const uint4 *>(T->wmode);
     const uint32_t mws[4] = {mw.x, mw.y, mw.z, mw.w};
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -392,9 +392,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       double v[3];
       WaveF w{};
       if (q == 0)
-        w.a = twf[0].a;
+        w.a = Twf[0].a;
       else if (q <= K)
-        w = twf[q];
+        w = Twf[q];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         if (q == 0)
@@ -410,6 +410,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       o0[q] = (float)v[0];
       o1[q] = (float)v[1];
       o2[q] = (float)v[2];
+    }
+  };
+  auto wt_at3 = [&](double t0, double t1, double t2, float(&o0)[NS], float(&o1)[NS],
+                    float(&o2)[NS]) {
+    if (!twf) {
+      wt_at(t0, o0);
+      wt_at(t1, o1);
+      wt_at(t2, o2);
+    } else if (tunU) {
+      wt_rows3(tunU, t0, t1, t2, o0, o1, o2);
+    } else {
+      wt_rows3(tun, t0, t1, t2, o0, o1, o2);
     }
   };
   // FP64 modes: the waveform weights, kept while the lab time repeats
