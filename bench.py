@@ -66,6 +66,27 @@ CLOCK_HZ = 2.4e9            # MI355X peak engine clock
 REF_NS_PER_ION_STEP = (87.0, 101.0)
 
 
+# Design choices behind the numbers (how batching, launch tuning and the
+# roofline are done), stated on the benchmark line itself.
+DESIGN_NOTES = {
+    "batching": "one launch per window over every live ion of every "
+                "ensemble (a per-ion tune row), consecutive windows "
+                "replayed from a captured hipGraph; no persistent kernel: "
+                "a lane already runs its ion's whole window in registers, "
+                "so a persistent grid would add only a work queue",
+    "autotuner": "searches block size, register budget (waves per SIMD) "
+                 "and the hard-sphere collision batch; ions per lane stay "
+                 "at one (the lane state fills the register budget) and "
+                 "the r-z route has no LDS tile (its 16 MB pack is L2/MALL "
+                 "resident and the bounds are the gather rate and vector "
+                 "issue); the 3-D FP32 step keeps its cell's corner sums "
+                 "in LDS",
+    "roofline": "the r-z pack never leaves L2/MALL, so the kernel is "
+                "priced against the measured gather rate of its own access "
+                "pattern (probe_gather.hip) with HBM traffic beside it",
+}
+
+
 def gather_bytes_per_ion_step(n_sets, value_bytes):
     """Algorithmic gather bytes of one r-z ion-step of the reference's
     kick-drift-kick step (tracer_rz.py:207-440): two bilinear field
@@ -876,6 +897,7 @@ def main():
         "precision": precision,
         "solver": solver,
         "configs": others,
+        "design_notes": DESIGN_NOTES,
     }
     print(json.dumps(line), flush=True)
     eng.close()
