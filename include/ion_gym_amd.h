@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define IG_ABI_VERSION 1
+#define IG_ABI_VERSION 2
 
 /* status codes */
 #define IG_OK 0
@@ -162,7 +162,26 @@ typedef struct {
   const double *tab_v;
   const int64_t *tab_off; /* (n_chan+1) slice offsets into tab_t/tab_v */
   int64_t n_tab;
+  /* IG_FOLD_LINKED: for an IG_PREC_F32 pack, store drive channels whose
+     waveforms are linked -- the same kind (sin, cos, or square at duty
+     0.5), the same frequency, phases a whole number of half periods
+     apart -- as one set.  w(x + n pi) = (-1)^n w(x) for these waveforms,
+     so sum_k (a_k w(om t + ph_k) + b_k) E_k = w(om t + ph_L) sum_k
+     (-1)^n_k a_k E_k + sum_k b_k E_k: one set per linked group (its grid
+     combined in FP64 before the FP32 rounding) plus one constant set when
+     an offset b_k is nonzero.  Fewer sets mean fewer gathered bytes and
+     interpolation FMAs per step; the field equals the unfolded one to the
+     FP32 rounding.  Applied only where it lowers the set count; never to
+     packs with table waveforms.  Tunes flown on a folded pack must keep
+     each group linked with the pack's amplitude ratios (ig_fly refuses
+     others).  A folded pack cannot be composed into (ig_fields_compose);
+     ig_fields_layout reports its stored sets.  0 (IG_FOLD_NONE) keeps
+     every set as given. */
+  int32_t fold_channels;
 } ig_fields_desc;
+
+#define IG_FOLD_NONE 0
+#define IG_FOLD_LINKED 1
 
 /* Flight parameters shared by every ion of one ig_fly call.  Replaces the
  * scalar/array arguments of _fly_rec_full (ion_gym/physics/tracer_rz.py:
@@ -336,6 +355,12 @@ int ig_fields_compose(ig_engine *h, ig_bases_id b, ig_fields_id f, int32_t set,
  * as doubles (stored value x the set's storage scale): (n0, n1, n2).
  * No counterpart in the reference (inspection of device packs). */
 int ig_fields_read_set(ig_engine *h, ig_fields_id f, int32_t set, int32_t comp, double *out);
+/* The stored layout of a resident pack: its field sets (static, drive
+ * channels, gate; after IG_FOLD_LINKED folding), its drive channels, the
+ * values per node record, and the drive channels it was given.  No
+ * counterpart in the reference (inspection of device packs). */
+int ig_fields_layout(ig_engine *h, ig_fields_id f, int32_t *n_sets, int32_t *n_chan,
+                     int32_t *stride, int32_t *n_chan_given);
 
 /* --- device-resident ensembles ------------------------------------------
  * Ion state kept in HBM across calls so that long flights, throughput

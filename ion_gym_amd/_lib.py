@@ -15,6 +15,7 @@ IG_OK = 0
 ROUTE_RZ, ROUTE_XYZ, ROUTE_PLANAR = 0, 1, 2
 PREC_F64, PREC_F32_STORE, PREC_F32 = 0, 1, 2
 PRECISIONS = {"f64": PREC_F64, "f32_store": PREC_F32_STORE, "f32": PREC_F32}
+FOLD_NONE, FOLD_LINKED = 0, 1
 WAVE_SIN, WAVE_COS, WAVE_SQUARE, WAVE_TAB_HOLD, WAVE_TAB_LINEAR = range(5)
 GAS_VACUUM, GAS_HS, GAS_SDS = 0, 1, 2
 FATE_FLYING = -2
@@ -40,7 +41,7 @@ class FieldsDesc(C.Structure):
                 ("ch_kind", _i32p), ("ch_om", _dp), ("ch_ph", _dp),
                 ("ch_duty", _dp), ("ch_amp", _dp), ("ch_off", _dp),
                 ("tab_t", _dp), ("tab_v", _dp), ("tab_off", _i64p),
-                ("n_tab", C.c_int64)]
+                ("n_tab", C.c_int64), ("fold_channels", C.c_int32)]
 
 
 class BasesDesc(C.Structure):
@@ -201,6 +202,9 @@ SIGNATURES = {
     "ig_bases_release": (C.c_int, [_VP, C.c_int64]),
     "ig_fields_compose": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int32,
                                     C.c_int32, _i32p, _dp]),
+    "ig_fields_layout": (C.c_int, [_VP, C.c_int64, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
     "ig_fields_read_set": (C.c_int, [_VP, C.c_int64, C.c_int32, C.c_int32,
                                      _dp]),
     "ig_last_fly_ms": (C.c_int, [_VP, C.POINTER(C.c_double)]),

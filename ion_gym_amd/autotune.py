@@ -68,7 +68,8 @@ def cache_key(device_name, cu, fields, flight):
     gas = int(flight.gas.get("model", L.GAS_VACUUM))
     rec = int(flight.rec_rows > 0)
     return (f"{device_name}|{cu}|route{fields.route}|{fields.precision}|"
-            f"sets{1 + fields.n_chan}|gas{gas}|rec{rec}")
+            f"sets{getattr(fields, 'n_sets', 1 + fields.n_chan)}|gas{gas}|"
+            f"rec{rec}")
 
 
 def _load_cache(path):

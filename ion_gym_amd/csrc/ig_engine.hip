@@ -135,10 +135,35 @@ int dev_alloc(DevBuf &b, size_t bytes) {
   return 0;
 }
 
+// ---- linked drive channels (IG_FOLD_LINKED) -----------------------------
+// Given channel k weighs its grid E_k by a_k w(om_k t + ph_k) + b_k (3-D;
+// the r-z and planar routes fly unit waveforms, a_k = 1 and b_k = 0, the
+// amplitude sits in the grid).  Channels of one kind (sin, cos, or square
+// at duty 0.5) and one frequency whose phases lie a whole number n_k of
+// half periods from the first such channel's (the group's leader L) obey
+// w(x + n pi) = (-1)^n w(x): the group is stored as one set
+// sum_k (-1)^n_k (a_k / a_L) E_k flown with the leader's waveform and
+// amplitude, and the groups' offsets as one constant set sum_k b_k E_k of
+// weight 1.  Other channels are stored as given.
+struct FoldPlan {
+  bool on = false;
+  bool affine = false;          // 3-D weights a w + b, else unit waveforms
+  int K0 = 0, K1 = 0;           // drive channels given, stored
+  std::vector<int> dest;        // (K0) stored channel of each given one
+  std::vector<double> coef;     // (K0) its grid's coefficient there
+  std::vector<int> sign;        // (K0) (-1)^n_k against its leader
+  std::vector<char> linked;     // (K0) member of a folded group
+  std::vector<double> amp, off; // (K0) the pack's a_k, b_k
+  std::vector<int> leader;      // (K1) given channel a stored one flies,
+                                // -1 for the constant set
+  int const_ch = -1;            // stored index of the constant set
+};
+
 struct Fields {
   FieldsK k{};
   DevBuf grid, metal, label, tab_t, tab_v, waves;
   Waves waves_host{};
+  FoldPlan fold;
   int64_t bytes = 0;
 };
 
@@ -168,6 +193,10 @@ struct Ensemble {
   int32_t rec_rows = 0, rec_cols = 0;
   int32_t n_tunes = 0;
   DevBuf tunes;         // Tune table (device)
+  std::vector<ig_tune> tunes_in; // the table as given (host)
+  DevBuf tunes_fold;    // the table re-laid for one folded pack (device)
+  int64_t fold_for = 0; // the pack tunes_fold holds the table for (0: none)
+  std::vector<Tune> fold_host; // staging of tunes_fold's last copy
   DevBuf counts;        // live_by_tune scratch (device)
   DevBuf sums;          // live/steps reduction (device)
   DevBuf live_idx;      // compacted live-ion list (int32, capacity)
@@ -347,6 +376,201 @@ int ig_sync(ig_engine *h) {
   return 0;
 }
 
+// a waveform w with w(x + pi) = -w(x)
+static bool linkable(int kind, double duty) {
+  return kind == IG_WAVE_SIN || kind == IG_WAVE_COS || (kind == IG_WAVE_SQUARE && duty == 0.5);
+}
+
+// dph = n pi for a whole n (to 1e-9 of a half period)
+static bool half_periods(double dph, int &n) {
+  if (!std::isfinite(dph)) return false;
+  const double q = dph / 3.141592653589793;
+  if (!(std::fabs(q) < 1.0e9)) return false;
+  const double r = std::nearbyint(q);
+  if (std::fabs(q - r) > 1.0e-9) return false;
+  n = (int)(int64_t)r;
+  return true;
+}
+
+static bool close_rel(double x, double y) {
+  return std::fabs(x - y) <= 1.0e-9 * std::max(std::fabs(x), std::fabs(y));
+}
+
+// whether a flight kernel is instantiated for n_sets field sets on the
+// route in the FP32-basis mode (the launchers' set-count switches)
+static bool f32_kernel_for(int route, int n_sets) {
+  if (route == IG_ROUTE_RZ) return n_sets >= 1 && n_sets <= 4;
+  if (route == IG_ROUTE_PLANAR) return n_sets >= 1 && n_sets <= 5;
+  return n_sets == 1 || n_sets == 2 || n_sets == 3 || n_sets == 7 || n_sets == 11;
+}
+
+// The fold of a pack's drive channels, or an inactive plan where folding is
+// not asked for, not allowed (FP64 arithmetic, table waveforms), would not
+// lower the set count or would give a set count no kernel is built for.
+static FoldPlan make_fold_plan(const ig_fields_desc *d, bool affine) {
+  FoldPlan P;
+  const int K = d->n_chan;
+  P.affine = affine;
+  P.K0 = K;
+  if (d->fold_channels != IG_FOLD_LINKED || d->precision != IG_PREC_F32 || K < 2) return P;
+  for (int k = 0; k < K; ++k)
+    if (d->ch_kind[k] == IG_WAVE_TAB_HOLD || d->ch_kind[k] == IG_WAVE_TAB_LINEAR) return P;
+  P.amp.assign(K, 1.0);
+  P.off.assign(K, 0.0);
+  if (affine)
+    for (int k = 0; k < K; ++k) {
+      P.amp[k] = d->ch_amp ? d->ch_amp[k] : 1.0;
+      P.off[k] = d->ch_off ? d->ch_off[k] : 0.0;
+    }
+  std::vector<int> lead(K);
+  P.sign.assign(K, 1);
+  for (int k = 0; k < K; ++k) {
+    lead[k] = k;
+    const bool ok = linkable(d->ch_kind[k], d->ch_duty[k]) && std::isfinite(d->ch_om[k]) &&
+                    std::isfinite(P.amp[k]) && P.amp[k] != 0.0;
+    if (!ok) continue;
+    for (int L = 0; L < k; ++L) {
+      int n = 0;
+      if (lead[L] != L || !linkable(d->ch_kind[L], d->ch_duty[L]) || P.amp[L] == 0.0 ||
+          !std::isfinite(P.amp[L]) || d->ch_kind[L] != d->ch_kind[k] ||
+          d->ch_om[L] != d->ch_om[k] || !half_periods(d->ch_ph[k] - d->ch_ph[L], n))
+        continue;
+      lead[k] = L;
+      P.sign[k] = (n & 1) ? -1 : 1;
+      break;
+    }
+  }
+  std::vector<int> members(K, 0);
+  for (int k = 0; k < K; ++k) members[lead[k]] += 1;
+  P.linked.assign(K, 0);
+  bool any_off = false;
+  for (int k = 0; k < K; ++k) {
+    P.linked[k] = members[lead[k]] >= 2;
+    any_off = any_off || (P.linked[k] && P.off[k] != 0.0);
+  }
+  int K1 = any_off ? 1 : 0;
+  for (int k = 0; k < K; ++k) K1 += lead[k] == k;
+  const int gate = d->route == IG_ROUTE_RZ && d->has_gate ? 1 : 0;
+  if (K1 >= K || !f32_kernel_for(d->route, 1 + K1 + gate)) return P;
+  P.dest.assign(K, -1);
+  P.coef.assign(K, 1.0);
+  for (int k = 0; k < K; ++k) {
+    if (lead[k] == k) {
+      P.dest[k] = (int)P.leader.size();
+      P.leader.push_back(k);
+    } else {
+      P.dest[k] = P.dest[lead[k]];
+    }
+    if (P.linked[k]) P.coef[k] = P.sign[k] * (P.amp[k] / P.amp[lead[k]]);
+  }
+  if (any_off) {
+    P.const_ch = (int)P.leader.size();
+    P.leader.push_back(-1);
+  }
+  P.K1 = (int)P.leader.size();
+  P.on = true;
+  return P;
+}
+
+// A tune's drive re-laid onto a folded pack's stored channels.  The tune
+// must keep every folded group linked as the pack has it: the same kind
+// and frequency, phases the same parity of half periods apart, effective
+// amplitudes (amplitude x set scale; the set scale alone on the unit-
+// waveform routes) in the pack's ratios, and offsets one common multiple
+// of the pack's.  Returns an empty string, or what breaks the fold.
+static std::string fold_tune(const FoldPlan &P, const ig_tune &a, ig_tune &o) {
+  std::memset(&o, 0, sizeof(o));
+  o.dt = a.dt;
+  o.t_max = a.t_max;
+  o.rng_key = a.rng_key;
+  for (int s = 0; s < IG_MAX_CHANNELS + 2; ++s) o.set_scale[s] = 1.0;
+  for (int c = 0; c < IG_MAX_CHANNELS; ++c) {
+    o.ch_kind[c] = IG_WAVE_SIN;
+    o.ch_duty[c] = 0.5;
+    o.ch_amp[c] = 1.0;
+  }
+  o.set_scale[0] = a.set_scale[0];
+  o.set_scale[1 + P.K1] = a.set_scale[1 + P.K0]; // the r-z gate set
+  auto alpha = [&](int k) { return P.affine ? a.ch_amp[k] * a.set_scale[1 + k] : a.set_scale[1 + k]; };
+  auto beta = [&](int k) { return P.affine ? a.ch_off[k] * a.set_scale[1 + k] : 0.0; };
+  // a channel whose weight is identically zero (no amplitude, or sin of a
+  // constant zero phase) and no offset: a group of only such channels
+  // (an idle tune row, say) stays zero whatever its phases
+  auto zero_drive = [&](int k) {
+    const bool still = a.ch_kind[k] == IG_WAVE_SIN && a.ch_om[k] == 0.0 && a.ch_ph[k] == 0.0;
+    return (alpha(k) == 0.0 || still) && beta(k) == 0.0;
+  };
+  std::vector<char> zero_group(P.K1, 1);
+  for (int k = 0; k < P.K0; ++k)
+    if (P.linked[k] && !zero_drive(k)) zero_group[P.dest[k]] = 0;
+  for (int j = 0; j < P.K1; ++j) {
+    const int L = P.leader[j];
+    if (L < 0) continue;
+    if (P.linked[L] && zero_group[j]) { // sin(0) = 0: weight 0
+      o.ch_amp[j] = 0.0;
+      continue;
+    }
+    o.ch_kind[j] = a.ch_kind[L];
+    o.ch_om[j] = a.ch_om[L];
+    o.ch_ph[j] = a.ch_ph[L];
+    o.ch_duty[j] = a.ch_duty[L];
+    o.ch_amp[j] = a.ch_amp[L];
+    o.ch_off[j] = P.linked[L] ? 0.0 : a.ch_off[L];
+    o.set_scale[1 + j] = a.set_scale[1 + L];
+  }
+  double mu = 0.0;
+  bool mu_set = false;
+  for (int k = 0; k < P.K0; ++k) {
+    if (!P.linked[k]) continue;
+    const int L = P.leader[P.dest[k]];
+    const std::string ch = "channel " + std::to_string(k) + " ";
+    if (zero_group[P.dest[k]]) {
+      if (P.off[k] != 0.0 && P.const_ch >= 0) { // its offset share is 0
+        if (!mu_set) {
+          mu = 0.0;
+          mu_set = true;
+        } else if (mu != 0.0) {
+          return ch + "scales its offset unlike the other linked channels";
+        }
+      }
+      continue;
+    }
+    if (!linkable(a.ch_kind[k], a.ch_duty[k]) || a.ch_kind[k] != a.ch_kind[L] ||
+        a.ch_duty[k] != a.ch_duty[L])
+      return ch + "changes its waveform kind away from channel " + std::to_string(L) + "'s";
+    if (a.ch_om[k] != a.ch_om[L])
+      return ch + "changes its frequency away from channel " + std::to_string(L) + "'s";
+    int n = 0;
+    if (!half_periods(a.ch_ph[k] - a.ch_ph[L], n) || ((n & 1) ? -1 : 1) != P.sign[k])
+      return ch + "changes its phase against channel " + std::to_string(L) + "'s";
+    if (!close_rel(alpha(k) * P.amp[L], alpha(L) * P.amp[k]))
+      return ch + "changes its amplitude against channel " + std::to_string(L) + "'s";
+    const double b = beta(k);
+    if (P.off[k] == 0.0) {
+      if (b != 0.0) return ch + "gains an offset the pack does not have";
+    } else {
+      const double m = b / P.off[k];
+      if (!mu_set) {
+        mu = m;
+        mu_set = true;
+      } else if (!close_rel(m, mu)) {
+        return ch + "scales its offset unlike the other linked channels";
+      }
+    }
+  }
+  if (P.const_ch >= 0) {
+    const int j = P.const_ch;
+    o.ch_kind[j] = IG_WAVE_SIN; // sin(0) = 0: the weight is the offset
+    o.ch_om[j] = 0.0;
+    o.ch_ph[j] = 0.0;
+    o.ch_duty[j] = 0.5;
+    o.ch_amp[j] = 0.0;
+    o.ch_off[j] = mu;
+    o.set_scale[1 + j] = 1.0;
+  }
+  return std::string();
+}
+
 int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   if (!h || !d || !out) return fail(IG_E_ARG, "ig_fields_upload: null argument");
   if (d->route != IG_ROUTE_RZ && d->route != IG_ROUTE_XYZ && d->route != IG_ROUTE_PLANAR)
@@ -389,35 +613,64 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   k.h = d->h_mm;
   k.inv_h = 1.0 / d->h_mm;
   k.u0 = d->u0;
-  k.K = d->n_chan;
   k.has_gate = rz && d->has_gate ? 1 : 0;
   k.tau_gate = k.has_gate ? d->tau_gate : -1.0;
+  const int n_sets_in = 1 + d->n_chan + k.has_gate;
+  for (int s = 0; s < n_sets_in; ++s) {
+    double sc = d->chan_scale ? d->chan_scale[s] : 1.0;
+    if (!(sc > 0.0) || !std::isfinite(sc))
+      return fail(IG_E_ARG, "ig_fields_upload: chan_scale must be finite and > 0");
+  }
+  for (int c = 0; c < d->n_chan; ++c) {
+    int kind = d->ch_kind[c];
+    if (kind < IG_WAVE_SIN || kind > IG_WAVE_TAB_LINEAR)
+      return fail(IG_E_ARG, "ig_fields_upload: unknown waveform kind " + std::to_string(kind));
+  }
+  if (d->fold_channels != IG_FOLD_NONE && d->fold_channels != IG_FOLD_LINKED)
+    return fail(IG_E_ARG, "ig_fields_upload: fold_channels must be IG_FOLD_NONE or IG_FOLD_LINKED");
+  // the stored sets: as given, or with linked drive channels folded
+  F->fold = make_fold_plan(d, !two_d);
+  const FoldPlan &P = F->fold;
+  k.K = P.on ? P.K1 : d->n_chan;
   k.n_sets = 1 + k.K + k.has_gate;
   k.n_comp = two_d ? 2 : 3;
   const int nvals = k.n_sets * k.n_comp;
   k.stride = record_stride(k.prec, nvals);
+  // given set of each stored set (-1: a folded drive set), and the stored
+  // storage scale: the leader's for a folded group, 1 for the constant set
+  std::vector<int> given((size_t)k.n_sets, -1);
+  given[0] = 0;
+  for (int j = 0; j < k.K; ++j) given[1 + j] = P.on ? -1 : 1 + j;
+  if (k.has_gate) given[1 + k.K] = 1 + d->n_chan;
   for (int s = 0; s < k.n_sets; ++s) {
-    double sc = d->chan_scale ? d->chan_scale[s] : 1.0;
-    if (!(sc > 0.0) || !std::isfinite(sc))
-      return fail(IG_E_ARG, "ig_fields_upload: chan_scale must be finite and > 0");
+    int src = given[s];
+    if (src < 0) src = P.leader[s - 1] >= 0 ? 1 + P.leader[s - 1] : -1;
+    const double sc = src >= 0 && d->chan_scale ? d->chan_scale[src] : 1.0;
     k.scale[s] = k.prec == IG_PREC_F64 ? 1.0 : sc;
   }
   int64_t tab_total = 0;
   Waves &wv = F->waves_host;
   std::memset(&wv, 0, sizeof(wv));
   for (int c = 0; c < k.K; ++c) {
-    int kind = d->ch_kind[c];
-    if (kind < IG_WAVE_SIN || kind > IG_WAVE_TAB_LINEAR)
-      return fail(IG_E_ARG, "ig_fields_upload: unknown waveform kind " + std::to_string(kind));
-    wv.kind[c] = kind;
-    wv.om[c] = d->ch_om[c];
-    wv.ph[c] = d->ch_ph[c];
-    wv.duty[c] = d->ch_duty[c];
-    wv.amp[c] = d->ch_amp ? d->ch_amp[c] : 1.0;
-    wv.off[c] = d->ch_off ? d->ch_off[c] : 0.0;
+    const int L = P.on ? P.leader[c] : c;
+    if (L < 0) { // the constant set: sin(0 t + 0) = 0, weight 0 * 0 + 1
+      wv.kind[c] = IG_WAVE_SIN;
+      wv.duty[c] = 0.5;
+      wv.amp[c] = 0.0;
+      wv.off[c] = 1.0;
+      continue;
+    }
+    wv.kind[c] = d->ch_kind[L];
+    wv.om[c] = d->ch_om[L];
+    wv.ph[c] = d->ch_ph[L];
+    wv.duty[c] = d->ch_duty[L];
+    wv.amp[c] = d->ch_amp ? d->ch_amp[L] : 1.0;
+    wv.off[c] = d->ch_off ? d->ch_off[L] : 0.0;
+    if (P.on && P.linked[L]) wv.off[c] = 0.0; // carried by the constant set
   }
-  for (int c = 0; c <= k.K; ++c) wv.tab_off[c] = d->tab_off ? d->tab_off[c] : 0;
-  if (d->tab_off) {
+  // (a folded pack has no table waveforms)
+  for (int c = 0; c <= k.K; ++c) wv.tab_off[c] = d->tab_off && !P.on ? d->tab_off[c] : 0;
+  if (d->tab_off && !P.on) {
     tab_total = d->n_tab;
     for (int c = 0; c < k.K; ++c)
       if (wv.tab_off[c] < 0 || wv.tab_off[c + 1] < wv.tab_off[c] || wv.tab_off[c + 1] > tab_total)
@@ -438,10 +691,31 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
     h32.assign((size_t)(nodes * S), 0.0f);
   else
     h64.assign((size_t)(nodes * S), 0.0);
+  std::vector<double> comb;
   for (int s = 0; s < k.n_sets; ++s) {
-    const double inv = 1.0 / (d->chan_scale ? d->chan_scale[s] : 1.0);
+    const double inv = 1.0 / (k.prec == IG_PREC_F64 ? 1.0 : k.scale[s]);
     for (int c = 0; c < k.n_comp; ++c) {
-      const double *src = d->grids + ((int64_t)s * k.n_comp + c) * nodes;
+      const double *src;
+      if (given[s] >= 0) {
+        src = d->grids + ((int64_t)given[s] * k.n_comp + c) * nodes;
+      } else {
+        // a folded set, summed in FP64 in the given channels' order
+        const int j = s - 1;
+        comb.assign((size_t)nodes, 0.0);
+        for (int g = 0; g < P.K0; ++g) {
+          double w;
+          if (P.leader[j] >= 0) {
+            if (P.dest[g] != j) continue;
+            w = P.coef[g];
+          } else {
+            if (!P.linked[g] || P.off[g] == 0.0) continue;
+            w = P.off[g];
+          }
+          const double *e = d->grids + ((int64_t)(1 + g) * k.n_comp + c) * nodes;
+          for (int64_t n = 0; n < nodes; ++n) comb[n] += w * e[n];
+        }
+        src = comb.data();
+      }
       const int q = s * k.n_comp + c;
       if (f32) {
         for (int64_t n = 0; n < nodes; ++n) h32[n * S + q] = (float)(src[n] * inv);
@@ -722,6 +996,39 @@ int ig_ensemble_read_at(ig_engine *h, ig_ensemble_id id, int64_t offset, int64_t
   return read_range(h, E, offset, n, o);
 }
 
+// an ig_tune as the kernels read it
+static int tune_from(const ig_tune &a, Tune &b, const std::string &who) {
+  std::memset(&b, 0, sizeof(b));
+  if (!(a.dt > 0.0) || !std::isfinite(a.dt)) return fail(IG_E_ARG, who + " dt <= 0");
+  b.dt = a.dt;
+  b.t_max = a.t_max;
+  b.rng_key = a.rng_key;
+  for (int s2 = 0; s2 < IG_MAX_CHANNELS + 2; ++s2) b.scale[s2] = a.set_scale[s2];
+  for (int c = 0; c < IG_MAX_CHANNELS; ++c) {
+    if (a.ch_kind[c] < IG_WAVE_SIN || a.ch_kind[c] > IG_WAVE_SQUARE)
+      return fail(IG_E_ARG, who + " channel " + std::to_string(c) +
+                                ": tunes carry sin, cos or square drives (tables stay with the "
+                                "field pack)");
+    b.w.kind[c] = a.ch_kind[c];
+    b.w.om[c] = a.ch_om[c];
+    b.w.ph[c] = a.ch_ph[c];
+    b.w.duty[c] = a.ch_duty[c];
+    b.w.amp[c] = a.ch_amp[c];
+    b.w.off[c] = a.ch_off[c];
+    WaveF &w = b.wf[c + 1];
+    w.om = a.ch_om[c];
+    w.ph = a.ch_ph[c];
+    w.a = a.ch_amp[c] * a.set_scale[c + 1];
+    w.b = a.ch_off[c] * a.set_scale[c + 1];
+    const uint32_t mode =
+        a.ch_kind[c] == IG_WAVE_SQUARE && a.ch_duty[c] != 0.5 ? kWaveSlow : a.ch_kind[c];
+    b.wmode[(c + 1) >> 3] |= mode << (4 * ((c + 1) & 7));
+  }
+  b.wmode[0] |= (uint32_t)kWaveConst;
+  b.wf[0].a = a.set_scale[0];
+  return 0;
+}
+
 int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_tune *tunes) {
   if (!h || (n > 0 && !tunes)) return fail(IG_E_ARG, "ig_ensemble_set_tunes: null argument");
   Ensemble *E = find_ens(h, id);
@@ -733,45 +1040,17 @@ int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_t
     if (E->tunes.p) (void)hipFree(E->tunes.p);
     E->tunes.p = nullptr;
     E->tunes.bytes = 0;
+    E->tunes_fold.reset();
+    E->tunes_in.clear();
+    E->fold_for = 0;
     E->n_tunes = 0;
     E->k.tunes = nullptr;
     return 0;
   }
   std::vector<Tune> host((size_t)n);
-  for (int32_t t = 0; t < n; ++t) {
-    const ig_tune &a = tunes[t];
-    Tune &b = host[t];
-    std::memset(&b, 0, sizeof(b));
-    if (!(a.dt > 0.0) || !std::isfinite(a.dt))
-      return fail(IG_E_ARG, "ig_ensemble_set_tunes: tune " + std::to_string(t) + " dt <= 0");
-    b.dt = a.dt;
-    b.t_max = a.t_max;
-    b.rng_key = a.rng_key;
-    for (int s2 = 0; s2 < IG_MAX_CHANNELS + 2; ++s2) b.scale[s2] = a.set_scale[s2];
-    for (int c = 0; c < IG_MAX_CHANNELS; ++c) {
-      if (a.ch_kind[c] < IG_WAVE_SIN || a.ch_kind[c] > IG_WAVE_SQUARE)
-        return fail(IG_E_ARG, "ig_ensemble_set_tunes: tune " + std::to_string(t) +
-                                  " channel " + std::to_string(c) +
-                                  ": tunes carry sin, cos or square drives (tables stay with "
-                                  "the field pack)");
-      b.w.kind[c] = a.ch_kind[c];
-      b.w.om[c] = a.ch_om[c];
-      b.w.ph[c] = a.ch_ph[c];
-      b.w.duty[c] = a.ch_duty[c];
-      b.w.amp[c] = a.ch_amp[c];
-      b.w.off[c] = a.ch_off[c];
-      WaveF &w = b.wf[c + 1];
-      w.om = a.ch_om[c];
-      w.ph = a.ch_ph[c];
-      w.a = a.ch_amp[c] * a.set_scale[c + 1];
-      w.b = a.ch_off[c] * a.set_scale[c + 1];
-      const uint32_t mode =
-          a.ch_kind[c] == IG_WAVE_SQUARE && a.ch_duty[c] != 0.5 ? kWaveSlow : a.ch_kind[c];
-      b.wmode[(c + 1) >> 3] |= mode << (4 * ((c + 1) & 7));
-    }
-    b.wmode[0] |= (uint32_t)kWaveConst;
-    b.wf[0].a = a.set_scale[0];
-  }
+  for (int32_t t = 0; t < n; ++t)
+    if (int rc = tune_from(tunes[t], host[t], "ig_ensemble_set_tunes: tune " + std::to_string(t)))
+      return rc;
   // loaded ions must keep a valid tune index
   if (E->n > 0) {
     std::vector<int32_t> idx((size_t)E->n);
@@ -780,13 +1059,53 @@ int ig_ensemble_set_tunes(ig_engine *h, ig_ensemble_id id, int32_t n, const ig_t
       if (v >= n) return fail(IG_E_ARG, "ig_ensemble_set_tunes: a loaded ion uses tune " +
                                             std::to_string(v) + " beyond the new table");
   }
-  DevBuf nb;
+  DevBuf nb, nf;
   if (int rc = dev_alloc(nb, host.size() * sizeof(Tune))) return rc;
+  // room for the table re-laid onto a folded pack's channels, filled when
+  // the ensemble flies on one (so that no flight call allocates for it)
+  if (int rc = dev_alloc(nf, host.size() * sizeof(Tune))) return rc;
   HIPX(hipMemcpy(nb.p, host.data(), host.size() * sizeof(Tune), hipMemcpyHostToDevice));
   std::swap(E->tunes.p, nb.p);
   std::swap(E->tunes.bytes, nb.bytes);
+  std::swap(E->tunes_fold.p, nf.p);
+  std::swap(E->tunes_fold.bytes, nf.bytes);
+  E->tunes_in.assign(tunes, tunes + n);
+  E->fold_for = 0;
   E->n_tunes = n;
   E->k.tunes = (const Tune *)E->tunes.p;
+  return 0;
+}
+
+// The tune table the next flight of E on pack F reads: the table as given,
+// or, on a pack whose linked channels were folded, the table re-laid onto
+// the stored channels (built on the host and copied into the ensemble's
+// second table when the pack differs from the last one it was built for).
+static int bind_tunes(ig_engine *h, Ensemble *E, ig_fields_id fid, const Fields &F,
+                      const std::string &who) {
+  if (E->n_tunes == 0) return 0;
+  if (!F.fold.on) {
+    E->k.tunes = (const Tune *)E->tunes.p;
+    return 0;
+  }
+  if (E->fold_for != fid) {
+    std::vector<Tune> host((size_t)E->n_tunes);
+    for (int32_t t = 0; t < E->n_tunes; ++t) {
+      ig_tune o;
+      const std::string why = fold_tune(F.fold, E->tunes_in[t], o);
+      if (!why.empty())
+        return fail(IG_E_ARG, who + ": tune " + std::to_string(t) + ": " + why +
+                                  ", which the FP32-basis pack folded at upload "
+                                  "(IG_FOLD_LINKED) cannot fly; upload it with IG_FOLD_NONE");
+      if (int rc = tune_from(o, host[t], who + ": tune " + std::to_string(t))) return rc;
+    }
+    // an earlier copy from the staging vector may still be in flight
+    HIPX(hipStreamSynchronize(h->stream));
+    E->fold_host.swap(host);
+    HIPX(hipMemcpyAsync(E->tunes_fold.p, E->fold_host.data(), E->fold_host.size() * sizeof(Tune),
+                        hipMemcpyHostToDevice, h->stream));
+    E->fold_for = fid;
+  }
+  E->k.tunes = (const Tune *)E->tunes_fold.p;
   return 0;
 }
 
@@ -1244,6 +1563,7 @@ static int prepare_flight(ig_engine *h, ig_fields_id fid, const ig_flight_params
     k.sds_stats = na ? (const double *)(na->ptr + kNaSds) : (const double *)h->sds.p;
   }
   fk = &fit->second->k;
+  if (int rc = bind_tunes(h, E, fid, *fit->second, w)) return rc;
   const int32_t clk = (fk->route == IG_ROUTE_XYZ && k.gas != IG_GAS_SDS) ? 1 : 0;
   if (E->clock >= 0 && (E->clock != clk || E->route != fk->route))
     return fail(IG_E_ARG, w + ": the ensemble was flown on another route or clock (3-D RK4 "
@@ -1466,12 +1786,16 @@ int ig_fields_compose(ig_engine *h, ig_bases_id bid, ig_fields_id fid, int32_t s
                       int32_t n_terms, const int32_t *base_index, const double *weight) {
   if (!h || (n_terms > 0 && (!base_index || !weight)))
     return fail(IG_E_ARG, "ig_fields_compose: null argument");
-  auto bit = h->bases.find(bid);
-  if (bit == h->bases.end()) return fail(IG_E_STATE, "ig_fields_compose: unknown bases id");
   auto fit = h->fields.find(fid);
   if (fit == h->fields.end()) return fail(IG_E_STATE, "ig_fields_compose: unknown field id");
-  Bases &B = *bit->second;
   Fields &F = *fit->second;
+  if (F.fold.on)
+    return fail(IG_E_ARG, "ig_fields_compose: the pack's linked drive channels were folded at "
+                          "upload (IG_FOLD_LINKED); upload the pack to compose into with "
+                          "IG_FOLD_NONE");
+  auto bit = h->bases.find(bid);
+  if (bit == h->bases.end()) return fail(IG_E_STATE, "ig_fields_compose: unknown bases id");
+  Bases &B = *bit->second;
   const FieldsK &k = F.k;
   const bool rz = B.route == IG_ROUTE_RZ;
   if (k.route != B.route) return fail(IG_E_ARG, "ig_fields_compose: route differs from the pack's");
@@ -1530,6 +1854,20 @@ int ig_fields_compose(ig_engine *h, ig_bases_id bid, ig_fields_id fid, int32_t s
   HIPX(hipGetLastError());
   // the host term arrays are the caller's: finish the copies before return
   HIPX(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int ig_fields_layout(ig_engine *h, ig_fields_id fid, int32_t *n_sets, int32_t *n_chan,
+                     int32_t *stride, int32_t *n_chan_given) {
+  if (!h || !n_sets || !n_chan || !stride || !n_chan_given)
+    return fail(IG_E_ARG, "ig_fields_layout: null argument");
+  auto fit = h->fields.find(fid);
+  if (fit == h->fields.end()) return fail(IG_E_STATE, "ig_fields_layout: unknown field id");
+  const Fields &F = *fit->second;
+  *n_sets = F.k.n_sets;
+  *n_chan = F.k.K;
+  *stride = F.k.stride;
+  *n_chan_given = F.fold.on ? F.fold.K0 : F.k.K;
   return 0;
 }
 

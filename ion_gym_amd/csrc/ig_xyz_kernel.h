@@ -1045,6 +1045,10 @@ int launch_xyz_prec(const FieldsK &f, const FlightK &p, const EnsK &e, int64_t n
   case 1: return launch_xyz_ns<PREC, 1>(f, p, e, n_launch, block, st);
   case 2: return launch_xyz_ns<PREC, 2>(f, p, e, n_launch, block, st);
   case 3: return launch_xyz_ns<PREC, 3>(f, p, e, n_launch, block, st);
+  case 7:
+    // the 11-set SLIM board with its linked channels folded (FP32 basis)
+    if constexpr (PREC == IG_PREC_F32) return launch_xyz_ns<PREC, 7>(f, p, e, n_launch, block, st);
+    return -2;
   case 11: return launch_xyz_ns<PREC, 11>(f, p, e, n_launch, block, st);
   default: return -2;
   }

@@ -77,6 +77,8 @@ class FieldPack:
         self.precision = precision
         self.shape = shape
         self.n_chan = n_chan
+        # field sets as stored (linked drive channels may be folded)
+        self.n_sets = self.layout()["n_sets"]
 
     @property
     def nbytes(self):
@@ -85,11 +87,31 @@ class FieldPack:
                                         C.byref(out)), "ig_fields_bytes")
         return out.value
 
+    def layout(self):
+        """The stored layout: field sets, drive channels (after folding
+        linked channels), values per node record, channels given."""
+        v = [C.c_int32() for _ in range(4)]
+        L.check(L.lib().ig_fields_layout(self.engine.h, self.id,
+                                         *[C.byref(x) for x in v]),
+                "ig_fields_layout")
+        n_sets, n_chan, stride, given = (x.value for x in v)
+        return dict(n_sets=n_sets, n_chan=n_chan, stride=stride,
+                    n_chan_given=given, folded=n_chan != given)
+
     def release(self):
         if self.id is not None and self.engine.h:
             L.check(L.lib().ig_fields_release(self.engine.h, self.id),
                     "ig_fields_release")
         self.id = None
+
+
+def _fold_mode(precision, fold):
+    """The pack's fold mode: linked drive channels are folded by default in
+    the FP32-basis mode (the only precision that folds); ``fold=False``
+    keeps every set as given (a pack to compose into)."""
+    if fold is None:
+        fold = precision == "f32"
+    return L.FOLD_LINKED if fold else L.FOLD_NONE
 
 
 class Ensemble:
@@ -323,7 +345,7 @@ class Engine:
         L.check(L.lib().ig_sync(self.h), "ig_sync")
 
     # ------------------------------------------------------------ fields
-    def upload_rz(self, pack, precision="f64", chan_scale=None):
+    def upload_rz(self, pack, precision="f64", chan_scale=None, fold=None):
         """Upload an r-z pack in the reference's layout: EzA, EuA
         (nx, nu) V/m; EzK, EuK (K, nx, nu); optional EzG, EuG, tau_gate;
         ele (nx, nu) mirrored metal; optional label (nx, nu); u0; mm;
@@ -360,13 +382,15 @@ class Engine:
         if pack.get("label") is not None:
             d.label = L.ptr(keep(pack["label"], np.uint8), L._u8p)
         self._waves(d, keep, pack, K, affine=False)
+        d.fold_channels = _fold_mode(precision, fold)
         fid = C.c_int64()
         L.check(L.lib().ig_fields_upload(self.h, C.byref(d), C.byref(fid)),
                 "ig_fields_upload")
         return FieldPack(self, fid.value, L.ROUTE_RZ, precision, (nx, nu),
                          K)
 
-    def upload_planar(self, pack, precision="f64", chan_scale=None):
+    def upload_planar(self, pack, precision="f64", chan_scale=None,
+                      fold=None):
         """Upload a planar pack in the reference's layout: ExA, EyA
         (nx, ny) V/m; ExK, EyK (K, nx, ny); ele (nx, ny) metal; mm;
         ch_kind, ch_om, ch_ph, ch_duty; optional tab_t, tab_v, tab_off
@@ -395,13 +419,14 @@ class Engine:
         d.metal = L.ptr(keep(np.asarray(pack["ele"]) > 0.5, np.uint8),
                         L._u8p)
         self._waves(d, keep, pack, K, affine=False)
+        d.fold_channels = _fold_mode(precision, fold)
         fid = C.c_int64()
         L.check(L.lib().ig_fields_upload(self.h, C.byref(d), C.byref(fid)),
                 "ig_fields_upload")
         return FieldPack(self, fid.value, L.ROUTE_PLANAR, precision,
                          (nx, ny), K)
 
-    def upload_xyz(self, pack, precision="f64", chan_scale=None):
+    def upload_xyz(self, pack, precision="f64", chan_scale=None, fold=None):
         """Upload a 3-D pack in the reference's layout: EAx, EAy, EAz
         (nx, ny, nz) V/mm; ExK, EyK, EzK (K, nx, ny, nz); ele (bool); h_mm;
         ch_kind, ch_om, ch_ph, ch_amp, ch_off, ch_duty; optional tab_t,
@@ -431,6 +456,7 @@ class Engine:
         d.metal = L.ptr(keep(np.asarray(pack["ele"]).astype(bool),
                              np.uint8), L._u8p)
         self._waves(d, keep, pack, K, affine=True)
+        d.fold_channels = _fold_mode(precision, fold)
         fid = C.c_int64()
         L.check(L.lib().ig_fields_upload(self.h, C.byref(d), C.byref(fid)),
                 "ig_fields_upload")
@@ -529,7 +555,9 @@ class Engine:
 
     def compose(self, bases, fields, set_index, terms):
         """Set ``set_index`` of ``fields`` := -grad(sum of w * phi_b /
-        base_volts over ``terms`` [(b, w), ...] in order), on the device."""
+        base_volts over ``terms`` [(b, w), ...] in order), on the device.
+        ``fields`` keeps its sets as given: an FP32-basis pack to compose
+        into is uploaded with ``fold=False``."""
         keep = _Keep()
         bi = keep(np.array([t[0] for t in terms], np.int32), np.int32)
         w = keep(np.array([t[1] for t in terms], np.float64))
@@ -538,7 +566,8 @@ class Engine:
             L.ptr(bi, L._i32p), L.ptr(w)), "ig_fields_compose")
 
     def read_set(self, fields, set_index, comp):
-        """One component grid of one set of a resident pack (doubles)."""
+        """One component grid of one stored set of a resident pack
+        (doubles; a folded pack's sets as stored, ``fields.layout()``)."""
         shape = tuple(int(v) for v in fields.shape)
         out = np.empty(shape, np.float64)
         L.check(L.lib().ig_fields_read_set(self.h, fields.id, int(set_index),

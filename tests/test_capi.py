@@ -33,7 +33,7 @@ def test_binding_covers_every_header_symbol():
 
 def test_abi_version_and_error_channel():
     L = _lib.lib()
-    assert L.ig_abi_version() == 1
+    assert L.ig_abi_version() == 2
     # a null argument is refused with a message, without touching a device
     assert L.ig_device_count(None) == -1
     assert b"null" in L.ig_last_error()

@@ -44,7 +44,7 @@ def test_rz_composition_equals_the_host_build(engine, funnel, prec):
     nz, nr = lab.shape
     B = engine.upload_bases_rz(np.stack([bases[k] for k in keys]), lab,
                                np.arange(nz) * h, np.arange(nr) * h)
-    fp = engine.upload_rz(_zero_pack(model), prec)
+    fp = engine.upload_rz(_zero_pack(model), prec, fold=False)
     _compose_all(engine, fp, B, lab, bases, dc, members, groups)
     want = [(model.EzA, model.EuA)] + [(model.EzK[k], model.EuK[k])
                                        for k in range(len(model.EzK))]
@@ -119,7 +119,7 @@ def test_3d_composition_equals_the_host_rule(engine):
                 ch_duty=np.full(1, 0.5))
     B = engine.upload_bases_3d(phis, ele, h)
     for prec in ("f64", "f32"):
-        fp = engine.upload_xyz(pack, prec)
+        fp = engine.upload_xyz(pack, prec, fold=False)
         engine.compose(B, fp, 1, terms)
         for c, w in enumerate((ex, ey, ez)):
             got = engine.read_set(fp, 1, c)
