@@ -474,13 +474,22 @@ __device__ __forceinline__ float hs_rate_const(const FlightK &p, double dt, doub
 // an Exp(1) variate from a word
 __device__ __forceinline__ float exp1_of(uint32_t w) { return -__logf(u24p(w)); }
 
-// c_rel(sp) / c_bar as a function of s = sp / c* (collision3d.py:265-272):
-// for s < 4 the degree-12 polynomial in x = s^2 / 8 - 1 fitted by
-// ion_gym_amd/csrc/fit_c_rel.py, above it (sqrt(pi)/2)(s + 1/(2 s)), whose
-// relative error there is below 3e-11
-__device__ __forceinline__ float c_rel_ratio_f(float s) {
-  if (s >= 4.0f)
-    return 0.886226925452758f * (s + 0.5f * __builtin_amdgcn_rcpf(s));
+// c_rel(sp) / c_bar as a function of s = sp / c* (collision3d.py:265-272),
+// from fits by ion_gym_amd/csrc/fit_c_rel.py: for s < 1 (heavy ions in a
+// light gas nearly always; a wave whose lanes all are takes only this
+// branch) the degree-5 polynomial in y = 2 s^2 - 1
+__device__ __forceinline__ float c_rel_small_f(float s) {
+  const float y = __builtin_fmaf(2.0f * s, s, -1.0f);
+  float q = 1.876325837e-06f;
+  q = __builtin_fmaf(q, y, -3.035061573e-05f);
+  q = __builtin_fmaf(q, y, 4.534179993e-04f);
+  q = __builtin_fmaf(q, y, -6.771401594e-03f);
+  q = __builtin_fmaf(q, y, 1.516326654e-01f);
+  return __builtin_fmaf(q, y, 1.158889719e+00f);
+}
+
+// for 1 <= s < 4 the degree-12 polynomial in x = s^2 / 8 - 1
+__device__ __forceinline__ float c_rel_mid_f(float s) {
   const float x = __builtin_fmaf(s * s, 0.125f, -1.0f);
   float q = -7.843976244e-04f;
   q = __builtin_fmaf(q, x, 1.642305600e-03f);
@@ -495,6 +504,13 @@ __device__ __forceinline__ float c_rel_ratio_f(float s) {
   q = __builtin_fmaf(q, x, -2.546248822e-01f);
   q = __builtin_fmaf(q, x, 1.174990659e+00f);
   return __builtin_fmaf(q, x, 2.663291561e+00f);
+}
+
+// above s = 4, (sqrt(pi)/2)(s + 1/(2 s)), relative error below 3e-11
+__device__ __forceinline__ float c_rel_ratio_f(float s) {
+  if (s < 1.0f) return c_rel_small_f(s);
+  if (s >= 4.0f) return 0.886226925452758f * (s + 0.5f * __builtin_amdgcn_rcpf(s));
+  return c_rel_mid_f(s);
 }
 
 // One step of the hazard clock.  hz is the remaining Exp(1) hazard (NaN:

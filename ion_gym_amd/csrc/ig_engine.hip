@@ -636,6 +636,13 @@ int ig_fields_upload(ig_engine *h, const ig_fields_desc *d, ig_fields_id *out) {
   k.n_comp = two_d ? 2 : 3;
   const int nvals = k.n_sets * k.n_comp;
   k.stride = record_stride(k.prec, nvals);
+  // the FP32-basis 2-D kernels address records and metal nodes with 32-bit
+  // offsets
+  if (two_d && k.prec == IG_PREC_F32 &&
+      ((double)d->n0 * (double)d->n1 >= 2147483648.0 ||
+       (double)d->n0 * (double)d->n1 * (double)k.stride * 4.0 >= 4294967296.0))
+    return fail(IG_E_ARG, "ig_fields_upload: an FP32-basis 2-D pack of 2^31 nodes or 4 GiB of "
+                          "records or more (the kernels' 32-bit offsets)");
   // given set of each stored set (-1: a folded drive set), and the stored
   // storage scale: the leader's for a folded group, 1 for the constant set
   std::vector<int> given((size_t)k.n_sets, -1);

@@ -36,6 +36,33 @@ __device__ __forceinline__ NodeProbe metal_nn_probe(const uint8_t *__restrict__ 
   return r;
 }
 
+// the hardware's FP64 -> int32 conversion, v_cvt_i32_f64: truncation
+// toward zero, saturating at the int32 range (and 0 for NaN), one
+// instruction (C's conversion is undefined out of range, so the compiler
+// would need a clamp first)
+__device__ __forceinline__ int32_t cvt_i32_sat(double v) {
+  int32_t r;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
+// metal_nn_probe in 32-bit integer arithmetic (the FP32-basis kernels):
+// the nearest node's indices by the saturating conversion, inside the grid
+// iff both are at most the last index as unsigned numbers (a negative or
+// saturated index is not), and a 32-bit node offset (2-D packs hold fewer
+// than 2^31 nodes, checked at upload).  The same answer as metal_nn_probe
+// for every finite point.
+__device__ __forceinline__ NodeProbe metal_nn_probe_u32(const uint8_t *__restrict__ m, double gx,
+                                                        double gy, uint32_t last_i,
+                                                        uint32_t last_j, uint32_t ny) {
+  const uint32_t ci = (uint32_t)cvt_i32_sat(floor(gx + 0.5));
+  const uint32_t cj = (uint32_t)cvt_i32_sat(floor(gy + 0.5));
+  NodeProbe r;
+  r.inside = ci <= last_i && cj <= last_j;
+  r.v = m[r.inside ? ci * ny + cj : 0u];
+  return r;
+}
+
 // tracer3d._in_metal_3d (tracer3d.py:174-189)
 __device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, double x, double y,
                                             double z, int64_t nx, int64_t ny, int64_t nz) {

@@ -78,6 +78,42 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
   using T = typename StoreT<PREC>::T;
   using L = RecLayout<T, NS, 2>;
   const int64_t nx = f.n0, nu = f.n1;
+  if constexpr (PREC == IG_PREC_F32) {
+    // int(g) by the saturating hardware conversion, clamped to the
+    // interior in integers (the same indices as the FP64 path's pre-clamp,
+    // conversion and clamp); the fractions converted to FP32, then clamped
+    // to [0, 1] (the same values as clamping in FP64 first: the rounding is
+    // monotone and 0 and 1 are exact); 32-bit byte offsets (an FP32 2-D
+    // pack stays below 4 GiB, checked at upload)
+    const int32_t i = min(max(cvt_i32_sat(gx), 0), (int32_t)(nx - 2));
+    const int32_t j = min(max(cvt_i32_sat(gu), 0), (int32_t)(nu - 2));
+    const float fxf = __builtin_amdgcn_fmed3f((float)(gx - (double)i), 0.0f, 1.0f);
+    const float fuf = __builtin_amdgcn_fmed3f((float)(gu - (double)j), 0.0f, 1.0f);
+    constexpr uint32_t kRec = L::kStride * sizeof(T);
+    const uint32_t o0 = ((uint32_t)i * (uint32_t)nu + (uint32_t)j) * kRec;
+    const uint32_t o1 = o0 + (uint32_t)nu * kRec;
+    const char *g = reinterpret_cast<const char *>(f.grid);
+    T a00[L::kStride], a01[L::kStride], a10[L::kStride], a11[L::kStride];
+    load_vals(reinterpret_cast<const T *>(g + o0), a00);
+    load_vals(reinterpret_cast<const T *>(g + o0 + kRec), a01);
+    load_vals(reinterpret_cast<const T *>(g + o1), a10);
+    load_vals(reinterpret_cast<const T *>(g + o1 + kRec), a11);
+    const float w00 = (1.0f - fxf) * (1.0f - fuf), w10 = fxf * (1.0f - fuf);
+    const float w01 = (1.0f - fxf) * fuf, w11 = fxf * fuf;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      float vz = __builtin_fmaf(a11[2 * s], w11,
+                                __builtin_fmaf(a01[2 * s], w01,
+                                               __builtin_fmaf(a10[2 * s], w10, a00[2 * s] * w00)));
+      float vr = __builtin_fmaf(
+          a11[2 * s + 1], w11,
+          __builtin_fmaf(a01[2 * s + 1], w01,
+                         __builtin_fmaf(a10[2 * s + 1], w10, a00[2 * s + 1] * w00)));
+      cz[s] = (OT)(vz * (float)f.scale[s]);
+      cr[s] = (OT)(vr * (float)f.scale[s]);
+    }
+    return;
+  }
   // the pre-clamp keeps the conversion defined and gives int(g) clamped as
   // the reference clamps it; within [-1, n] a 32-bit index is exact
   int32_t i = (int32_t)fmin(fmax(gx, -1.0), f.dn[0]);
@@ -101,37 +137,19 @@ __device__ __forceinline__ void rz_gather(const FieldsK &f, double gx, double gu
   load_vals(p0 + L::kStride, a01);
   load_vals(p1, a10);
   load_vals(p1 + L::kStride, a11);
-  if constexpr (PREC == IG_PREC_F32) {
-    const float fxf = (float)fx, fuf = (float)fu;
-    const float w00 = (1.0f - fxf) * (1.0f - fuf), w10 = fxf * (1.0f - fuf);
-    const float w01 = (1.0f - fxf) * fuf, w11 = fxf * fuf;
+  const double ax = 1 - fx, au = 1 - fu;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      float vz = __builtin_fmaf(a11[2 * s], w11,
-                                __builtin_fmaf(a01[2 * s], w01,
-                                               __builtin_fmaf(a10[2 * s], w10, a00[2 * s] * w00)));
-      float vr = __builtin_fmaf(
-          a11[2 * s + 1], w11,
-          __builtin_fmaf(a01[2 * s + 1], w01,
-                         __builtin_fmaf(a10[2 * s + 1], w10, a00[2 * s + 1] * w00)));
-      cz[s] = (OT)(vz * (float)f.scale[s]);
-      cr[s] = (OT)(vr * (float)f.scale[s]);
-    }
-  } else {
-    const double ax = 1 - fx, au = 1 - fu;
+  for (int s = 0; s < NS; ++s) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int q = 2 * s + c;
-        double v = (double)a00[q] * ax * au + (double)a10[q] * fx * au +
-                   (double)a01[q] * ax * fu + (double)a11[q] * fx * fu;
-        if constexpr (PREC == IG_PREC_F32_STORE) v = v * f.scale[s];
-        if (c == 0)
-          cz[s] = v;
-        else
-          cr[s] = v;
-      }
+    for (int c = 0; c < 2; ++c) {
+      const int q = 2 * s + c;
+      double v = (double)a00[q] * ax * au + (double)a10[q] * fx * au +
+                 (double)a01[q] * ax * fu + (double)a11[q] * fx * fu;
+      if constexpr (PREC == IG_PREC_F32_STORE) v = v * f.scale[s];
+      if (c == 0)
+        cz[s] = v;
+      else
+        cr[s] = v;
     }
   }
 }
@@ -307,10 +325,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       kz = er_ * z / rr_;
     }
   };
-  auto efield = [&](double px, double pr, double tt, double &ez_o, double &er_o) {
+  // FP32-basis mode: the last evaluation's radial component in FP32 (the
+  // radial kick is taken from it; er is converted only where it is kept)
+  float erf_c = 0.0f;
+  const float acc9f = (float)acc9;
+  // fresh: the point and lab time are known to differ from the cached ones
+  // (the step's second evaluation), so the comparisons are skipped
+  auto efield = [&](double px, double pr, double tt, double &ez_o, double &er_o, bool fresh) {
     const double gx = node(px);
     const double gu = node(pr - u0);
-    if (!(gx == cgx && gu == cgu)) {
+    if (fresh || !(gx == cgx && gu == cgu)) {
       rz_gather<PREC, NS>(f, gx, gu, cz, cr);
       if (tscale) {
         if (Tu) {
@@ -330,7 +354,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       cgx = gx;
       cgu = gu;
     }
-    if (!(tt == wtt)) {
+    if (fresh || !(tt == wtt)) {
       auto wave_at = [&](const Waves *Wp, int k) {
         return PREC == IG_PREC_F32 ? (CT)wave_eval_fast<false>(Wp, f, k, tt)
                                    : (CT)wave_eval<false>(Wp, f, k, tt);
@@ -372,11 +396,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       }
     }
     if constexpr (PREC == IG_PREC_F32) {
-      ez_o = (double)zz * acc9;
-      er_o = (double)rz * acc9;
+      ez_o = (double)(zz * acc9f);
+      erf_c = rz * acc9f;
+      (void)er_o;
     } else {
       ez_o = zz * 1e-9 * acc;
       er_o = rz * 1e-9 * acc;
+    }
+  };
+  // er of the last evaluation, in every mode
+  auto er_last = [&](double er_) { return PREC == IG_PREC_F32 ? (double)erf_c : er_; };
+  // the radial kick components at radius r from the last evaluation: in
+  // FP32 from the FP32 radial field and the hardware reciprocal
+  auto kick_r = [&](double er_, double r_, double &ky, double &kz) {
+    if constexpr (PREC == IG_PREC_F32) {
+      const float q = erf_c * __builtin_amdgcn_rcpf(fmaxf((float)r_, 1e-9f));
+      ky = (double)q * y;
+      kz = (double)q * z;
+    } else {
+      const double rr = r_ > 1e-9 ? r_ : 1e-9;
+      radial(er_, rr, ky, kz);
     }
   };
 
@@ -384,15 +423,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   if (steps0 == 0) {
     // field at the birth point for row 0 (tracer_rz.py:184-202)
     double r0 = radius(y, z);
-    efield(x, r0, tob, ez, er);
+    efield(x, r0, tob, ez, er, false);
+    er = er_last(er);
     if constexpr (REC) {
       rz_write_row(e.rec + (size_t)i * p.rec_rows * p.rec_cols, p.rec_mask, tob, x, y, z, vx, vy,
-                   vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
+                   vz, ez * fscale, er_last(er) * fscale, ncol, m_ion, path, eat, ket);
       nrec = 1;
     }
   } else {
     ez = e.e_last[3 * i];
     er = e.e_last[3 * i + 1];
+    erf_c = (float)er; // (an FP32 value, kept as a double)
   }
 
   int fate = IG_FATE_FLYING, face = -1, label = 0;
@@ -438,9 +479,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
           ky = ky_last;
           kz = kz_last;
         } else {
-          efield(x, r, tob + t, ez, er);
-          double rr = r > 1e-9 ? r : 1e-9;
-          radial(er, rr, ky, kz);
+          efield(x, r, tob + t, ez, er, false);
+          kick_r(er, r, ky, kz);
         }
         if constexpr (GAS == IG_GAS_SDS) {
           vx += 0.5 * sds_fac * (ez - (vx - p.vg[0]) * sds_damp) * dt;
@@ -473,12 +513,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
         // point; issued beside the field gathers, its load overlaps theirs
         // (without an SDS jump the ion does not move before that test)
         if constexpr (GAS != IG_GAS_SDS)
-          metal_next = metal_nn_probe(f.metal, node(x), node(r - u0), f.dnm1[0], f.dnm1[1], nu);
-        efield(x, r, tob + t + dt, ez, er);
-        {
-          const double rr = r > 1e-9 ? r : 1e-9;
-          radial(er, rr, ky, kz);
-        }
+          metal_next = PREC == IG_PREC_F32
+                           ? metal_nn_probe_u32(f.metal, node(x), node(r - u0),
+                                                (uint32_t)(nx - 1), (uint32_t)(nu - 1),
+                                                (uint32_t)nu)
+                           : metal_nn_probe(f.metal, node(x), node(r - u0), f.dnm1[0],
+                                            f.dnm1[1], nu);
+        efield(x, r, tob + t + dt, ez, er, true);
+        kick_r(er, r, ky, kz);
         ky_last = ky;
         kz_last = kz;
         kfresh = true;
@@ -570,7 +612,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       if constexpr (REC) {
         if ((steps0 + s) % p.rec_every == 0 && nrec < p.rec_rows - 1) {
           rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t,
-                       x, y, z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat,
+                       x, y, z, vx, vy, vz, ez * fscale, er_last(er) * fscale, ncol, m_ion, path, eat,
                        ket);
           nrec += 1;
         }
@@ -675,7 +717,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   if constexpr (REC) {
     if (fate != IG_FATE_FLYING) {
       rz_write_row(e.rec + ((size_t)i * p.rec_rows + nrec) * p.rec_cols, p.rec_mask, tob + t, x,
-                   y, z, vx, vy, vz, ez * fscale, er * fscale, ncol, m_ion, path, eat, ket);
+                   y, z, vx, vy, vz, ez * fscale, er_last(er) * fscale, ncol, m_ion, path, eat, ket);
       nrec += 1;
     }
   }
@@ -696,7 +738,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   e.has_gauss[i] = rng.has_gauss;
   e.gauss[i] = rng.gauss;
   e.e_last[3 * i] = ez;
-  e.e_last[3 * i + 1] = er;
+  e.e_last[3 * i + 1] = er_last(er);
   if constexpr (REC) {
     e.path[i] = path;
     e.eat[i] = eat;
