@@ -19,7 +19,7 @@ SOURCES = ("ig_engine.hip", "ig_rz_f64.hip", "ig_rz_f32s.hip", "ig_rz_f32.hip",
            "ig_xyz_f64.hip", "ig_xyz_f32s.hip", "ig_xyz_f32.hip",
            "ig_planar_f64.hip", "ig_planar_f32s.hip", "ig_planar_f32.hip",
            "ig_births.hip", "ig_solve.hip")
-PROBE_SOURCES = ("probe_gather.hip",)
+PROBE_SOURCES = ("probe_gather.hip", "probe_hw_math.hip")
 PROBE_OUT = os.path.join(HERE, "libig_probe.so")
 ARCH = os.environ.get("ION_GYM_AMD_ARCH", "gfx950")
 

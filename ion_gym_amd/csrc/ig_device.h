@@ -312,29 +312,21 @@ __device__ __forceinline__ double wave_eval(const Waves *__restrict__ W, const F
 // probability are computed in FP32 from FP64 inputs.  Their error against
 // the FP64 path is stated by the tests (a few 1e-7 relative per value).
 
-// sin/cos of an FP64 phase: reduced to [-pi, pi] in FP64, evaluated in FP32
-__device__ __forceinline__ float phase_reduce(double arg) {
-  double n = rint(arg * (1.0 / IG_TWO_PI));
-  return (float)(arg - n * IG_TWO_PI);
+// Sines, cosines and logarithms of the FP32 mode use the hardware's FP32
+// instructions: v_sin_f32 and v_cos_f32 take the angle in revolutions
+// (measured on the MI355X over [-1/2, 1) revolutions: absolute error below
+// 1.3e-7), v_log_f32 is log2 (relative error below 1.2e-7; every argument
+// here is a normal number in (0, 1]).
+
+// the phase om t + ph in revolutions, reduced to [-1/2, 1/2] in FP64
+__device__ __forceinline__ float phase_rev(double arg) {
+  const double q = arg * (1.0 / IG_TWO_PI);
+  return (float)(q - rint(q));
 }
 
-// sin(r) for r in [-pi, pi] (as phase_reduce leaves it) in FP32: reflected
-// into [-pi/2, pi/2] by sin(r) = sin(+-pi - r), then the Taylor polynomial
-// through r^11, whose truncation error there is below (pi/2)^13 / 13! =
-// 6e-8; the reflection adds the FP32 rounding of pi (9e-8).
-__device__ __forceinline__ float sin_pi_range(float r) {
-  const float kHalfPi = 1.57079632679489662f, kPi = 3.14159265358979324f;
-  if (r > kHalfPi)
-    r = kPi - r;
-  else if (r < -kHalfPi)
-    r = -kPi - r;
-  const float r2 = r * r;
-  float q = -2.50521083854417188e-08f;             // -1/11!
-  q = __builtin_fmaf(q, r2, 2.75573192239858907e-06f);  //  1/9!
-  q = __builtin_fmaf(q, r2, -1.98412698412698413e-04f); // -1/7!
-  q = __builtin_fmaf(q, r2, 8.33333333333333333e-03f);  //  1/5!
-  q = __builtin_fmaf(q, r2, -1.66666666666666667e-01f); // -1/3!
-  return __builtin_fmaf(q * r2, r, r);
+// the natural logarithm of a normal x > 0
+__device__ __forceinline__ float ln_f(float x) {
+  return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
 }
 
 // the general waveform, out of line: tables and square waves of other
@@ -349,13 +341,10 @@ __device__ __noinline__ double wave_eval_slow(const Waves *__restrict__ W, const
 // IG_WAVE_SIN, IG_WAVE_COS or IG_WAVE_SQUARE at duty 0.5 (sin >= 0, i.e.
 // the reduced phase >= 0)
 __device__ __forceinline__ float fast_unit(int kind, double om, double ph, double t) {
-  float r = phase_reduce(om * t + ph);
+  const float r = phase_rev(om * t + ph);
   if (kind == IG_WAVE_SQUARE) return r >= 0.0f ? 1.0f : -1.0f;
-  if (kind == IG_WAVE_COS) { // cos(r) = sin(r + pi/2), back into [-pi, pi]
-    r = r + 1.57079632679489662f;
-    if (r > 3.14159265358979324f) r = r - 6.28318530717958648f;
-  }
-  return sin_pi_range(r);
+  if (kind == IG_WAVE_COS) return __builtin_amdgcn_cosf(r);
+  return __builtin_amdgcn_sinf(r);
 }
 
 // The tune row every active lane of the wave flies (its index read from
@@ -413,17 +402,10 @@ __device__ __forceinline__ float u24_low(uint32_t a, uint32_t b, uint32_t c) {
          5.9604644775390625e-8f;
 }
 
-// sin and cos of r in [-pi, pi]
-__device__ __forceinline__ void sincos_pi_range(float r, float &sn, float &cs) {
-  sn = sin_pi_range(r);
-  float rc = r + 1.57079632679489662f;
-  if (rc > 3.14159265358979324f) rc = rc - 6.28318530717958648f;
-  cs = sin_pi_range(rc);
-}
-
-// (sin, cos) of the uniformly random angle 2 pi U - pi
+// (sin, cos) of the uniformly random angle 2 pi U, U in [0, 1)
 __device__ __forceinline__ void azimuth(float u, float &sn, float &cs) {
-  sincos_pi_range(__builtin_fmaf(6.28318530717958648f, u, -3.14159265358979324f), sn, cs);
+  sn = __builtin_amdgcn_sinf(u);
+  cs = __builtin_amdgcn_cosf(u);
 }
 
 // ---- hard sphere, FP32-basis mode, flattened into the lane state machine
@@ -454,10 +436,11 @@ __device__ __forceinline__ void azimuth(float u, float &sn, float &cs) {
 //   1.0-1.45 tries on average.
 // * Scattering: the reference's impact angle asin(sqrt(U)), uniform
 //   azimuth and line-of-centres attenuation, in FP32 unit vectors.
-// Logarithms, reciprocals and square roots use the hardware's FP32
-// instructions (v_log_f32, v_rcp_f32, v_rsq_f32, v_sqrt_f32, each within
-// about one ulp); the angles use sin_pi_range.  A lane performs one partner
-// try per pass of its kernel loop while the other lanes keep stepping.
+// Logarithms, reciprocals, square roots, sines and cosines use the
+// hardware's FP32 instructions (v_log_f32, v_rcp_f32, v_rsq_f32,
+// v_sqrt_f32, v_sin_f32, v_cos_f32, each within about one ulp or 1.3e-7
+// absolute).  A lane performs one partner try per pass of its kernel loop
+// while the other lanes keep stepping.
 struct HsTry {
   float wx, wy, wz; // ion velocity in the gas frame
   float sp;         // max(|w|, 1e-7)
@@ -472,7 +455,7 @@ __device__ __forceinline__ float hs_rate_const(const FlightK &p, double dt, doub
 }
 
 // an Exp(1) variate from a word
-__device__ __forceinline__ float exp1_of(uint32_t w) { return -__logf(u24p(w)); }
+__device__ __forceinline__ float exp1_of(uint32_t w) { return -ln_f(u24p(w)); }
 
 // c_rel(sp) / c_bar as a function of s = sp / c* (collision3d.py:265-272),
 // from fits by ion_gym_amd/csrc/fit_c_rel.py: for s < 1 (heavy ions in a
@@ -524,20 +507,11 @@ __device__ __forceinline__ bool hs_hazard_step(Rng &rng, float &hz, double vx, d
     rng_block(rng, w);
     hz = exp1_of(w[0]);
   }
-  // in still gas (a uniform test) v - 0 is v: the same values without the
-  // three FP64 subtractions
-  float wx, wy, wz;
-  if (p.vg[0] == 0.0 && p.vg[1] == 0.0 && p.vg[2] == 0.0) {
-    wx = (float)vx;
-    wy = (float)vy;
-    wz = (float)vz;
-  } else {
-    wx = (float)(vx - p.vg[0]);
-    wy = (float)(vy - p.vg[1]);
-    wz = (float)(vz - p.vg[2]);
-  }
-  float sp = __builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz);
-  if (sp < 1e-7f) sp = 1e-7f;
+  // (in still gas v - 0 is v exactly)
+  const float wx = (float)(vx - p.vg[0]);
+  const float wy = (float)(vy - p.vg[1]);
+  const float wz = (float)(vz - p.vg[2]);
+  const float sp = fmaxf(__builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz), 1e-7f);
   hz -= kconst * c_rel_ratio_f(sp * inv_c_star);
   if (hz > 0.0f) return false;
   st.wx = wx;
@@ -566,8 +540,8 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
   float gx, gy, gz, gl;
   if (u24_low(w[1], w[2], w[3]) * (st.sp + gmean) < st.sp) {
     // Maxwell: three normals from two Box-Muller pairs
-    const float r1 = sig * __builtin_amdgcn_sqrtf(-2.0f * __logf(u24p(w[1])));
-    const float r2 = sig * __builtin_amdgcn_sqrtf(-2.0f * __logf(u24p(w[3])));
+    const float r1 = sig * __builtin_amdgcn_sqrtf(-2.0f * ln_f(u24p(w[1])));
+    const float r2 = sig * __builtin_amdgcn_sqrtf(-2.0f * ln_f(u24p(w[3])));
     float s1, c1, s2, c2;
     azimuth(u24(w[2]), s1, c1);
     azimuth(u24(w[4]), s2, c2);
@@ -577,7 +551,7 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
     gl = __builtin_amdgcn_sqrtf(gx * gx + gy * gy + gz * gz);
   } else {
     // speed-weighted Maxwell: |g| = sig1d sqrt(chi^2_4), isotropic
-    gl = sig * __builtin_amdgcn_sqrtf(-2.0f * __logf(u24p(w[1]) * u24p(w[2])));
+    gl = sig * __builtin_amdgcn_sqrtf(-2.0f * ln_f(u24p(w[1]) * u24p(w[2])));
     const float c = __builtin_fmaf(2.0f, u24(w[3]), -1.0f);
     float sn, cs;
     azimuth(u24(w[4]), sn, cs);

@@ -1,9 +1,7 @@
 """The FP32-basis mode's polynomial helpers, read from the kernel header
 and evaluated in FP32 on the host: the fitted exact-Maxwell mean relative
 speed (collision3d.py:265-272) within 6e-7 relative over s in [0, 30]
-(2.5e-7 below s = 1),
-and the reduced-range sine within 2e-7 absolute over [-pi, pi]."""
-import math
+(2.5e-7 below s = 1)."""
 import os
 import re
 
@@ -66,22 +64,3 @@ def test_c_rel_polynomials():
     rel = np.abs(out.astype(float) - ex) / ex
     assert rel[lo].max() < 2.5e-7, rel[lo].max()
     assert rel.max() < 6e-7, rel.max()
-
-
-def test_sin_polynomial():
-    body = _body("float sin_pi_range(float r)")
-    c = _coeffs(body)
-    assert np.allclose(c, [-1 / math.factorial(11), 1 / math.factorial(9),
-                           -1 / math.factorial(7), 1 / math.factorial(5),
-                           -1 / math.factorial(3)], rtol=1e-7)
-    f32 = np.float32
-    r = np.linspace(-math.pi, math.pi, 200001).astype(np.float32)
-    rr = r.copy()
-    hp, pi = f32(1.57079632679489662), f32(3.14159265358979324)
-    rr = np.where(rr > hp, pi - rr, np.where(rr < -hp, -pi - rr, rr))
-    r2 = (rr * rr).astype(np.float32)
-    q = np.full_like(rr, f32(c[0]))
-    for a in c[1:]:
-        q = (q * r2 + f32(a)).astype(np.float32)
-    v = (q * r2 * rr + rr).astype(np.float32)
-    assert np.max(np.abs(v - np.sin(r.astype(float)))) < 2e-7
