@@ -347,6 +347,34 @@ __device__ __forceinline__ float fast_unit(int kind, double om, double ph, doubl
   return __builtin_amdgcn_sinf(r);
 }
 
+// An affine drive weight (a w(om t + ph) + b) sc at the lab times
+// t0 + m hh, m < M (a 3-D step's t, t + h/2 and t + h with hh = h/2), for w
+// sin, cos or square at duty 0.5 (mode): the phase reduced in revolutions
+// in FP64 at t0 and advanced by om hh / 2 pi in FP32 (the same phases to
+// the FP32 rounding; the square's sign taken from the hardware sine, so a
+// phase past half a revolution needs no second reduction), the affine map
+// in FP32.
+template <int M>
+__device__ __forceinline__ void affine_weights_fast(int mode, double om, double ph, double a,
+                                                    double b, double sc, double t0, double hh,
+                                                    float (&o)[M]) {
+  float r = phase_rev(om * t0 + ph);
+  const float dr = (float)(om * hh * (1.0 / IG_TWO_PI));
+  const float af = (float)(a * sc), bf = (float)(b * sc);
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    float u;
+    if (mode == IG_WAVE_SQUARE)
+      u = __builtin_amdgcn_sinf(r) >= 0.0f ? 1.0f : -1.0f;
+    else if (mode == IG_WAVE_COS)
+      u = __builtin_amdgcn_cosf(r);
+    else
+      u = __builtin_amdgcn_sinf(r);
+    o[m] = __builtin_fmaf(af, u, bf);
+    r = r + dr;
+  }
+}
+
 // The tune row every active lane of the wave flies (its index read from
 // the first active lane, so the row's address derives from the table's
 // with a wave-uniform offset and loads through it are scalar), or -1 when

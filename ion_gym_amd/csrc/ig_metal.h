@@ -68,7 +68,9 @@ __device__ __forceinline__ bool in_metal_3d(const uint8_t *__restrict__ m, doubl
                                             double z, int64_t nx, int64_t ny, int64_t nz) {
   if (!(x >= 0.0) || !(y >= 0.0) || !(z >= 0.0) || x > nx - 1 || y > ny - 1 || z > nz - 1)
     return false;
-  int64_t i = (int64_t)x, j = (int64_t)y, k = (int64_t)z;
+  // inside the box each coordinate is below 2^31: the 32-bit truncation
+  // is exact (one instruction; an int64 conversion is a long sequence)
+  int64_t i = cvt_i32_sat(x), j = cvt_i32_sat(y), k = cvt_i32_sat(z);
   if (i > nx - 2) i = nx - 2;
   if (j > ny - 2) j = ny - 2;
   if (k > nz - 2) k = nz - 2;

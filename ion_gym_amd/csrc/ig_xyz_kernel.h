@@ -54,8 +54,11 @@ __device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy
   if (x > nx - 1) x = nx - 1.0;
   if (y > ny - 1) y = ny - 1.0;
   if (z > nz - 1) z = nz - 1.0;
-  // NaN coordinates pass the clamps above; keep the conversion defined
-  int64_t i = (int64_t)fmax(x, 0.0), j = (int64_t)fmax(y, 0.0), k = (int64_t)fmax(z, 0.0);
+  // the clamped coordinates lie in [0, n - 1] (n below 2^31) or are NaN,
+  // which passes the clamps above: the hardware's 32-bit truncation gives
+  // the same index, and 0 for NaN (one instruction each; a conversion to
+  // int64 is a long sequence)
+  int64_t i = cvt_i32_sat(x), j = cvt_i32_sat(y), k = cvt_i32_sat(z);
   if (i > nx - 2) i = nx - 2;
   if (j > ny - 2) j = ny - 2;
   if (k > nz - 2) k = nz - 2;
@@ -115,35 +118,35 @@ __device__ __forceinline__ void xyz_field(const FieldsK &f, double gx, double gy
   ez = e[2];
 }
 
-// The cell and fractions of _tri (tracer3d.py:121-151) in FP64 grid units:
-// coordinates clamped to the box (NaN passes the clamps and is kept
-// defined by the fmax), the index clamped to n - 2, the fractions rounded
-// to FP32 for the FP32-basis blend.
+// The cell and fractions of _tri (tracer3d.py:121-151) from FP64 grid
+// units: coordinates clamped to the box, the index clamped to n - 2, the
+// fractions rounded to FP32 for the FP32-basis blend.
 struct Cell3 {
   int32_t i, j, k; // a 3-D pack holds fewer than 2^31 nodes
   float tx, ty, tz;
 };
 
+// In integer and FP32 arithmetic: the index by the saturating hardware
+// conversion clamped to [0, n - 2], the fraction converted to FP32 and
+// clamped to [0, 1].  For every finite point these are the index and
+// fraction of clamping the coordinate to [0, n - 1] first (a coordinate
+// below 0 gives index 0 and fraction 0, one above n - 1 index n - 2 and
+// fraction 1, and the FP32 rounding is monotone with 0 and 1 exact),
+// without the FP64 selects and the int64 conversions.
+__device__ __forceinline__ int32_t cell_index(double g, int64_t n) {
+  return min(max(cvt_i32_sat(g), 0), (int32_t)(n - 2));
+}
+__device__ __forceinline__ float cell_frac(double g, int32_t i) {
+  return __builtin_amdgcn_fmed3f((float)(g - (double)i), 0.0f, 1.0f);
+}
 __device__ __forceinline__ Cell3 cell3(const FieldsK &f, double gx, double gy, double gz) {
-  const int64_t nx = f.n0, ny = f.n1, nz = f.n2;
-  double x = gx, y = gy, z = gz;
-  if (x < 0.0) x = 0.0;
-  if (y < 0.0) y = 0.0;
-  if (z < 0.0) z = 0.0;
-  if (x > nx - 1) x = nx - 1.0;
-  if (y > ny - 1) y = ny - 1.0;
-  if (z > nz - 1) z = nz - 1.0;
-  int64_t i = (int64_t)fmax(x, 0.0), j = (int64_t)fmax(y, 0.0), k = (int64_t)fmax(z, 0.0);
-  if (i > nx - 2) i = nx - 2;
-  if (j > ny - 2) j = ny - 2;
-  if (k > nz - 2) k = nz - 2;
   Cell3 c;
-  c.i = (int32_t)i;
-  c.j = (int32_t)j;
-  c.k = (int32_t)k;
-  c.tx = (float)(x - i);
-  c.ty = (float)(y - j);
-  c.tz = (float)(z - k);
+  c.i = cell_index(gx, f.n0);
+  c.j = cell_index(gy, f.n1);
+  c.k = cell_index(gz, f.n2);
+  c.tx = cell_frac(gx, c.i);
+  c.ty = cell_frac(gy, c.j);
+  c.tz = cell_frac(gz, c.k);
   return c;
 }
 
@@ -345,83 +348,71 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   constexpr int NW = NS > 1 ? NS - 1 : 1;
   double w[NW];
   double wtt = __builtin_nan("");
-  // FP32 mode: the per-set composition weights at lab time tt; a tuned
-  // ion reads its tune's packed records (three loads a set, the tune's set
-  // scale already in a and b)
+  // FP32 mode: the per-set composition weights at lab times t0 + m hh,
+  // m < M (a step's t, t + h/2, t + h with hh = h/2; M = 1 for one time).
+  // A tuned ion reads its tune's packed records (the tune's set scale
+  // already in a and b) -- through scalar loads when every lane of the
+  // wave flies the same row -- an untuned one the pack's waveforms through
+  // the kernel argument (scalar loads); sin, cos and square at duty 0.5 by
+  // affine_weights_fast, other drives (tables, other duties) in FP64
   const Tune *tun = e.tunes ? e.tunes + e.tune[i] : nullptr;
-  const WaveF *twf = tun ? tun->wf : nullptr;
-  auto wt_at = [&](double tt, float(&o)[NS]) {
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      double v;
-      if (twf) {
-        const int mode = wave_mode(tun->wmode, q);
-        if (q == 0)
-          v = twf[0].a;
-        else if (q > K)
-          v = 0.0;
-        else if (mode == kWaveSlow)
-          v = wave_eval_slow<true>(W, f, q - 1, tt) * tscale[q];
-        else
-          v = twf[q].a * (double)fast_unit(mode, twf[q].om, twf[q].ph, tt) + twf[q].b;
-      } else {
-        // untuned (no tune row): the pack's waveforms through the kernel
-        // argument, one address for the wave (scalar loads)
-        v = q == 0 ? 1.0 : (q <= K ? wave_eval_fast<true>(f.waves, f, q - 1, tt) : 0.0);
-      }
-      v = v * f.scale[q];
-      o[q] = (float)v;
-    }
-  };
-  // the weights of a step's three lab times, each tuned set's record read
-  // once for all three (the same arithmetic as three wt_at calls)
-  // the tune row every lane of the wave flies, if they share one: its
-  // records then come through scalar loads (the row index is read from
-  // the first lane, so the address is wave-uniform)
   const int32_t row0 = e.tunes ? wave_shared_row(e.tune[i]) : -1;
   const Tune *const tunU = row0 >= 0 ? e.tunes + row0 : nullptr;
-  auto wt_rows3 = [&](const Tune *T, double t0, double t1, double t2, float(&o0)[NS],
-                      float(&o1)[NS], float(&o2)[NS]) {
-    const WaveF *Twf = T->wf;
-    const uint4 mw = *reinterpret_cast<const uint4 *>(T->wmode);
-    const uint32_t mws[4] = {mw.x, mw.y, mw.z, mw.w};
+  auto weights_f32 = [&](auto &o, double t0, double hh) {
+    constexpr int M = sizeof(o) / sizeof(o[0]);
+    const Tune *T = tunU ? tunU : tun;
+    const Waves *Wq = tunU ? &tunU->w : W;
+    const double *sq = tunU ? tunU->scale : tscale;
+    uint32_t mws[4] = {0u, 0u, 0u, 0u};
+    if (T) {
+      const uint4 mw = *reinterpret_cast<const uint4 *>(T->wmode);
+      mws[0] = mw.x;
+      mws[1] = mw.y;
+      mws[2] = mw.z;
+      mws[3] = mw.w;
+    }
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      const int mode = wave_mode(mws, q);
-      const double tt[3] = {t0, t1, t2};
-      double v[3];
-      WaveF w{};
-      if (q == 0)
-        w.a = Twf[0].a;
-      else if (q <= K)
-        w = Twf[q];
+      float v[M];
+      if (q == 0) {
+        const float a0 = (float)((T ? T->wf[0].a : 1.0) * f.scale[0]);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        if (q == 0)
-          v[m] = w.a;
-        else if (q > K)
-          v[m] = 0.0;
-        else if (mode == kWaveSlow)
-          v[m] = wave_eval_slow<true>(W, f, q - 1, tt[m]) * tscale[q];
-        else
-          v[m] = w.a * (double)fast_unit(mode, w.om, w.ph, tt[m]) + w.b;
-        v[m] = v[m] * f.scale[q];
+        for (int m = 0; m < M; ++m) v[m] = a0;
+      } else if (q > K) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = 0.0f;
+      } else {
+        int mode;
+        double om, ph, a, b;
+        if (T) {
+          mode = wave_mode(mws, q);
+          const WaveF w = T->wf[q];
+          om = w.om;
+          ph = w.ph;
+          a = w.a;
+          b = w.b;
+        } else {
+          const int kind = f.waves->kind[q - 1];
+          mode = (kind == IG_WAVE_SIN || kind == IG_WAVE_COS ||
+                  (kind == IG_WAVE_SQUARE && f.waves->duty[q - 1] == 0.5))
+                     ? kind
+                     : kWaveSlow;
+          om = f.waves->om[q - 1];
+          ph = f.waves->ph[q - 1];
+          a = f.waves->amp[q - 1];
+          b = f.waves->off[q - 1];
+        }
+        if (mode == kWaveSlow) {
+#pragma unroll
+          for (int m = 0; m < M; ++m)
+            v[m] = (float)(wave_eval_slow<true>(T ? Wq : f.waves, f, q - 1, t0 + m * hh) *
+                           (T ? sq[q] : 1.0) * f.scale[q]);
+        } else {
+          affine_weights_fast<M>(mode, om, ph, a, b, f.scale[q], t0, hh, v);
+        }
       }
-      o0[q] = (float)v[0];
-      o1[q] = (float)v[1];
-      o2[q] = (float)v[2];
-    }
-  };
-  auto wt_at3 = [&](double t0, double t1, double t2, float(&o0)[NS], float(&o1)[NS],
-                    float(&o2)[NS]) {
-    if (!twf) {
-      wt_at(t0, o0);
-      wt_at(t1, o1);
-      wt_at(t2, o2);
-    } else if (tunU) {
-      wt_rows3(tunU, t0, t1, t2, o0, o1, o2);
-    } else {
-      wt_rows3(tun, t0, t1, t2, o0, o1, o2);
+#pragma unroll
+      for (int m = 0; m < M; ++m) o[m][q] = v[m];
     }
   };
   // FP64 modes: the waveform weights, kept while the lab time repeats
@@ -447,9 +438,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       // weights made afresh: FP32 steps gather through the cell cache, so
       // this path (a stage that left its cell, a record row) is rare and
       // keeps no weights live across steps
-      float wl[NS];
-      wt_at(tt, wl);
-      xyz_field_f32<NS>(f, px * inv_h, py * inv_h, pz * inv_h, wl, a, b, c);
+      float wl[1][NS];
+      weights_f32(wl, tt, 0.0);
+      xyz_field_f32<NS>(f, px * inv_h, py * inv_h, pz * inv_h, wl[0], a, b, c);
     } else {
       weights(tt);
       xyz_field<PREC, NS>(f, px * inv_h, py * inv_h, pz * inv_h, w, K, tscale, a, b, c);
@@ -534,9 +525,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
           if constexpr (PREC == IG_PREC_F32) {
             const Cell3 cl = cell3(f, px * inv_h, py * inv_h, pz * inv_h);
             if (st == 0) {
-              float w0[NS], w1[NS], w2[NS];
-              wt_at3(t_us, t_us + 0.5 * h_us, t_us + h_us, w0, w1, w2);
-              xyz_cell_f32<NS>(f, cl, w0, w1, w2, ea, eb, ec, kc, bs);
+              float ww[3][NS];
+              weights_f32(ww, t_us, 0.5 * h_us);
+              xyz_cell_f32<NS>(f, cl, ww[0], ww[1], ww[2], ea, eb, ec, kc, bs);
               cl0 = cl;
             } else if (cl.i == cl0.i && cl.j == cl0.j && cl.k == cl0.k) {
               blend_cell_f32(cl, kc, bs, st == 3, ea, eb, ec);
