@@ -155,12 +155,14 @@ __device__ __forceinline__ float corner_w(const Cell3 &c, int di, int dj, int dk
   return (di ? c.tx : 1.0f - c.tx) * (dj ? c.ty : 1.0f - c.ty) * (dk ? c.tz : 1.0f - c.tz);
 }
 
-// One corner record (3 NS floats, read as float4s) composed with per-set
+// One corner record (3 NS floats, as float4s) composed with per-set
 // weights: acc[comp] = sum over sets of w[set] * E[set][comp], FMAs in
-// record order.
+// record order.  compose_vals takes the record from registers,
+// compose_rec loads it first.
 template <int NS, int M>
-__device__ __forceinline__ void compose_rec(const float4 *rec, const float (*const (&w)[M])[NS],
-                                            float (&acc)[M][3]) {
+__device__ __forceinline__ void compose_vals(const float4 (&rec)[RecLayout3<float, NS>::kStride / 4],
+                                             const float (*const (&w)[M])[NS],
+                                             float (&acc)[M][3]) {
   constexpr int Q = RecLayout3<float, NS>::kStride / 4;
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0f;
@@ -178,6 +180,14 @@ __device__ __forceinline__ void compose_rec(const float4 *rec, const float (*con
       }
     }
   }
+}
+template <int NS, int M>
+__device__ __forceinline__ void compose_rec(const float4 *rec, const float (*const (&w)[M])[NS],
+                                            float (&acc)[M][3]) {
+  float4 r[RecLayout3<float, NS>::kStride / 4];
+#pragma unroll
+  for (int q4 = 0; q4 < RecLayout3<float, NS>::kStride / 4; ++q4) r[q4] = rec[q4];
+  compose_vals<NS, M>(r, w, acc);
 }
 
 // The FP32-basis field: each corner record is read as float4s once and
@@ -242,23 +252,22 @@ __device__ __forceinline__ void xyz_cell_f32(const FieldsK &f, const Cell3 &cl,
                 dJ = nz * Q;
   const float (*const ws[3])[NS] = {&w0, &w1, &w2};
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  // one corner record in flight at a time: the three weight sets and nine
-  // sums leave no room for a second
+  // one corner record in flight at a time (the three weight sets and nine
+  // sums leave no room for a second), its float4s all loaded before the
+  // first is used
 #pragma unroll 1
   for (int n = 0; n < 8; ++n) {
-    {
-      const int col = n >> 1, dk = n & 1, di = (col >> 1) & 1, dj = col & 1;
-      const float wc = corner_w(cl, di, dj, dk);
-      float c[3][3];
-      compose_rec<NS, 3>(g + r00 + di * dI + dj * dJ + dk * Q, ws, c);
-      ax = __builtin_fmaf(wc, c[0][0], ax);
-      ay = __builtin_fmaf(wc, c[0][1], ay);
-      az = __builtin_fmaf(wc, c[0][2], az);
+    const int col = n >> 1, dk = n & 1, di = (col >> 1) & 1, dj = col & 1;
+    const float wc = corner_w(cl, di, dj, dk);
+    float c[3][3];
+    compose_rec<NS, 3>(g + r00 + di * dI + dj * dJ + dk * Q, ws, c);
+    ax = __builtin_fmaf(wc, c[0][0], ax);
+    ay = __builtin_fmaf(wc, c[0][1], ay);
+    az = __builtin_fmaf(wc, c[0][2], az);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        kc[(6 * n + m) * bs] = c[1][m];
-        kc[(6 * n + 3 + m) * bs] = c[2][m];
-      }
+    for (int m = 0; m < 3; ++m) {
+      kc[(6 * n + m) * bs] = c[1][m];
+      kc[(6 * n + 3 + m) * bs] = c[2][m];
     }
   }
   ex = ax;
