@@ -630,14 +630,21 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
 }
 
 // the SDS jump with the table interpolation, the log-log mass-ratio blend
-// and the direction in FP32, on one Philox block per step
+// and the direction in FP32, on one Philox block per step; rj0 is the
+// ion's step-independent factor sqrt(V / mfp dt / 1e5) mfp of the jump
+// length (sds.py:401, computed once per ion by sds_rj0).  The log-log
+// blend 10^(log10 lo + t (log10 hi - log10 lo)) is the same function in
+// base 2, on the hardware's v_log_f32 and v_exp_f32.
+__device__ __forceinline__ double sds_rj0(double V, double mfp, double dt) {
+  return sqrt(V / mfp * dt / 100000.0) * mfp;
+}
 __device__ __forceinline__ void sds_jump_fast(Rng &rng, const double *__restrict__ st,
-                                              double log_mr, double V, double mfp, double dt,
-                                              double &jx, double &jy, double &jz) {
+                                              double log_mr, double rj0, double &jx, double &jy,
+                                              double &jz) {
   uint32_t wd[4];
   rng_block(rng, wd);
   // a quantile strictly inside (0, 1): the table's end quantiles are 0
-  // and its largest step count, and log10 of a 0 quantile is -inf
+  // and its largest step count, and the logarithm of a 0 quantile is -inf
   float x = u24h(wd[0]) * (float)(1002 - 2);
   int i = (int)x;
   if (i > 999) i = 999;
@@ -648,10 +655,9 @@ __device__ __forceinline__ void sds_jump_fast(Rng &rng, const double *__restrict
   const double *a = st + k * 1002, *b = a + 1002;
   float lo = (float)a[i] + w * ((float)a[i + 1] - (float)a[i]);
   float hi = (float)b[i] + w * ((float)b[i + 1] - (float)b[i]);
-  float llo = log10f(lo);
-  float dsteps = exp10f(llo + t * (log10f(hi) - llo));
-  double nc = V / mfp * dt;
-  double rj = sqrt(nc / 100000.0) * (double)dsteps * mfp;
+  const float llo = __builtin_amdgcn_logf(lo);
+  const float dsteps = __builtin_amdgcn_exp2f(__builtin_fmaf(t, __builtin_amdgcn_logf(hi) - llo, llo));
+  double rj = rj0 * (double)dsteps;
   float c = __builtin_fmaf(2.0f, u24(wd[1]), -1.0f);
   float one_m = 1.0f - c * c;
   float sn, cs;

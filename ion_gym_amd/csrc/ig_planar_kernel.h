@@ -123,6 +123,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     const double tt = sds_damp * dt;
     sds_fac = (1.0 - exp(-tt)) / tt;
   }
+  // the FP32-basis jump's per-ion length factor
+  const double sds_rj = GAS == IG_GAS_SDS && sds_mfp > 0.0 ? sds_rj0(sds_V, sds_mfp, dt) : 0.0;
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
   constexpr bool kFlat = PREC == IG_PREC_F32 && GAS == IG_GAS_HS;
   float hs_k = 0.0f, hs_inv_cstar = 0.0f, hz = 0.0f;
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
           if (sds_mfp > 0.0) {
             double jx, jy, jz;
             if constexpr (PREC == IG_PREC_F32)
-              sds_jump_fast(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
+              sds_jump_fast(rng, p.sds_stats, sds_logmr, sds_rj, jx, jy, jz);
             else
               sds_jump(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
             x += jx;

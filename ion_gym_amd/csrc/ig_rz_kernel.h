@@ -281,6 +281,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       sds_fac = (1.0 - exp(-tt)) / tt;
     }
   }
+  // the FP32-basis jump's per-ion length factor
+  const double sds_rj = GAS == IG_GAS_SDS && sds_mfp > 0.0 ? sds_rj0(sds_V, sds_mfp, dt) : 0.0;
   const double sigma = GAS == IG_GAS_HS ? e.sigma[i] : 0.0;
   // FP32-basis hard sphere: collision exponent constant and the
   // flattened partner-try state
@@ -553,7 +555,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
               pre = sqrt((x - xo) * (x - xo) + (y - yo) * (y - yo) + (z - zo) * (z - zo));
             double jx, jy, jz;
             if constexpr (PREC == IG_PREC_F32)
-              sds_jump_fast(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
+              sds_jump_fast(rng, p.sds_stats, sds_logmr, sds_rj, jx, jy, jz);
             else
               sds_jump(rng, p.sds_stats, sds_logmr, sds_V, sds_mfp, dt, jx, jy, jz);
             x += jx;

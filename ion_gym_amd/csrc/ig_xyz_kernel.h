@@ -829,6 +829,8 @@ __global__ __launch_bounds__(256) void xyz_sds_kernel(const FieldsK f, const Fli
     const double tterm = damp * dt;
     factor = (1.0 - exp(-tterm)) / tterm;
   }
+  // the FP32-basis jump's per-ion length factor
+  const double sds_rj = mfp > 0.0 ? sds_rj0(V, mfp, dt) : 0.0;
   constexpr int NW = NS > 1 ? NS - 1 : 1;
   double w[NW];
   float wt[NS];
@@ -918,7 +920,7 @@ __global__ __launch_bounds__(256) void xyz_sds_kernel(const FieldsK f, const Fli
     if (mfp > 0.0) {
       double jx, jy, jz;
       if constexpr (PREC == IG_PREC_F32)
-        sds_jump_fast(rng, p.sds_stats, logmr, V, mfp, dt, jx, jy, jz);
+        sds_jump_fast(rng, p.sds_stats, logmr, sds_rj, jx, jy, jz);
       else
         sds_jump(rng, p.sds_stats, logmr, V, mfp, dt, jx, jy, jz);
       x = x + jx;
