@@ -199,3 +199,126 @@ def test_composing_into_a_folded_pack_is_refused(engine):
     with pytest.raises(RuntimeError, match="folded"):
         engine.compose(NoBases(), fp, 1, [])
     fp.release()
+
+
+def _rz_pack(kinds, oms, phs, duties, gate=False, seed=7):
+    """A synthetic r-z pack with the given drive channels (random grids)."""
+    nx, nr, h = 41, 11, 0.25
+    nu = 2 * nr - 1
+    rng = np.random.default_rng(seed)
+    K = len(kinds)
+    pack = dict(EzA=np.full((nx, nu), 3000.0) + rng.normal(size=(nx, nu)),
+                EuA=rng.normal(size=(nx, nu)) * 200.0,
+                EzK=rng.normal(size=(K, nx, nu)) * 500.0,
+                EuK=rng.normal(size=(K, nx, nu)) * 500.0,
+                ele=np.zeros((nx, nu)), label=np.zeros((nx, nu), np.uint8),
+                u0=-(nr - 1) * h, mm=h, ch_kind=np.array(kinds),
+                ch_om=np.array(oms, float), ch_ph=np.array(phs, float),
+                ch_duty=np.array(duties, float), tau_gate=-1.0)
+    if gate:
+        pack.update(EzG=np.full((nx, nu), -2500.0), EuG=np.zeros((nx, nu)),
+                    tau_gate=0.3)
+    return pack
+
+
+def test_what_folds_and_what_does_not(engine):
+    """Linked channels fold whatever their position among the others;
+    channels of different kinds, frequencies or quarter-period phases do
+    not; a table waveform keeps the whole pack as given; the r-z gate set
+    stays last."""
+    import math
+    w = 2 * math.pi
+    cases = [
+        # (kinds, oms, phases, duties, gate) -> (sets, drive channels)
+        (([0, 0], [w, w], [0.0, math.pi], [0.5, 0.5], False), (2, 1)),
+        (([0, 0], [w, w], [0.3, 0.3 + 3 * math.pi], [0.5] * 2, False),
+         (2, 1)),                                   # odd multiple of pi
+        (([0, 0], [w, w], [0.0, 2 * math.pi], [0.5] * 2, False), (2, 1)),
+        (([2, 2], [w, w], [0.0, math.pi], [0.5] * 2, False), (2, 1)),
+        (([2, 2], [w, w], [0.0, math.pi], [0.3, 0.3], False), (3, 2)),
+        (([0, 1], [w, w], [0.0, math.pi], [0.5] * 2, False), (3, 2)),
+        (([0, 0], [w, 2 * w], [0.0, math.pi], [0.5] * 2, False), (3, 2)),
+        (([0, 0], [w, w], [0.0, math.pi / 2], [0.5] * 2, False), (3, 2)),
+        (([1, 0, 0], [3 * w, w, w], [0.2, 0.1, 0.1 + math.pi],
+          [0.5] * 3, False), (3, 2)),               # linked pair after another
+        (([0, 0], [w, w], [0.0, math.pi], [0.5] * 2, True), (3, 1)),
+    ]
+    for (kinds, oms, phs, duties, gate), (n_sets, n_chan) in cases:
+        fp = engine.upload_rz(_rz_pack(kinds, oms, phs, duties, gate), "f32")
+        lay = fp.layout()
+        assert (lay["n_sets"], lay["n_chan"]) == (n_sets, n_chan), \
+            (kinds, phs, lay)
+        fp.release()
+    pk = _rz_pack([0, 3], [w, 0.0], [0.0, 0.0], [0.5, 0.5])
+    pk.update(tab_t=np.array([0.0, 1.0]), tab_v=np.array([0.0, 1.0]),
+              tab_off=np.array([0, 0, 2]))
+    fp = engine.upload_rz(pk, "f32")
+    assert fp.layout()["n_sets"] == 3
+    fp.release()
+
+
+def test_gated_pack_folded_flight(engine):
+    """An r-z pack with a linked RF pair and a gate set: the folded flight
+    against the unfolded one and FP64 (vacuum, 600 steps across the gate
+    time)."""
+    import math
+    from ion_gym_amd.engine import Flight
+    pack = _rz_pack([0, 0], [2 * math.pi * 3.0] * 2, [0.4, 0.4 + math.pi],
+                    [0.5, 0.5], gate=True)
+    rng = np.random.default_rng(3)
+    n, m = 400, 150.0
+    ions = dict(x=rng.uniform(1, 4, n), y=rng.uniform(-1.0, 1.0, n),
+                z=rng.uniform(-1.0, 1.0, n), vx=rng.uniform(0.5, 2, n),
+                vy=rng.normal(0, 0.2, n), vz=rng.normal(0, 0.2, n),
+                tob=rng.uniform(0, 0.2, n), m_amu=np.full(n, m),
+                acc=np.full(n, 1.602176634e-19 / (m * 1.66053906660e-27)),
+                sigma_m2=np.zeros(n), seed=np.arange(n, dtype=np.int64))
+    fl = Flight(1e-3, 8.0, max_steps=600)
+    up = engine.upload_rz
+    ref, _ = _fly(engine, up, pack, ions, fl, "f64", None)
+    plain, lp = _fly(engine, up, pack, ions, fl, "f32", False)
+    fold, lf = _fly(engine, up, pack, ions, fl, "f32", True)
+    assert lp == (4, 2, 2) and lf == (3, 1, 2)
+    e_plain, s_plain = _err(plain, ref, 1.0)
+    e_fold, s_fold = _err(fold, ref, 1.0)
+    print(f"  gated: error vs FP64 unfolded {e_plain:.3g}, folded "
+          f"{e_fold:.3g}", flush=True)
+    assert s_fold >= s_plain - 0.005
+    assert e_fold <= 2.0 * e_plain + 1e-6
+
+
+def test_planar_folded_flight(engine):
+    """The planar quadrupole guide (RF pair in opposite phase): 2 stored
+    sets, the folded flight within the unfolded FP32 error against FP64."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    c = configs.planar_quad(n_ions=1000, coarse=2)
+    fl = Flight(c["dt"], c["t_max"], max_steps=5000)
+    up = engine.upload_planar
+    ref, _ = _fly(engine, up, c["pack"], c["ions"], fl, "f64", None)
+    plain, _ = _fly(engine, up, c["pack"], c["ions"], fl, "f32", False)
+    fold, lf = _fly(engine, up, c["pack"], c["ions"], fl, "f32", True)
+    assert lf == (2, 1, 2)
+    e_plain, s_plain = _err(plain, ref, 1.0)
+    e_fold, s_fold = _err(fold, ref, 1.0)
+    print(f"  planar: error vs FP64 unfolded {e_plain:.3g}, folded "
+          f"{e_fold:.3g}", flush=True)
+    assert s_fold >= s_plain - 0.002
+    assert e_fold <= 2.0 * e_plain + 1e-6
+
+
+def test_idle_zero_drive_tune_flies_a_folded_pack(engine):
+    """A tune row whose drive is identically zero (sin at zero frequency
+    and phase, as a server's idle rows are) breaks no linked group."""
+    from ion_gym_amd import configs
+    from ion_gym_amd.engine import Flight
+    from ion_gym_amd.serve import make_tune
+    c = configs.funnel_rz(n_ions=64, seed=5, coarse=4)
+    pack = c["model"].pack()
+    idle = make_tune(pack, dt=c["dt"], t_max=c["t_max"],
+                     ch_om=np.zeros(2), ch_ph=np.zeros(2))
+    fl = Flight(c["dt"], c["t_max"], max_steps=50)
+    out, lay = _fly(engine, engine.upload_rz, pack, c["ions"], fl, "f32",
+                    True, [idle], np.zeros(64, int))
+    assert lay == (2, 1, 2)
+    assert np.all(np.isfinite(out["x"]))
