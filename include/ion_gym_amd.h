@@ -430,7 +430,11 @@ int ig_last_fly_ms(ig_engine *h, double *out);
  * call allocates nothing (the ensemble and the field pack are the
  * caller's, created beforehand).  At most IG_NOALLOC_MAX_IONS ions per
  * ensemble (IG_E_ARG beyond).  Every call on a device shares its region:
- * calls that overlap in time on one device race. */
+ * calls that overlap in time on one device race.  Two inputs are copied
+ * to the device (host work, then a synchronizing copy, no allocation) when
+ * they change: the SDS jump table, and an ensemble's tune table re-laid for
+ * an IG_FOLD_LINKED pack it has not flown on last; a caller capturing the
+ * call into a graph flies once before capturing. */
 #define IG_NOALLOC_MAX_IONS (1 << 20)
 int ig_fly_noalloc(ig_engine *h, ig_fields_id f, const ig_flight_params *p,
                    ig_ensemble_id e);
