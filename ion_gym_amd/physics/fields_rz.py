@@ -216,9 +216,25 @@ def spec_terms_rz(spec):
     return dc, members, list(spec.geometry.rf_groups)
 
 
+def collide_labels(spec, labels):
+    """The labels an r-z model carries (``ele``) from the labels a deck's
+    solve fixes: an electrode declared ``is_grid`` (a mesh, a reflectron's
+    entrance or back plane, a detector face) is a Dirichlet boundary of the
+    solve but is transparent to ions, so its nodes get label 0 -- no splat
+    on it, and the field is differenced through it (build_rz.py:397-411).
+    Every other label is kept."""
+    labels = np.asarray(labels)
+    out = labels.copy()
+    for pos, el in enumerate(spec.geometry.electrodes, start=1):
+        if el.is_grid:
+            out[labels == (el.basis if el.basis is not None else pos)] = 0
+    return out
+
+
 def compose_spec_rz(spec, bases, ele):
     """The r-z model of a deck from its solved bases (compose_rz with
-    spec_terms_rz); the grid pitch is geometry.mm_per_gu."""
+    spec_terms_rz); the grid pitch is geometry.mm_per_gu.  ``ele`` is the
+    deck's collide labels (collide_labels: its grids transparent)."""
     dc, members, groups = spec_terms_rz(spec)
     return compose_rz(bases, ele, spec.geometry.mm_per_gu, dc, members,
                       groups)
