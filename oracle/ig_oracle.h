@@ -6,7 +6,7 @@
  * the product path links, loads or calls it.
  *
  * It restates, in plain C with strict IEEE double arithmetic
- * (-O2 -fno-fast-math -ffp-contract=off), the reference kernels:
+ * (-O3 -fno-fast-math -ffp-contract=off), the reference kernels:
  *   ion_gym/physics/tracer_rz.py:113-446   _fly_rec_full (r-z leapfrog)
  *   ion_gym/physics/tracer3d.py:280-575    _fly3d        (3-D RK4)
  *   ion_gym/physics/build_planar.py:1111-1429 _fly_planar (planar leapfrog)
