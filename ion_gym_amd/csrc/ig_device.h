@@ -553,8 +553,8 @@ __device__ __forceinline__ bool hs_hazard_step(Rng &rng, float &hz, double vx, d
 // One partner try on two Philox blocks (8 words): w0 acceptance, w1-w4
 // the proposal (two Box-Muller pairs for the Maxwell branch; U1, U2,
 // cos theta and azimuth for the speed-weighted one), w5 the next hazard,
-// w6 and w7 the impact angle and azimuth, and the low bytes of w1-w3 the
-// mixture choice.  Returns 1 when the collision is complete (velocity
+// w6 and w7 the scattered direction (its cosine and azimuth), and the low
+// bytes of w1-w3 the mixture choice.  Returns 1 when the collision is complete (velocity
 // updated, a fresh hazard drawn), 0 to try again, -1 when
 // MAX_PARTNER_TRIES (collision3d.py:72) ran out.
 __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, double &vx,
@@ -596,36 +596,25 @@ __device__ __forceinline__ int hs_try_flat(Rng &rng, HsTry &st, float &hz, doubl
   }
   hz = exp1_of(w[5]);
   if (rlen < 1e-12f) return 1;
-  // sin(asin(sqrt(U))) = sqrt(U), cos = sqrt(1 - U), U < 1
-  const float au = u24(w[6]);
-  const float sa = __builtin_amdgcn_sqrtf(au), ca = __builtin_amdgcn_sqrtf(1.0f - au);
+  // The reference reflects the relative velocity r about the line of
+  // centres rh, drawn at angle asin(sqrt(U)) from r with a uniform azimuth
+  // (collision3d.py:310-330): r' = r - 2 (r . rh) rh.  Its angle chi to r
+  // then has cos chi = 2U - 1, uniform on [-1, 1] (the reference scales U
+  // by 0.999999999), and a uniform azimuth:
+  // r' is |r| times a uniform unit vector (hard spheres scatter
+  // isotropically in the centre-of-mass frame), drawn here directly from
+  // the same two uniforms.  The ion takes m_gas / (m_ion + m_gas) of the
+  // change, the reference's v - f rh with f = 2 m_gas / (m_ion + m_gas)
+  // (r . rh).
+  const float c = __builtin_fmaf(2.0f, u24(w[6]), -1.0f);
+  const float one_m = 1.0f - c * c;
+  const float s = rlen * __builtin_amdgcn_sqrtf(one_m > 0.0f ? one_m : 0.0f);
   float sth, cth;
   azimuth(u24(w[7]), sth, cth);
-  const float inv = __builtin_amdgcn_rcpf(rlen);
-  const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
-  float px, py, pz;
-  if (fabsf(ux) < 0.9f) {
-    px = 0.0f;
-    py = -uz;
-    pz = uy;
-  } else {
-    px = -uz;
-    py = 0.0f;
-    pz = ux;
-  }
-  const float pinv = __builtin_amdgcn_rsqf(px * px + py * py + pz * pz);
-  px *= pinv;
-  py *= pinv;
-  pz *= pinv;
-  const float qx = uy * pz - uz * py, qy = uz * px - ux * pz, qz = ux * py - uy * px;
-  const float rhx = ca * ux + sa * (cth * px + sth * qx);
-  const float rhy = ca * uy + sa * (cth * py + sth * qy);
-  const float rhz = ca * uz + sa * (cth * pz + sth * qz);
-  const double f = 2.0 * p.m_gas / (m_ion + p.m_gas) *
-                   (double)(rx * rhx + ry * rhy + rz * rhz);
-  vx = (vx - p.vg[0] - f * (double)rhx) + p.vg[0];
-  vy = (vy - p.vg[1] - f * (double)rhy) + p.vg[1];
-  vz = (vz - p.vg[2] - f * (double)rhz) + p.vg[2];
+  const double mf = p.m_gas / (m_ion + p.m_gas);
+  vx = (vx - p.vg[0] + mf * (double)(s * cth - rx)) + p.vg[0];
+  vy = (vy - p.vg[1] + mf * (double)(s * sth - ry)) + p.vg[1];
+  vz = (vz - p.vg[2] + mf * (double)(rlen * c - rz)) + p.vg[2];
   return 1;
 }
 
