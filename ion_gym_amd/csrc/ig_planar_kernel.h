@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
-        if (__popcll(parked) >= p.coll_batch || moving == 0) {
+        if (__popcll(parked) >= p.coll_batch || __popcll(moving) < p.coll_batch) {
           if (stage == kCollide) {
             const int rc = hs_try_flat(rng, hst, hz, vx, vy, vz, p, m_ion);
             if (rc > 0) {
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
-        if (__popcll(parked) >= p.coll_batch || moving == 0) {
+        if (__popcll(parked) >= p.coll_batch || __popcll(moving) < p.coll_batch) {
           if (stage == kCollide) {
             if (hs_scatter(rng, vx, vy, vz, p, m_ion)) {
               ncol += 1;

@@ -575,11 +575,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     }
     if constexpr (kFlat) {
       // one partner try per pass for every lane whose collision is due,
-      // once coll_batch of them are parked (or no lane can step)
+      // once coll_batch of them are parked, or once fewer than coll_batch
+      // lanes can step (at a window's end: waiting for those few would
+      // only delay the parked lanes' own remaining steps)
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
-        if (__popcll(parked) >= coll_batch || moving == 0) {
+        if (__popcll(parked) >= coll_batch || __popcll(moving) < coll_batch) {
           if (stage == kCollide) {
             const int rc = hs_try_flat(rng, hst, hz, vx, vy, vz, p, m_ion);
             if (rc > 0) {
@@ -596,7 +598,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
-        if (__popcll(parked) >= coll_batch || moving == 0) {
+        if (__popcll(parked) >= coll_batch || __popcll(moving) < coll_batch) {
           if (stage == kCollide) {
             if (hs_scatter(rng, vx, vy, vz, p, m_ion)) {
               ncol += 1;

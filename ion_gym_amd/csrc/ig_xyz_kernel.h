@@ -602,7 +602,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
-        if (__popcll(parked) >= coll_batch || moving == 0) {
+        if (__popcll(parked) >= coll_batch || __popcll(moving) < coll_batch) {
           if (stage == kCollide) {
             double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
             const int rc = hs_try_flat(rng, hst, hz, sx, sy, sz, p, m_amu);
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
       const uint64_t parked = __ballot(stage == kCollide);
       if (parked) {
         const uint64_t moving = __ballot(stage == kStep || stage == kPost);
-        if (__popcll(parked) >= coll_batch || moving == 0) {
+        if (__popcll(parked) >= coll_batch || __popcll(moving) < coll_batch) {
           if (stage == kCollide) {
             double sx = vx * 1e-3, sy = vy * 1e-3, sz = vz * 1e-3;
             if (hs_scatter(rng, sx, sy, sz, p, m_amu)) {
