@@ -741,9 +741,10 @@ def main():
         counter_probe_main(args)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} "
-              f"is not supported (ions shard across GPUs with a replicated "
-              f"field pack; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not "
+              f"supported (across GPUs the ions would shard, one process per "
+              f"GPU, each with its own copy of the field pack)",
+              file=sys.stderr)
         sys.exit(2)
 
     from ion_gym_amd import configs
