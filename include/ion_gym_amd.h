@@ -224,8 +224,9 @@ typedef struct {
                           precision, route and gas; results do not depend
                           on it */
   int32_t block;       /* lanes per workgroup: 64, 128, 192 or 256
-                          (0 = engine default 256); results do not depend
-                          on it */
+                          (0 = the engine's choice: 256, or 64 for an
+                          ensemble of fewer than eight 64-lane waves per
+                          compute unit); results do not depend on it */
   int32_t compact;     /* 1: gather the ensemble's live ions into a dense
                           index list (one wave ballot each) before the
                           window, so dead and empty slots take no lanes;

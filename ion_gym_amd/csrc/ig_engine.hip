@@ -282,11 +282,23 @@ struct ig_engine {
   std::map<int64_t, std::unique_ptr<Ensemble>> ens;
   std::map<int64_t, std::unique_ptr<Bases>> bases;
   int block = 256;
+  int cus = 256; // compute units of the device
   std::vector<GraphEntry> graphs; // captured window loops (ig_fly_graph)
   DevBuf bw, bj, bflag, btab, bout; // births: words, jumps, status, tables, rows
 };
 
 namespace {
+// lanes per workgroup of a flight launch: the caller's, else the engine's
+// (256) for an ensemble of at least eight 64-lane waves per compute unit,
+// and 64 below that, so that a small ensemble's waves spread evenly over
+// every compute unit instead of filling some with whole 256-lane
+// workgroups (a 100,000-ion 3-D FP64 window: 49 ms against 57); results
+// do not depend on it
+int launch_block(const ig_engine *h, const ig_flight_params *p, int64_t n) {
+  if (p->block > 0) return p->block;
+  return (n + 63) / 64 < 8 * (int64_t)h->cus ? 64 : h->block;
+}
+
 void drop_graphs(ig_engine *h) {
   for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
   h->graphs.clear();
@@ -346,6 +358,13 @@ int ig_engine_create(int device, ig_engine **out) {
   std::unique_ptr<ig_engine> h(new ig_engine());
   h->device = device;
   HIPX(hipSetDevice(device));
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
+            hipSuccess &&
+        cus > 0)
+      h->cus = cus;
+  }
   noalloc_setup(device); // an error is kept and reported by ig_fly_noalloc
   HIPX(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&h->ev0));
@@ -1603,7 +1622,7 @@ int ig_fly(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_ensembl
   HIPX(hipEventRecord(h->ev0, h->stream));
   EnsK ek;
   if (int rc = window_ens(h, E, *f, p, ek)) return rc;
-  int rc = launch_route(*f, k, ek, E->n, p->block > 0 ? p->block : h->block, h->stream);
+  int rc = launch_route(*f, k, ek, E->n, launch_block(h, p, E->n), h->stream);
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
   if (rc) return launch_error(rc, "ig_fly");
@@ -1640,7 +1659,7 @@ int ig_fly_noalloc(ig_engine *h, ig_fields_id fid, const ig_flight_params *p,
   HIPX(hipEventRecord(h->ev0, h->stream));
   EnsK ek;
   if (int rc = window_ens(h, E, *f, p, ek, &b)) return rc;
-  int rc = launch_route(*f, k, ek, E->n, p->block > 0 ? p->block : h->block, h->stream);
+  int rc = launch_route(*f, k, ek, E->n, launch_block(h, p, E->n), h->stream);
   HIPX(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
   if (rc) return launch_error(rc, "ig_fly_noalloc");
@@ -1655,7 +1674,7 @@ int ig_fly_graph(ig_engine *h, ig_fields_id fid, const ig_flight_params *p, ig_e
   FlightK k;
   Ensemble *E = nullptr;
   if (int rc = prepare_flight(h, fid, p, eid, "ig_fly_graph", f, k, E)) return rc;
-  const int block = p->block > 0 ? p->block : h->block;
+  const int block = launch_block(h, p, E->n);
   if (compact_mode(p, *f))
     if (int rc = live_bufs(E, compact_mode(p, *f) == 2)) return rc;
   // the graph is a function of the exact kernel arguments and launch shape
