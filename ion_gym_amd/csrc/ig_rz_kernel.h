@@ -248,7 +248,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
   const Tune *const Tu = row0 >= 0 ? e.tunes + row0 : nullptr;
   const Waves *const Wu = e.tunes ? (row0 >= 0 ? &e.tunes[row0].w : nullptr) : f.waves;
   const int64_t nx = f.n0, nu = f.n1;
-  const int K = f.K;
   const bool gate = f.has_gate && f.tau_gate >= 0.0;
   int64_t ncol = e.ncol[i];
   const int64_t steps0 = e.steps[i];
